@@ -1,0 +1,262 @@
+// extern "C" boundary of libhsg (declared in include/hsg.h). Every entry point catches C++
+// exceptions and returns an error code; the message is kept per thread.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hsg.h"
+#include "host/analysis.hpp"
+#include "kernels/engine.hpp"
+#include "kernels/krylov.hpp"
+
+struct hsg_factor {
+  std::unique_ptr<hsg::Engine> engine;
+  hsg_analysis_info info{};
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct InvalidInput : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return HSG_OK;
+  } catch (const InvalidInput& e) {
+    return fail(HSG_INVALID_INPUT, e.what());
+  } catch (const std::bad_alloc& e) {
+    return fail(HSG_OUT_OF_MEMORY_BUDGET, std::string("allocation failed: ") + e.what());
+  } catch (const std::exception& e) {
+    const std::string m = e.what();
+    if (m.find("hipMalloc") != std::string::npos && m.find("out of memory") != std::string::npos)
+      return fail(HSG_OUT_OF_MEMORY_BUDGET, m);
+    return fail(HSG_INTERNAL, m);
+  } catch (...) {
+    return fail(HSG_INTERNAL, "unknown exception");
+  }
+}
+
+// One scratch arena for the reduction partials of the Krylov kernels, per process. Calls that
+// use it are serialized by its mutex; the kernels themselves are stream-ordered.
+std::mutex g_scratch_mu;
+hsg::Scratch& scratch() {
+  static hsg::Scratch s;
+  return s;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+void require(bool ok, const char* msg) {
+  if (!ok) throw InvalidInput(msg);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hsg_version(void) { return "0.1.0"; }
+
+const char* hsg_last_error_message(void) { return g_last_error.c_str(); }
+
+int hsg_analyze(int64_t n, const int64_t* rowptr, const int32_t* colind, const int64_t* a_index,
+                const int64_t* b_index, const int32_t* order, const hsg_analysis_options* opt, hsg_factor** out) {
+  return guarded([&] {
+    require(out != nullptr, "hsg_analyze: out is NULL");
+    *out = nullptr;
+    require(n > 0 && n < (int64_t(1) << 31), "hsg_analyze: n must be in [1, 2^31)");
+    require(rowptr && colind && a_index && b_index, "hsg_analyze: NULL pattern array");
+    const int64_t nnz = rowptr[n];
+    require(rowptr[0] == 0 && nnz >= 0, "hsg_analyze: invalid rowptr");
+    for (int64_t i = 0; i < n; ++i) require(rowptr[i + 1] >= rowptr[i], "hsg_analyze: rowptr not monotone");
+    for (int64_t e = 0; e < nnz; ++e) require(colind[e] >= 0 && colind[e] < n, "hsg_analyze: column index out of range");
+    hsg::OrderingOptions oo;
+    hsg::AmalgamationOptions ao;
+    hsg::EngineOptions eo;
+    if (opt) {
+      if (opt->leaf_size > 0) oo.leaf_size = opt->leaf_size;
+      if (opt->panel_width > 0) eo.panel_width = opt->panel_width;
+      if (opt->small_front_max > 0) eo.small_front_max = opt->small_front_max;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    hsg::Symbolic sym = hsg::analyze_pattern(static_cast<int32_t>(n), rowptr, colind, oo, ao, order);
+    std::vector<int64_t> dst = hsg::assembly_map(sym, static_cast<int32_t>(n), rowptr, colind);
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<int64_t> ia(a_index, a_index + nnz), ib(b_index, b_index + nnz);
+    auto f = std::make_unique<hsg_factor>();
+    hsg_analysis_info& in = f->info;
+    in.n = n;
+    in.nnz = nnz;
+    in.supernodes = sym.nsn;
+    in.levels = sym.nlevels;
+    in.max_front = sym.max_front;
+    in.max_pivots = sym.max_ncol;
+    in.factor_entries = sym.factor_entries;
+    in.flops = sym.flops;
+    in.panel_bytes_double = 8.0 * static_cast<double>(sym.panel_off[sym.nsn]);
+    in.cb_bytes_double = 8.0 * static_cast<double>(sym.cb_off[sym.nsn]);
+    in.analyze_s = std::chrono::duration<double>(t1 - t0).count();
+    in.ordering_s = 0.0;
+    f->engine = std::make_unique<hsg::Engine>(std::move(sym), std::move(dst), std::move(ia), std::move(ib), eo);
+    in.front_rows = static_cast<int64_t>(f->engine->symbolic().rows.size());
+    in.small_fronts = f->engine->small_fronts();
+    in.large_fronts = f->engine->large_fronts();
+    *out = f.release();
+  });
+}
+
+int hsg_analysis_get_info(const hsg_factor* f, hsg_analysis_info* info) {
+  return guarded([&] {
+    require(f && info, "hsg_analysis_get_info: NULL argument");
+    *info = f->info;
+  });
+}
+
+int hsg_analysis_get_order(const hsg_factor* f, int32_t* order) {
+  return guarded([&] {
+    require(f && order, "hsg_analysis_get_order: NULL argument");
+    const auto& o = f->engine->symbolic().order;
+    std::memcpy(order, o.data(), o.size() * sizeof(int32_t));
+  });
+}
+
+int hsg_analysis_get_fronts(const hsg_factor* f, int32_t* first, int32_t* ncol, int32_t* m, int32_t* parent,
+                            int64_t* rowoff, int32_t* rows) {
+  return guarded([&] {
+    require(f && first && ncol && m && parent && rowoff && rows, "hsg_analysis_get_fronts: NULL argument");
+    const hsg::Symbolic& s = f->engine->symbolic();
+    std::memcpy(first, s.sn_first.data(), s.nsn * sizeof(int32_t));
+    std::memcpy(ncol, s.sn_ncol.data(), s.nsn * sizeof(int32_t));
+    std::memcpy(m, s.sn_m.data(), s.nsn * sizeof(int32_t));
+    std::memcpy(parent, s.sn_parent.data(), s.nsn * sizeof(int32_t));
+    std::memcpy(rowoff, s.sn_rowoff.data(), (s.nsn + 1) * sizeof(int64_t));
+    std::memcpy(rows, s.rows.data(), s.rows.size() * sizeof(int32_t));
+  });
+}
+
+int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_values, const double* b_values,
+                  int32_t precision, double null_pivot_tol, void* stream, hsg_inertia* inertia, int32_t* singular) {
+  return guarded([&] {
+    require(f && inertia, "hsg_factorize: NULL argument");
+    require(precision == HSG_DOUBLE || precision == HSG_SINGLE, "hsg_factorize: precision must be 0 or 1");
+    require(null_pivot_tol >= 0.0 && null_pivot_tol < 1.0, "hsg_factorize: null_pivot_tol must be in [0, 1)");
+    require(a_values != nullptr, "hsg_factorize: a_values is NULL");
+    const hsg::InertiaResult r =
+        f->engine->factorize(alpha, beta, a_values, b_values, precision, null_pivot_tol, as_stream(stream));
+    inertia->neg = r.neg;
+    inertia->zero = r.zero;
+    inertia->pos = r.pos;
+    inertia->valid = r.singular ? 0 : 1;
+    inertia->two_by_two = static_cast<int32_t>(r.two);
+    if (singular) *singular = r.singular ? 1 : 0;
+  });
+}
+
+int hsg_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream) {
+  return guarded([&] {
+    require(f && x, "hsg_solve: NULL argument");
+    require(nrhs >= 0, "hsg_solve: nrhs < 0");
+    require(ldx >= f->info.n, "hsg_solve: ldx < n");
+    if (!f->engine->has_factor()) throw std::runtime_error("hsg_solve: no usable factorization");
+    f->engine->solve(x, nrhs, ldx, as_stream(stream));
+  });
+}
+
+int hsg_release_factors(hsg_factor* f) {
+  return guarded([&] {
+    require(f != nullptr, "hsg_release_factors: NULL argument");
+    f->engine->release();
+  });
+}
+
+void hsg_factor_destroy(hsg_factor* f) { delete f; }
+
+int hsg_csr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values, const double* x,
+                 int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_csr_spmm: invalid sizes");
+    hsg::csr_spmm(n, rowptr, colind, values, x, ldx, y, ldy, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_diag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                  double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_diag_spmm: invalid sizes");
+    hsg::diag_spmm(n, d, x, ldx, y, ldy, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_shifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_colind, const double* a_values,
+                         const int64_t* b_rowptr, const int32_t* b_colind, const double* b_values,
+                         const double* b_diag, double sigma, const double* x, int64_t ldx, const double* y,
+                         int64_t ldy, double* r, int64_t ldr, int64_t p, double* rnorm2, double* omega,
+                         void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && ldr >= n && p >= 0, "hsg_shifted_residual: invalid sizes");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    hsg::shifted_residual(scratch(), n, a_rowptr, a_colind, a_values, b_rowptr, b_colind, b_values, b_diag, sigma, x,
+                          ldx, y, ldy, r, ldr, p, rnorm2, omega, as_stream(stream));
+  });
+}
+
+int hsg_gram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w, int64_t ldw,
+             int64_t p, double* c, int64_t ldc, void* stream) {
+  return guarded([&] {
+    require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_gram: invalid sizes");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    hsg::gram(scratch(), n, v, ldv, k, w_diag, w, ldw, p, c, ldc, as_stream(stream));
+  });
+}
+
+int hsg_block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                     int64_t ldw, int64_t p, double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_block_update: invalid sizes");
+    hsg::block_update(n, v, ldv, k, c, ldc, w, ldw, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                    const double* w_diag, double* out, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_column_dots: invalid sizes");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    hsg::column_dots(scratch(), n, x, ldx, y, ldy, p, w_diag, out, as_stream(stream));
+  });
+}
+
+int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
+                  const double* lambda, double* out, void* stream) {
+  return guarded([&] {
+    require(ld >= n && p >= 0, "hsg_pair_sums: invalid sizes");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    hsg::pair_sums(scratch(), n, ax, bx, x, ld, p, lambda, out, as_stream(stream));
+  });
+}
+
+int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
+              void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_axpby: invalid sizes");
+    hsg::axpby(n, a, x, ldx, b, y, ldy, p, as_stream(stream));
+  });
+}
+
+}  // extern "C"

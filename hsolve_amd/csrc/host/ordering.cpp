@@ -1,0 +1,532 @@
+// Nested-dissection fill-reducing ordering (host).
+//
+// Plays the role of hsolve's own nested dissection (src/factor/ordering.cpp, contract
+// include/hsolve/factor.hpp:197-246, 598-633: multilevel separators, refinement, minimum-degree
+// leaves), written independently: every subgraph larger than a leaf is bisected by a multilevel
+// edge partition (heavy-edge matching coarsening, greedy graph growing on the coarsest graph,
+// Fiduccia-Mattheyses boundary refinement while uncoarsening); the cut edges' boundary on the
+// smaller side becomes the vertex separator, which is then made minimal. Separators are numbered
+// last; leaves get a minimum-degree order computed on bit-set elimination graphs. The result is
+// deterministic (a fixed-seed generator drives the matching order).
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <queue>
+#include <stdexcept>
+#include <utility>
+#include <vector>
+
+#include "analysis.hpp"
+
+namespace hsg {
+
+Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind) {
+  // Count both directions, then sort/unique each row.
+  std::vector<int64_t> deg(static_cast<size_t>(n) + 1, 0);
+  for (int32_t i = 0; i < n; ++i)
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int32_t j = colind[p];
+      if (j == i) continue;
+      if (j < 0 || j >= n) throw std::runtime_error("pattern column index out of range");
+      ++deg[i + 1];
+      ++deg[j + 1];
+    }
+  for (int32_t i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<int32_t> tmp(static_cast<size_t>(deg[n]));
+  std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+  for (int32_t i = 0; i < n; ++i)
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int32_t j = colind[p];
+      if (j == i) continue;
+      tmp[fill[i]++] = j;
+      tmp[fill[j]++] = i;
+    }
+  Graph g;
+  g.n = n;
+  g.ptr.assign(static_cast<size_t>(n) + 1, 0);
+  g.adj.reserve(tmp.size() / 2 + 16);
+  for (int32_t i = 0; i < n; ++i) {
+    auto b = tmp.begin() + deg[i], e = tmp.begin() + deg[i + 1];
+    std::sort(b, e);
+    auto last = std::unique(b, e);
+    g.adj.insert(g.adj.end(), b, last);
+    g.ptr[i + 1] = static_cast<int64_t>(g.adj.size());
+  }
+  return g;
+}
+
+namespace {
+
+// Small deterministic generator (splitmix64) for tie-breaking orders.
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+};
+
+// Weighted graph of one multilevel stage.
+struct WGraph {
+  int32_t n = 0;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> adj;
+  std::vector<int32_t> ew;  // edge weights
+  std::vector<int32_t> vw;  // vertex weights
+  int64_t total_vw = 0;
+};
+
+// Heavy-edge matching: returns coarse graph and the fine->coarse map.
+bool coarsen(const WGraph& g, WGraph& c, std::vector<int32_t>& cmap, Rng& rng) {
+  const int32_t n = g.n;
+  std::vector<int32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  for (int32_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[rng.next() % (uint64_t)(i + 1)]);
+  std::vector<int32_t> match(n, -1);
+  const int32_t max_vw = static_cast<int32_t>(std::max<int64_t>(1, g.total_vw / 20));
+  for (int32_t v : perm) {
+    if (match[v] >= 0) continue;
+    int32_t best = -1, bw = -1;
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+      const int32_t u = g.adj[p];
+      if (match[u] >= 0 || u == v) continue;
+      if (g.vw[u] + g.vw[v] > max_vw) continue;
+      if (g.ew[p] > bw) {
+        bw = g.ew[p];
+        best = u;
+      }
+    }
+    if (best >= 0) {
+      match[v] = best;
+      match[best] = v;
+    } else {
+      match[v] = v;
+    }
+  }
+  cmap.assign(n, -1);
+  int32_t nc = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    if (cmap[v] >= 0) continue;
+    cmap[v] = nc;
+    if (match[v] != v) cmap[match[v]] = nc;
+    ++nc;
+  }
+  if (nc > static_cast<int32_t>(0.92 * n)) return false;  // too little reduction
+  c.n = nc;
+  c.vw.assign(nc, 0);
+  for (int32_t v = 0; v < n; ++v) c.vw[cmap[v]] += g.vw[v];
+  c.total_vw = g.total_vw;
+  c.ptr.assign(static_cast<size_t>(nc) + 1, 0);
+  c.adj.clear();
+  c.ew.clear();
+  c.adj.reserve(g.adj.size() / 2);
+  c.ew.reserve(g.adj.size() / 2);
+  std::vector<int32_t> slot(nc, -1);  // position of coarse neighbour in the current row
+  // Visit coarse vertices in order; the fine members are v and match[v].
+  std::vector<int32_t> first(nc, -1), second(nc, -1);
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t cv = cmap[v];
+    if (first[cv] < 0) first[cv] = v;
+    else second[cv] = v;
+  }
+  for (int32_t cv = 0; cv < nc; ++cv) {
+    const int64_t row_start = static_cast<int64_t>(c.adj.size());
+    for (int k = 0; k < 2; ++k) {
+      const int32_t v = k == 0 ? first[cv] : second[cv];
+      if (v < 0) continue;
+      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+        const int32_t cu = cmap[g.adj[p]];
+        if (cu == cv) continue;
+        if (slot[cu] >= 0) {
+          c.ew[slot[cu]] += g.ew[p];
+        } else {
+          slot[cu] = static_cast<int32_t>(c.adj.size());
+          c.adj.push_back(cu);
+          c.ew.push_back(g.ew[p]);
+        }
+      }
+    }
+    for (int64_t p = row_start; p < static_cast<int64_t>(c.adj.size()); ++p) slot[c.adj[p]] = -1;
+    c.ptr[cv + 1] = static_cast<int64_t>(c.adj.size());
+  }
+  return true;
+}
+
+int64_t cut_of(const WGraph& g, const std::vector<int8_t>& part) {
+  int64_t cut = 0;
+  for (int32_t v = 0; v < g.n; ++v)
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p)
+      if (part[v] != part[g.adj[p]]) cut += g.ew[p];
+  return cut / 2;
+}
+
+// Fiduccia-Mattheyses style refinement of a 2-way edge partition with a balance bound; several
+// passes, each moving vertices by best gain with rollback to the best prefix.
+void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int passes) {
+  const int32_t n = g.n;
+  const int64_t maxw = static_cast<int64_t>(max_frac * static_cast<double>(g.total_vw)) + 1;
+  std::vector<int64_t> pw(2, 0);
+  for (int32_t v = 0; v < n; ++v) pw[part[v]] += g.vw[v];
+  std::vector<int32_t> gain(n), ext(n);
+  std::vector<char> moved(n);
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int32_t v = 0; v < n; ++v) {
+      int32_t e = 0, i = 0;
+      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) (part[g.adj[p]] != part[v] ? e : i) += g.ew[p];
+      gain[v] = e - i;
+      ext[v] = e;
+    }
+    std::fill(moved.begin(), moved.end(), 0);
+    using Item = std::pair<int32_t, int32_t>;  // (gain, -vertex) max-heap
+    std::priority_queue<Item> heap[2];
+    for (int32_t v = 0; v < n; ++v)
+      if (ext[v] > 0) heap[part[v]].push({gain[v], -v});
+    std::vector<int32_t> seq;
+    int64_t cur = 0, best = 0;
+    size_t best_len = 0;
+    const size_t limit = static_cast<size_t>(std::max<int32_t>(50, n / 10));
+    while (seq.size() < limit) {
+      // pick the side to move from: the heavier side first if it has candidates
+      int from = pw[0] >= pw[1] ? 0 : 1;
+      int32_t v = -1;
+      for (int attempt = 0; attempt < 2 && v < 0; ++attempt) {
+        auto& h = heap[from];
+        while (!h.empty()) {
+          const auto [gv, nv] = h.top();
+          const int32_t cand = -nv;
+          if (moved[cand] || part[cand] != from || gv != gain[cand]) {
+            h.pop();
+            continue;
+          }
+          if (pw[1 - from] + g.vw[cand] > maxw) break;
+          v = cand;
+          h.pop();
+          break;
+        }
+        if (v < 0) from = 1 - from;
+      }
+      if (v < 0) break;
+      const int to = 1 - part[v];
+      cur -= gain[v];
+      pw[part[v]] -= g.vw[v];
+      pw[to] += g.vw[v];
+      part[v] = static_cast<int8_t>(to);
+      moved[v] = 1;
+      seq.push_back(v);
+      gain[v] = -gain[v];
+      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+        const int32_t u = g.adj[p];
+        const int32_t w = g.ew[p];
+        if (part[u] == to) {
+          gain[u] -= 2 * w;
+        } else {
+          gain[u] += 2 * w;
+        }
+        if (!moved[u]) heap[part[u]].push({gain[u], -u});
+      }
+      if (cur < best) {
+        best = cur;
+        best_len = seq.size();
+      }
+    }
+    // roll back beyond the best prefix
+    for (size_t k = seq.size(); k > best_len; --k) {
+      const int32_t v = seq[k - 1];
+      pw[part[v]] -= g.vw[v];
+      part[v] = static_cast<int8_t>(1 - part[v]);
+      pw[part[v]] += g.vw[v];
+    }
+    if (best_len == 0) break;
+  }
+}
+
+// Greedy graph growing from seed: BFS adding vertices to part 0 until half the weight.
+void grow_from(const WGraph& g, int32_t seed, std::vector<int8_t>& part) {
+  part.assign(g.n, 1);
+  int64_t w = 0;
+  const int64_t half = g.total_vw / 2;
+  std::vector<int32_t> queue;
+  queue.reserve(g.n);
+  std::vector<char> seen(g.n, 0);
+  queue.push_back(seed);
+  seen[seed] = 1;
+  size_t head = 0;
+  while (w < half) {
+    if (head == queue.size()) {  // disconnected: take any unseen vertex
+      int32_t v = -1;
+      for (int32_t u = 0; u < g.n; ++u)
+        if (!seen[u]) {
+          v = u;
+          break;
+        }
+      if (v < 0) break;
+      seen[v] = 1;
+      queue.push_back(v);
+    }
+    const int32_t v = queue[head++];
+    part[v] = 0;
+    w += g.vw[v];
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+      const int32_t u = g.adj[p];
+      if (!seen[u]) {
+        seen[u] = 1;
+        queue.push_back(u);
+      }
+    }
+  }
+}
+
+int32_t bfs_farthest(const WGraph& g, int32_t s) {
+  std::vector<int32_t> dist(g.n, -1);
+  std::vector<int32_t> q{s};
+  dist[s] = 0;
+  int32_t last = s;
+  for (size_t h = 0; h < q.size(); ++h) {
+    const int32_t v = q[h];
+    last = v;
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p)
+      if (dist[g.adj[p]] < 0) {
+        dist[g.adj[p]] = dist[v] + 1;
+        q.push_back(g.adj[p]);
+      }
+  }
+  return last;
+}
+
+// Multilevel 2-way edge partition of g; part[v] in {0,1}.
+void multilevel_bisect(const WGraph& g0, std::vector<int8_t>& part, Rng& rng) {
+  std::vector<WGraph> levels;
+  std::vector<std::vector<int32_t>> maps;
+  levels.push_back(g0);
+  while (levels.back().n > 160) {
+    WGraph c;
+    std::vector<int32_t> cmap;
+    if (!coarsen(levels.back(), c, cmap, rng)) break;
+    levels.push_back(std::move(c));
+    maps.push_back(std::move(cmap));
+  }
+  const WGraph& gc = levels.back();
+  // initial partitions: grown from a few seeds (peripheral and random), keep the best cut
+  std::vector<int8_t> best;
+  int64_t best_cut = -1;
+  std::vector<int32_t> seeds;
+  seeds.push_back(bfs_farthest(gc, 0));
+  seeds.push_back(bfs_farthest(gc, seeds[0]));
+  for (int t = 0; t < 4 && gc.n > 0; ++t) seeds.push_back(static_cast<int32_t>(rng.next() % (uint64_t)gc.n));
+  for (int32_t s : seeds) {
+    std::vector<int8_t> p;
+    grow_from(gc, s, p);
+    fm_refine(gc, p, 0.55, 6);
+    const int64_t c = cut_of(gc, p);
+    if (best_cut < 0 || c < best_cut) {
+      best_cut = c;
+      best = p;
+    }
+  }
+  part = std::move(best);
+  for (size_t l = levels.size() - 1; l > 0; --l) {
+    const WGraph& fine = levels[l - 1];
+    const std::vector<int32_t>& cmap = maps[l - 1];
+    std::vector<int8_t> fp(fine.n);
+    for (int32_t v = 0; v < fine.n; ++v) fp[v] = part[cmap[v]];
+    fm_refine(fine, fp, 0.55, 4);
+    part = std::move(fp);
+  }
+}
+
+// Minimum-degree order of a small subgraph by explicit elimination on bit sets.
+void min_degree_leaf(const Graph& g, const std::vector<int32_t>& verts, const std::vector<int32_t>& local,
+                     std::vector<int32_t>& out) {
+  const int32_t L = static_cast<int32_t>(verts.size());
+  const int32_t W = (L + 63) / 64;
+  std::vector<uint64_t> bits(static_cast<size_t>(L) * W, 0);
+  for (int32_t a = 0; a < L; ++a) {
+    const int32_t v = verts[a];
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+      const int32_t b = local[g.adj[p]];
+      if (b >= 0 && b != a) bits[static_cast<size_t>(a) * W + b / 64] |= 1ull << (b % 64);
+    }
+  }
+  std::vector<char> gone(L, 0);
+  for (int32_t step = 0; step < L; ++step) {
+    int32_t best = -1, bd = 1 << 30;
+    for (int32_t a = 0; a < L; ++a) {
+      if (gone[a]) continue;
+      int32_t d = 0;
+      for (int32_t w = 0; w < W; ++w) d += __builtin_popcountll(bits[static_cast<size_t>(a) * W + w]);
+      if (d < bd) {
+        bd = d;
+        best = a;
+      }
+    }
+    gone[best] = 1;
+    out.push_back(verts[best]);
+    uint64_t* rb = &bits[static_cast<size_t>(best) * W];
+    for (int32_t b = 0; b < L; ++b) {
+      if (!(rb[b / 64] >> (b % 64) & 1ull)) continue;
+      uint64_t* r = &bits[static_cast<size_t>(b) * W];
+      for (int32_t w = 0; w < W; ++w) r[w] |= rb[w];
+      r[b / 64] &= ~(1ull << (b % 64));
+      r[best / 64] &= ~(1ull << (best % 64));
+    }
+    for (int32_t w = 0; w < W; ++w) rb[w] = 0;
+  }
+}
+
+}  // namespace
+
+std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt) {
+  const int32_t n = g.n;
+  std::vector<int32_t> perm(n, -1);
+  std::vector<int32_t> label(n, 0);   // subgraph id of each unnumbered vertex
+  std::vector<int32_t> local(n, -1);  // local index scratch
+  Rng rng(0x5EED2026ull);
+  struct Task {
+    std::vector<int32_t> verts;
+    int32_t lo, hi;  // numbers [lo, hi) are assigned to these vertices
+    int32_t id;
+  };
+  int32_t next_id = 1;
+  std::vector<Task> stack;
+  {
+    Task t;
+    t.verts.resize(n);
+    std::iota(t.verts.begin(), t.verts.end(), 0);
+    t.lo = 0;
+    t.hi = n;
+    t.id = 0;
+    stack.push_back(std::move(t));
+  }
+  while (!stack.empty()) {
+    Task t = std::move(stack.back());
+    stack.pop_back();
+    const int32_t nv = static_cast<int32_t>(t.verts.size());
+    if (nv == 0) continue;
+    for (int32_t v : t.verts) label[v] = t.id;
+    if (nv <= opt.leaf_size) {
+      std::vector<int32_t> ord;
+      ord.reserve(nv);
+      if (opt.min_degree_leaves) {
+        for (int32_t a = 0; a < nv; ++a) local[t.verts[a]] = a;
+        min_degree_leaf(g, t.verts, local, ord);
+        for (int32_t v : t.verts) local[v] = -1;
+      } else {
+        ord = t.verts;
+      }
+      for (int32_t a = 0; a < nv; ++a) perm[t.lo + a] = ord[a];
+      continue;
+    }
+    // local weighted graph of the subgraph
+    for (int32_t a = 0; a < nv; ++a) local[t.verts[a]] = a;
+    WGraph wg;
+    wg.n = nv;
+    wg.ptr.assign(static_cast<size_t>(nv) + 1, 0);
+    for (int32_t a = 0; a < nv; ++a) {
+      const int32_t v = t.verts[a];
+      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+        const int32_t u = g.adj[p];
+        if (label[u] == t.id && local[u] >= 0) wg.adj.push_back(local[u]);
+      }
+      wg.ptr[a + 1] = static_cast<int64_t>(wg.adj.size());
+    }
+    wg.ew.assign(wg.adj.size(), 1);
+    wg.vw.assign(nv, 1);
+    wg.total_vw = nv;
+    // connected components: dissect each separately (no separator needed between them)
+    std::vector<int32_t> comp(nv, -1);
+    int32_t ncomp = 0;
+    {
+      std::vector<int32_t> q;
+      for (int32_t s = 0; s < nv; ++s) {
+        if (comp[s] >= 0) continue;
+        q.assign(1, s);
+        comp[s] = ncomp;
+        for (size_t h = 0; h < q.size(); ++h)
+          for (int64_t p = wg.ptr[q[h]]; p < wg.ptr[q[h] + 1]; ++p)
+            if (comp[wg.adj[p]] < 0) {
+              comp[wg.adj[p]] = ncomp;
+              q.push_back(wg.adj[p]);
+            }
+        ++ncomp;
+      }
+    }
+    if (ncomp > 1) {
+      std::vector<std::vector<int32_t>> parts(ncomp);
+      for (int32_t a = 0; a < nv; ++a) parts[comp[a]].push_back(t.verts[a]);
+      for (int32_t v : t.verts) local[v] = -1;
+      int32_t lo = t.lo;
+      for (auto& pv : parts) {
+        Task c;
+        c.lo = lo;
+        c.hi = lo + static_cast<int32_t>(pv.size());
+        lo = c.hi;
+        c.verts = std::move(pv);
+        c.id = next_id++;
+        stack.push_back(std::move(c));
+      }
+      continue;
+    }
+    std::vector<int8_t> part;
+    multilevel_bisect(wg, part, rng);
+    // vertex separator: boundary vertices of the side with fewer of them
+    std::vector<int32_t> bnd[2];
+    for (int32_t a = 0; a < nv; ++a) {
+      for (int64_t p = wg.ptr[a]; p < wg.ptr[a + 1]; ++p)
+        if (part[wg.adj[p]] != part[a]) {
+          bnd[part[a]].push_back(a);
+          break;
+        }
+    }
+    const int side = bnd[0].size() <= bnd[1].size() ? 0 : 1;
+    std::vector<int8_t> where(nv);  // 0, 1 or 2 (separator)
+    for (int32_t a = 0; a < nv; ++a) where[a] = part[a];
+    for (int32_t a : bnd[side]) where[a] = 2;
+    // minimal separator: a separator vertex with no neighbour on the other side joins its side
+    for (int32_t a : bnd[side]) {
+      bool touches_other = false;
+      for (int64_t p = wg.ptr[a]; p < wg.ptr[a + 1] && !touches_other; ++p)
+        touches_other = where[wg.adj[p]] == 1 - side;
+      if (!touches_other) where[a] = static_cast<int8_t>(side);
+    }
+    // and those touching only the other side (never the kept side) may move there
+    for (int32_t a : bnd[side]) {
+      if (where[a] != 2) continue;
+      bool touches_side = false;
+      for (int64_t p = wg.ptr[a]; p < wg.ptr[a + 1] && !touches_side; ++p)
+        touches_side = where[wg.adj[p]] == side;
+      if (!touches_side) where[a] = static_cast<int8_t>(1 - side);
+    }
+    std::vector<int32_t> p0, p1, sep;
+    for (int32_t a = 0; a < nv; ++a) {
+      const int32_t v = t.verts[a];
+      (where[a] == 0 ? p0 : where[a] == 1 ? p1 : sep).push_back(v);
+    }
+    for (int32_t v : t.verts) local[v] = -1;
+    if (p0.empty() || p1.empty()) {  // degenerate: order as a leaf
+      for (int32_t a = 0; a < nv; ++a) perm[t.lo + a] = t.verts[a];
+      continue;
+    }
+    const int32_t ns = static_cast<int32_t>(sep.size());
+    for (int32_t a = 0; a < ns; ++a) perm[t.hi - ns + a] = sep[a];
+    for (int32_t v : sep) label[v] = -1;
+    Task c0, c1;
+    c0.lo = t.lo;
+    c0.hi = t.lo + static_cast<int32_t>(p0.size());
+    c1.lo = c0.hi;
+    c1.hi = t.hi - ns;
+    c0.verts = std::move(p0);
+    c1.verts = std::move(p1);
+    c0.id = next_id++;
+    c1.id = next_id++;
+    stack.push_back(std::move(c0));
+    stack.push_back(std::move(c1));
+  }
+  for (int32_t i = 0; i < n; ++i)
+    if (perm[i] < 0) throw std::runtime_error("nested_dissection: incomplete ordering");
+  return perm;
+}
+
+}  // namespace hsg

@@ -1,0 +1,381 @@
+// Supernodal symbolic analysis for the multifrontal LDL^T (host).
+//
+// Replaces the symbolic half of MUMPS' analysis (JOB = 1; src/factor/mumps_factorization.cpp:188-283,
+// estimates read at :262-283) with an independent implementation:
+//   1. elimination tree of the permuted pattern (Liu's algorithm with path halving);
+//   2. postorder, composed into the ordering so that every subtree is numbered contiguously;
+//   3. column counts of L by the row-subtree skeleton method (Gilbert, Ng and Peyton, 1994);
+//   4. fundamental supernodes, then relaxed amalgamation along each node's last child;
+//   5. front row structures bottom-up (own columns, children's contribution rows, original
+//      entries), extend-add maps, storage offsets and the level schedule (height in the tree).
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#include "analysis.hpp"
+
+namespace hsg {
+
+namespace {
+
+// Lower-triangular pattern (rows > col) of P A P^T in CSC form, new numbering.
+void permuted_lower(int32_t n, const Graph& g, const std::vector<int32_t>& order,
+                    const std::vector<int32_t>& iorder, std::vector<int64_t>& cp, std::vector<int32_t>& ri) {
+  cp.assign(static_cast<size_t>(n) + 1, 0);
+  for (int32_t jn = 0; jn < n; ++jn) {
+    const int32_t j = order[jn];
+    for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p)
+      if (iorder[g.adj[p]] > jn) ++cp[jn + 1];
+  }
+  for (int32_t j = 0; j < n; ++j) cp[j + 1] += cp[j];
+  ri.resize(static_cast<size_t>(cp[n]));
+  std::vector<int64_t> fill(cp.begin(), cp.end() - 1);
+  for (int32_t jn = 0; jn < n; ++jn) {
+    const int32_t j = order[jn];
+    for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p) {
+      const int32_t in = iorder[g.adj[p]];
+      if (in > jn) ri[fill[jn]++] = in;
+    }
+    std::sort(ri.begin() + cp[jn], ri.begin() + cp[jn + 1]);
+  }
+}
+
+// Elimination tree from the lower CSC pattern: for each row i, walk from every column j < i with
+// L(i, j) structurally present up to its current root (path halving) and hang it below i.
+std::vector<int32_t> etree_lower(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& ri) {
+  // Row-wise access to the upper part (= columns j of row i with j < i): transpose.
+  std::vector<int64_t> rp(static_cast<size_t>(n) + 1, 0);
+  for (int64_t p = 0; p < cp[n]; ++p) ++rp[ri[p] + 1];
+  for (int32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+  std::vector<int32_t> cj(static_cast<size_t>(cp[n]));
+  {
+    std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+    for (int32_t j = 0; j < n; ++j)
+      for (int64_t p = cp[j]; p < cp[j + 1]; ++p) cj[fill[ri[p]]++] = j;
+  }
+  std::vector<int32_t> parent(n, -1), anc(n, -1);
+  for (int32_t i = 0; i < n; ++i) {
+    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+      int32_t j = cj[p];
+      while (j != -1 && j < i) {
+        const int32_t next = anc[j];
+        anc[j] = i;
+        if (next == -1) {
+          parent[j] = i;
+          break;
+        }
+        j = next;
+      }
+    }
+  }
+  return parent;
+}
+
+std::vector<int32_t> postorder(int32_t n, const std::vector<int32_t>& parent) {
+  std::vector<int32_t> head(n, -1), next(n, -1);
+  for (int32_t j = n - 1; j >= 0; --j) {
+    if (parent[j] < 0) continue;
+    next[j] = head[parent[j]];
+    head[parent[j]] = j;
+  }
+  std::vector<int32_t> post;
+  post.reserve(n);
+  std::vector<int32_t> stack;
+  for (int32_t r = 0; r < n; ++r) {
+    if (parent[r] != -1) continue;
+    stack.push_back(r);
+    while (!stack.empty()) {
+      const int32_t v = stack.back();
+      const int32_t c = head[v];
+      if (c == -1) {
+        post.push_back(v);
+        stack.pop_back();
+      } else {
+        head[v] = next[c];
+        stack.push_back(c);
+      }
+    }
+  }
+  return post;
+}
+
+// Column counts (incl. diagonal) of L for a postordered etree (post = identity here) by the
+// row-subtree skeleton method.
+std::vector<int64_t> column_counts(int32_t n, const std::vector<int32_t>& parent, const std::vector<int64_t>& cp,
+                                   const std::vector<int32_t>& ri) {
+  std::vector<int32_t> first(n, -1), maxfirst(n, -1), prevleaf(n, -1), anc(n);
+  std::vector<int64_t> delta(n, 0);
+  // first[j]: smallest postorder number in the subtree of j (identity postorder)
+  for (int32_t k = 0; k < n; ++k) {
+    int32_t j = k;
+    delta[j] = first[j] == -1 ? 1 : 0;
+    for (; j != -1 && first[j] == -1; j = parent[j]) first[j] = k;
+  }
+  std::iota(anc.begin(), anc.end(), 0);
+  for (int32_t j = 0; j < n; ++j) {
+    if (parent[j] != -1) --delta[parent[j]];
+    for (int64_t p = cp[j]; p < cp[j + 1]; ++p) {
+      const int32_t i = ri[p];  // i > j: entry (i, j) of the lower pattern
+      if (first[j] <= maxfirst[i]) continue;  // j is not a leaf of row subtree i
+      maxfirst[i] = first[j];
+      const int32_t jprev = prevleaf[i];
+      prevleaf[i] = j;
+      ++delta[j];
+      if (jprev != -1) {
+        int32_t q = jprev;
+        while (q != anc[q]) q = anc[q];
+        for (int32_t s = jprev; s != q;) {
+          const int32_t sp = anc[s];
+          anc[s] = q;
+          s = sp;
+        }
+        --delta[q];
+      }
+    }
+    if (parent[j] != -1) anc[j] = parent[j];
+  }
+  for (int32_t j = 0; j < n; ++j)
+    if (parent[j] != -1) delta[parent[j]] += delta[j];
+  return delta;
+}
+
+}  // namespace
+
+Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind, const OrderingOptions& oopt,
+                         const AmalgamationOptions& aopt, const int32_t* given_order) {
+  if (n <= 0) throw std::runtime_error("analyze_pattern: empty matrix");
+  Graph g = graph_from_csr(n, rowptr, colind);
+  std::vector<int32_t> order = given_order ? std::vector<int32_t>(given_order, given_order + n)
+                                           : nested_dissection(g, oopt);
+  std::vector<int32_t> iorder(n, -1);
+  for (int32_t k = 0; k < n; ++k) {
+    if (order[k] < 0 || order[k] >= n || iorder[order[k]] != -1)
+      throw std::runtime_error("analyze_pattern: invalid ordering");
+    iorder[order[k]] = k;
+  }
+  std::vector<int64_t> cp;
+  std::vector<int32_t> ri;
+  permuted_lower(n, g, order, iorder, cp, ri);
+  std::vector<int32_t> parent = etree_lower(n, cp, ri);
+  // postorder and compose
+  {
+    const std::vector<int32_t> post = postorder(n, parent);
+    std::vector<int32_t> order2(n);
+    for (int32_t k = 0; k < n; ++k) order2[k] = order[post[k]];
+    order.swap(order2);
+    for (int32_t k = 0; k < n; ++k) iorder[order[k]] = k;
+    permuted_lower(n, g, order, iorder, cp, ri);
+    parent = etree_lower(n, cp, ri);
+  }
+  const std::vector<int64_t> cc = column_counts(n, parent, cp, ri);
+  // number of children per column
+  std::vector<int32_t> nchild(n, 0);
+  for (int32_t j = 0; j < n; ++j)
+    if (parent[j] >= 0) ++nchild[parent[j]];
+  // fundamental supernodes
+  std::vector<int32_t> fs_first;
+  for (int32_t j = 0; j < n; ++j) {
+    const bool merge = j > 0 && parent[j - 1] == j && nchild[j] == 1 && cc[j - 1] == cc[j] + 1;
+    if (!merge) fs_first.push_back(j);
+  }
+  const int32_t nfs = static_cast<int32_t>(fs_first.size());
+  fs_first.push_back(n);
+  std::vector<int32_t> col_fs(n);
+  for (int32_t s = 0; s < nfs; ++s)
+    for (int32_t j = fs_first[s]; j < fs_first[s + 1]; ++j) col_fs[j] = s;
+  // supernode tree and true entries
+  std::vector<int32_t> fs_parent(nfs, -1);
+  std::vector<int64_t> fs_m(nfs), fs_entries(nfs, 0);
+  std::vector<int32_t> fs_ncol(nfs);
+  for (int32_t s = 0; s < nfs; ++s) {
+    const int32_t last = fs_first[s + 1] - 1;
+    fs_parent[s] = parent[last] >= 0 ? col_fs[parent[last]] : -1;
+    fs_ncol[s] = fs_first[s + 1] - fs_first[s];
+    fs_m[s] = cc[fs_first[s]];
+    for (int32_t j = fs_first[s]; j < fs_first[s + 1]; ++j) fs_entries[s] += cc[j];
+  }
+  // relaxed amalgamation: supernode s absorbs its last child c (c ends right before s starts)
+  // while the merged front stays within the relaxation rule. Merged nodes are tracked by
+  // union-find on the fundamental supernodes; columns stay contiguous.
+  std::vector<int32_t> rep(nfs);
+  std::iota(rep.begin(), rep.end(), 0);
+  std::vector<int32_t> m_first(fs_first.begin(), fs_first.end() - 1), m_ncol = fs_ncol;
+  std::vector<int64_t> m_m = fs_m, m_entries = fs_entries;
+  std::vector<char> absorbed(nfs, 0);
+  for (int32_t s = 0; s < nfs; ++s) {
+    for (;;) {
+      const int32_t f = m_first[s];
+      if (f == 0) break;
+      const int32_t c = rep[col_fs[f - 1]];
+      // c must be a child of s in the amalgamated tree
+      int32_t pc = fs_parent[c];
+      if (pc < 0) break;
+      while (rep[pc] != pc) pc = rep[pc];
+      if (pc != s) break;
+      const int64_t ncol = m_ncol[c] + m_ncol[s];
+      const int64_t mm = m_ncol[c] + m_m[s];
+      const double dense = static_cast<double>(ncol) * static_cast<double>(mm) -
+                           0.5 * static_cast<double>(ncol) * static_cast<double>(ncol - 1);
+      const double zeros = dense - static_cast<double>(m_entries[c] + m_entries[s]);
+      bool ok = false;
+      for (int r = 0; r < 3 && !ok; ++r)
+        ok = ncol <= aopt.ncol[r] && zeros <= aopt.zero_frac[r] * dense;
+      if (!ok) break;
+      // merge c into s
+      rep[c] = s;
+      absorbed[c] = 1;
+      m_first[s] = m_first[c];
+      m_ncol[s] = static_cast<int32_t>(ncol);
+      m_m[s] = mm;
+      m_entries[s] = m_entries[c] + m_entries[s];  // true entries; the zeros are re-derived
+    }
+  }
+  auto find = [&](int32_t x) {
+    while (rep[x] != x) x = rep[x];
+    return x;
+  };
+  Symbolic S;
+  S.n = n;
+  S.order = order;
+  S.iorder = iorder;
+  std::vector<int32_t> sn_of_fs(nfs, -1);
+  for (int32_t s = 0; s < nfs; ++s)
+    if (!absorbed[s]) {
+      sn_of_fs[s] = S.nsn++;
+      S.sn_first.push_back(m_first[s]);
+      S.sn_ncol.push_back(m_ncol[s]);
+    }
+  // supernodes are in increasing first-column order because fundamental ones are and every kept
+  // node keeps its position relative to the others (it only absorbs columns right before it)
+  S.col_sn.assign(n, -1);
+  for (int32_t s = 0; s < S.nsn; ++s)
+    for (int32_t j = S.sn_first[s]; j < S.sn_first[s] + S.sn_ncol[s]; ++j) S.col_sn[j] = s;
+  for (int32_t j = 0; j < n; ++j)
+    if (S.col_sn[j] < 0) throw std::runtime_error("analyze_pattern: column without supernode");
+  S.sn_parent.assign(S.nsn, -1);
+  for (int32_t s = 0; s < S.nsn; ++s) {
+    const int32_t last = S.sn_first[s] + S.sn_ncol[s] - 1;
+    S.sn_parent[s] = parent[last] >= 0 ? S.col_sn[parent[last]] : -1;
+    if (S.sn_parent[s] >= 0 && S.sn_parent[s] <= s) throw std::runtime_error("analyze_pattern: tree not postordered");
+  }
+  (void)find;
+  // children lists
+  S.child_ptr.assign(static_cast<size_t>(S.nsn) + 1, 0);
+  for (int32_t s = 0; s < S.nsn; ++s)
+    if (S.sn_parent[s] >= 0) ++S.child_ptr[S.sn_parent[s] + 1];
+  for (int32_t s = 0; s < S.nsn; ++s) S.child_ptr[s + 1] += S.child_ptr[s];
+  S.child_list.resize(S.child_ptr[S.nsn]);
+  {
+    std::vector<int32_t> fill(S.child_ptr.begin(), S.child_ptr.end() - 1);
+    for (int32_t s = 0; s < S.nsn; ++s)
+      if (S.sn_parent[s] >= 0) S.child_list[fill[S.sn_parent[s]]++] = s;
+  }
+  // row structures bottom-up
+  S.sn_rowoff.assign(static_cast<size_t>(S.nsn) + 1, 0);
+  S.sn_m.assign(S.nsn, 0);
+  std::vector<int32_t> mark(n, -1);
+  std::vector<int32_t> tmp;
+  S.rows.clear();
+  for (int32_t s = 0; s < S.nsn; ++s) {
+    const int32_t f = S.sn_first[s], l = f + S.sn_ncol[s];
+    tmp.clear();
+    for (int32_t j = f; j < l; ++j) {
+      mark[j] = s;
+      tmp.push_back(j);
+    }
+    for (int32_t j = f; j < l; ++j)
+      for (int64_t p = cp[j]; p < cp[j + 1]; ++p) {
+        const int32_t i = ri[p];
+        if (mark[i] != s) {
+          mark[i] = s;
+          tmp.push_back(i);
+        }
+      }
+    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
+      const int32_t c = S.child_list[q];
+      for (int64_t p = S.sn_rowoff[c] + S.sn_ncol[c]; p < S.sn_rowoff[c + 1]; ++p) {
+        const int32_t i = S.rows[p];
+        if (mark[i] != s) {
+          mark[i] = s;
+          tmp.push_back(i);
+        }
+      }
+    }
+    std::sort(tmp.begin() + (l - f), tmp.end());
+    for (size_t k = static_cast<size_t>(l - f); k < tmp.size(); ++k)
+      if (tmp[k] < l) throw std::runtime_error("analyze_pattern: row above the supernode's columns");
+    S.rows.insert(S.rows.end(), tmp.begin(), tmp.end());
+    S.sn_rowoff[s + 1] = static_cast<int64_t>(S.rows.size());
+    S.sn_m[s] = static_cast<int32_t>(tmp.size());
+  }
+  // extend-add maps: local index of each child CB row in the parent's front
+  S.relmap.assign(S.rows.size(), -1);
+  for (int32_t c = 0; c < S.nsn; ++c) {
+    const int32_t p = S.sn_parent[c];
+    if (p < 0) {
+      if (S.sn_m[c] != S.sn_ncol[c]) throw std::runtime_error("analyze_pattern: root with contribution rows");
+      continue;
+    }
+    const int32_t* prow = &S.rows[S.sn_rowoff[p]];
+    const int32_t pm = S.sn_m[p];
+    int32_t q = 0;
+    for (int64_t k = S.sn_rowoff[c] + S.sn_ncol[c]; k < S.sn_rowoff[c + 1]; ++k) {
+      const int32_t i = S.rows[k];
+      while (q < pm && prow[q] < i) ++q;
+      if (q >= pm || prow[q] != i) throw std::runtime_error("analyze_pattern: child row missing in parent");
+      S.relmap[k] = q;
+    }
+  }
+  // offsets, statistics
+  S.panel_off.assign(static_cast<size_t>(S.nsn) + 1, 0);
+  S.cb_off.assign(static_cast<size_t>(S.nsn) + 1, 0);
+  for (int32_t s = 0; s < S.nsn; ++s) {
+    const int64_t m = S.sn_m[s], nc = S.sn_ncol[s], cb = m - nc;
+    S.panel_off[s + 1] = S.panel_off[s] + m * nc;
+    S.cb_off[s + 1] = S.cb_off[s] + cb * cb;
+    S.factor_entries += static_cast<double>(m * nc) - 0.5 * static_cast<double>(nc * (nc - 1));
+    const double pp = static_cast<double>(nc), c = static_cast<double>(cb);
+    S.flops += pp * pp * pp / 3.0 + pp * pp * c + pp * c * c;
+    S.max_front = std::max<int32_t>(S.max_front, static_cast<int32_t>(m));
+    S.max_ncol = std::max<int32_t>(S.max_ncol, static_cast<int32_t>(nc));
+  }
+  // levels (height): children first
+  S.level.assign(S.nsn, 0);
+  for (int32_t s = 0; s < S.nsn; ++s)
+    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q)
+      S.level[s] = std::max(S.level[s], S.level[S.child_list[q]] + 1);
+  S.nlevels = 0;
+  for (int32_t s = 0; s < S.nsn; ++s) S.nlevels = std::max(S.nlevels, S.level[s] + 1);
+  S.level_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  for (int32_t s = 0; s < S.nsn; ++s) ++S.level_ptr[S.level[s] + 1];
+  for (int32_t l = 0; l < S.nlevels; ++l) S.level_ptr[l + 1] += S.level_ptr[l];
+  S.level_sn.resize(S.nsn);
+  {
+    std::vector<int32_t> fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
+    for (int32_t s = 0; s < S.nsn; ++s) S.level_sn[fill[S.level[s]]++] = s;
+  }
+  return S;
+}
+
+std::vector<int64_t> assembly_map(const Symbolic& S, int32_t n, const int64_t* rowptr, const int32_t* colind) {
+  std::vector<int64_t> dst(static_cast<size_t>(rowptr[n]), -1);
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t in = S.iorder[i];
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int32_t jn = S.iorder[colind[p]];
+      if (in < jn) continue;  // mirrored copy above the diagonal
+      const int32_t s = S.col_sn[jn];
+      const int32_t* r = &S.rows[S.sn_rowoff[s]];
+      const int32_t m = S.sn_m[s];
+      const int32_t* it = std::lower_bound(r, r + m, in);
+      if (it == r + m || *it != in) throw std::runtime_error("assembly_map: entry outside the front");
+      const int64_t lr = it - r, lc = jn - S.sn_first[s];
+      dst[p] = S.panel_off[s] + lc * m + lr;
+    }
+  }
+  return dst;
+}
+
+}  // namespace hsg

@@ -1,0 +1,98 @@
+// Host-side driver of the GPU multifrontal factorization: owns the device copy of the symbolic
+// analysis, the level schedule and the numeric buffers of one factorization object (the role of
+// one MumpsFactorization instance in the reference, src/factor/mumps_factorization.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../host/analysis.hpp"
+
+namespace hsg {
+
+struct EngineOptions {
+  int panel_width = 64;       // nb of the large-front path (<= 64)
+  int small_front_max = 96;   // fronts of order <= this use the one-workgroup LDS kernel
+};
+
+struct InertiaResult {
+  int64_t neg = 0, zero = 0, pos = 0, two = 0, exact_zero = 0;
+  bool singular = false;
+};
+
+class Engine {
+ public:
+  Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> ia, std::vector<int64_t> ib,
+         const EngineOptions& opt);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  const Symbolic& symbolic() const { return sym_; }
+  int64_t small_fronts() const { return n_small_; }
+  int64_t large_fronts() const { return n_large_; }
+
+  // Numeric factorization; returns the inertia counts (synchronizes the stream).
+  InertiaResult factorize(double alpha, double beta, const double* a_values, const double* b_values, int precision,
+                          double null_tol, hipStream_t stream);
+  // In-place solve of nrhs FP64 columns with the last factorization.
+  void solve(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
+  void release();
+  bool has_factor() const { return valid_; }
+  int precision() const { return precision_; }
+
+ private:
+  struct Level {
+    std::vector<int32_t> h_all, h_small, h_large;
+    int32_t* d_all = nullptr;
+    int n_all = 0;
+    int32_t* d_small = nullptr;
+    int n_small = 0;
+    int small_maxm = 0;
+    int32_t* d_large = nullptr;
+    int n_large = 0;
+    int large_maxm = 0;
+    int large_max_ncol = 0;
+    int large_max_cb = 0;
+  };
+  template <class T>
+  InertiaResult factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
+                            hipStream_t stream);
+  template <class T>
+  void solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
+  void alloc_numeric(int precision);
+  void ensure_device();  // uploads the analysis on first use (the analysis itself is host-only)
+  void free_numeric();
+
+  Symbolic sym_;
+  EngineOptions opt_;
+  std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_;
+  bool on_device_ = false;
+  int64_t nnz_ = 0;
+  int64_t n_small_ = 0, n_large_ = 0;
+  int64_t panel_total_ = 0, cb_total_ = 0, w_total_ = 0, rows_total_ = 0;
+  int max_large_per_level_ = 0;
+  // device symbolic arrays
+  int32_t *d_first_ = nullptr, *d_ncol_ = nullptr, *d_m_ = nullptr, *d_child_ptr_ = nullptr,
+          *d_child_list_ = nullptr, *d_relmap_ = nullptr, *d_rows_orig_ = nullptr;
+  int64_t *d_rowoff_ = nullptr, *d_panel_off_ = nullptr, *d_cb_off_ = nullptr, *d_w_off_ = nullptr;
+  int64_t *d_asm_ = nullptr, *d_ia_ = nullptr, *d_ib_ = nullptr;
+  std::vector<Level> levels_;
+  // numeric
+  int precision_ = -1;
+  bool valid_ = false;
+  void *d_L_ = nullptr, *d_CB_ = nullptr, *d_W_ = nullptr, *d_dval_ = nullptr, *d_doff_ = nullptr;
+  int8_t* d_ptype_ = nullptr;
+  int32_t *d_piv_ = nullptr, *d_ipiv_ = nullptr, *d_bperm_ = nullptr;
+  unsigned long long* d_cnt_ = nullptr;
+  void *d_Z_ = nullptr, *d_U_ = nullptr;
+  int64_t ws_cols_ = 0;
+};
+
+// Throws std::runtime_error with the HIP error string on failure.
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace hsg

@@ -1,0 +1,44 @@
+// Launchers of the operator / Krylov kernels (krylov.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace hsg {
+
+// Device scratch that grows on demand (partial sums of the two-pass reductions).
+class Scratch {
+ public:
+  Scratch() = default;
+  ~Scratch();
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  void* get(size_t bytes);
+
+ private:
+  void* ptr_ = nullptr;
+  size_t size_ = 0;
+};
+
+void csr_spmm(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* x, int64_t ldx,
+              double* y, int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st);
+void diag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+               double alpha, double beta, hipStream_t st);
+void shifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const double* av,
+                      const int64_t* brp, const int32_t* bci, const double* bv, const double* bd, double sigma,
+                      const double* x, int64_t ldx, const double* y, int64_t ldy, double* r, int64_t ldr, int64_t p,
+                      double* rnorm2, double* omega, hipStream_t st);
+void gram(Scratch& scratch, int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w,
+          int64_t ldw, int64_t p, double* c, int64_t ldc, hipStream_t st);
+void column_dots(Scratch& scratch, int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                 const double* wd, double* out, hipStream_t st);
+void pair_sums(Scratch& scratch, int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
+               const double* lambda, double* out, hipStream_t st);
+void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
+           hipStream_t st);
+void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                  int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st);
+
+}  // namespace hsg

@@ -1,0 +1,273 @@
+"""The pencil K(alpha, beta) = alpha A + beta B and its GPU factorization.
+
+Pencil mirrors ShiftedPencil<T> (include/hsolve/sparse.hpp:176-223, src/sparse/pencil.cpp:67-170):
+one union pattern of A and B serves every shift. GpuFactorization mirrors the factorization
+contract SparseFactorization<T> (include/hsolve/factor.hpp:341-443): analyze once, factorize
+for any (alpha, beta), a singular K makes factorize() return False (never raises for null
+pivots), count() gives the Sylvester inertia, solve() works in place on column-major blocks.
+The numeric work runs in libhsg on the GPU (hsg_factorize / hsg_solve); device buffers are torch
+tensors. Blocks of vectors are torch tensors of shape (p, n): row c is column c of the n x p
+column-major block the C ABI sees (leading dimension n).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from .errors import InvalidInputError
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("hsolve_amd needs a ROCm GPU: torch.cuda.is_available() is False (no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> C.c_void_p:
+    if t is None:
+        return C.c_void_p(0)
+    return C.c_void_p(t.data_ptr())
+
+
+def host_ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def _csr(m, name: str) -> sp.csr_matrix:
+    m = sp.csr_matrix(m)
+    m.sum_duplicates()
+    m.sort_indices()
+    if m.shape[0] != m.shape[1]:
+        raise InvalidInputError(f"{name} must be square, got shape {m.shape}")
+    return m
+
+
+@dataclass
+class Inertia:
+    """Inertia of K (include/hsolve/factor.hpp:295-300): valid <=> the factorization is usable."""
+    neg: int = -1
+    zero: int = -1
+    pos: int = -1
+    valid: bool = False
+    two_by_two: int = 0
+
+
+@dataclass
+class FactorStats:
+    """FactorStats (include/hsolve/factor.hpp:312-336), the fields this engine measures."""
+    analyses: int = 0
+    analyze_s: float = 0.0
+    factorize_s: float = 0.0
+    solve_s: float = 0.0
+    factorizations: int = 0
+    solves: int = 0
+    rhs_solved: int = 0
+    factor_entries: int = 0
+    flops: float = 0.0
+    null_pivots: int = 0
+    delayed_pivots: int = 0
+    two_by_two_pivots: int = 0
+    supernodes: int = 0
+    max_front: int = 0
+    extra: dict = field(default_factory=dict)
+
+
+class Pencil:
+    """A and B (B = I when None) on the host and on the device, with the union pattern."""
+
+    def __init__(self, A, B=None):
+        A = _csr(A, "A").astype(np.float64)
+        n = A.shape[0]
+        self.n = n
+        self.b_is_identity = B is None
+        if B is None:
+            Bm = sp.identity(n, format="csr", dtype=np.float64)
+        else:
+            Bm = _csr(B, "B").astype(np.float64)
+            if Bm.shape != A.shape:
+                raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {Bm.shape}")
+        self.A = A
+        self.B = None if B is None else Bm
+        self._Bfull = Bm
+        diag_b = Bm.nnz == n and np.array_equal(Bm.indices, np.arange(n)) and np.array_equal(
+            Bm.indptr, np.arange(n + 1))
+        self.b_is_diagonal = bool(diag_b)
+        self._build_union(A, Bm)
+        self._dev = None
+
+    def _build_union(self, A, Bm):
+        n = self.n
+        ra = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.indptr))
+        rb = np.repeat(np.arange(n, dtype=np.int64), np.diff(Bm.indptr))
+        ka = ra * n + A.indices.astype(np.int64)
+        kb = rb * n + Bm.indices.astype(np.int64)
+        keys = np.union1d(ka, kb)
+        self.u_rowptr = np.zeros(n + 1, dtype=np.int64)
+        rows = keys // n
+        np.add.at(self.u_rowptr, rows + 1, 1)
+        np.cumsum(self.u_rowptr, out=self.u_rowptr)
+        self.u_col = (keys % n).astype(np.int32)
+        self.a_index = np.full(keys.size, -1, dtype=np.int64)
+        self.b_index = np.full(keys.size, -1, dtype=np.int64)
+        self.a_index[np.searchsorted(keys, ka)] = np.arange(ka.size, dtype=np.int64)
+        self.b_index[np.searchsorted(keys, kb)] = np.arange(kb.size, dtype=np.int64)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.u_col.size)
+
+    def to_device(self):
+        """Device copies: A (CSR), B (CSR values, and its diagonal when B is diagonal)."""
+        if self._dev is None:
+            dev = device()
+            d = {
+                "a_rowptr": torch.from_numpy(self.A.indptr.astype(np.int64)).to(dev),
+                "a_col": torch.from_numpy(self.A.indices.astype(np.int32)).to(dev),
+                "a_val": torch.from_numpy(self.A.data.astype(np.float64)).to(dev),
+                "b_val": torch.from_numpy(self._Bfull.data.astype(np.float64)).to(dev),
+            }
+            if self.b_is_diagonal:
+                d["b_diag"] = d["b_val"]
+                d["b_rowptr"] = d["b_col"] = None
+            else:
+                d["b_diag"] = None
+                d["b_rowptr"] = torch.from_numpy(self._Bfull.indptr.astype(np.int64)).to(dev)
+                d["b_col"] = torch.from_numpy(self._Bfull.indices.astype(np.int32)).to(dev)
+            self._dev = d
+        return self._dev
+
+
+class GpuFactorization:
+    """Sparse LDL^T of the pencil on the GPU (SparseFactorization contract)."""
+
+    DOUBLE, SINGLE = 0, 1
+    NULL_PIVOT_TOL = {0: 1e-8, 1: 1e-4}  # factor.hpp:279-283
+
+    def __init__(self, pencil: Pencil, precision: str = "double", leaf_size: int = 0, panel_width: int = 0,
+                 small_front_max: int = 0, order: np.ndarray | None = None):
+        self.pencil = pencil
+        self.precision = precision
+        self.stats = FactorStats()
+        self.detection = True
+        self._inertia = Inertia()
+        self._handle = C.c_void_p(0)
+        self._opts = _lib.AnalysisOptions(leaf_size, panel_width, small_front_max)
+        self._given_order = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        self.sigma_held = None
+        self._analyze()
+
+    # ------------------------------------------------------------------ analysis
+    def _analyze(self):
+        p = self.pencil
+        t0 = time.perf_counter()
+        h = C.c_void_p(0)
+        order_ptr = host_ptr(self._given_order) if self._given_order is not None else C.c_void_p(0)
+        _lib.call("hsg_analyze", p.n, host_ptr(p.u_rowptr), host_ptr(p.u_col), host_ptr(p.a_index),
+                  host_ptr(p.b_index), order_ptr, C.byref(self._opts), C.byref(h))
+        self._handle = h
+        self.stats.analyses += 1
+        self.stats.analyze_s += time.perf_counter() - t0
+        info = _lib.AnalysisInfo()
+        _lib.call("hsg_analysis_get_info", self._handle, C.byref(info))
+        self.info = info
+        self.stats.factor_entries = int(info.factor_entries)
+        self.stats.flops = float(info.flops)
+        self.stats.supernodes = int(info.supernodes)
+        self.stats.max_front = int(info.max_front)
+
+    def order(self) -> np.ndarray:
+        out = np.empty(self.pencil.n, dtype=np.int32)
+        _lib.call("hsg_analysis_get_order", self._handle, host_ptr(out))
+        return out
+
+    def fronts(self) -> dict:
+        """Supernodal front structure (see hsg_analysis_get_fronts)."""
+        ns = int(self.info.supernodes)
+        out = {k: np.empty(ns, dtype=np.int32) for k in ("first", "ncol", "m", "parent")}
+        out["rowoff"] = np.empty(ns + 1, dtype=np.int64)
+        out["rows"] = np.empty(int(self.info.front_rows), dtype=np.int32)
+        _lib.call("hsg_analysis_get_fronts", self._handle, host_ptr(out["first"]), host_ptr(out["ncol"]),
+                  host_ptr(out["m"]), host_ptr(out["parent"]), host_ptr(out["rowoff"]), host_ptr(out["rows"]))
+        return out
+
+    def estimate(self) -> dict:
+        bytes_per = 8 if self.precision == "double" else 4
+        return {"factor_entries": int(self.info.factor_entries), "flops": float(self.info.flops),
+                "incore_mb": (self.info.panel_bytes_double + self.info.cb_bytes_double) / 8 * bytes_per / 1e6,
+                "ooc_mb": 0.0}
+
+    # ------------------------------------------------------------------ numeric
+    def set_null_pivot_detection(self, on: bool):
+        self.detection = bool(on)
+
+    def factorize(self, alpha: float, beta: float, precision: str | None = None) -> bool:
+        """K = alpha A + beta B. False when K is (numerically) singular."""
+        prec = self.precision if precision is None else precision
+        code = self.DOUBLE if prec == "double" else self.SINGLE
+        dev = self.pencil.to_device()
+        tol = self.NULL_PIVOT_TOL[code] if self.detection else 0.0
+        inert = _lib.Inertia()
+        sing = C.c_int32(0)
+        t0 = time.perf_counter()
+        _lib.call("hsg_factorize", self._handle, float(alpha), float(beta), ptr(dev["a_val"]), ptr(dev["b_val"]),
+                  code, tol, stream_ptr(), C.byref(inert), C.byref(sing))
+        self.stats.factorize_s += time.perf_counter() - t0
+        self.stats.factorizations += 1
+        self.stats.null_pivots = int(inert.zero)
+        self.stats.two_by_two_pivots = int(inert.two_by_two)
+        self._inertia = Inertia(int(inert.neg), int(inert.zero), int(inert.pos), bool(inert.valid),
+                                int(inert.two_by_two))
+        self.factored_precision = prec
+        self.sigma_held = (alpha, beta, prec, self.detection) if inert.valid else None
+        return bool(inert.valid)
+
+    def inertia(self) -> Inertia:
+        return self._inertia
+
+    def count(self, alpha: float, beta: float) -> Inertia:
+        """Sylvester inertia of alpha A + beta B from a double factorization (the factors are kept
+        and usable for solves when valid)."""
+        self.factorize(alpha, beta, precision="double")
+        return self._inertia
+
+    def solve(self, x: torch.Tensor) -> torch.Tensor:
+        """In place: x (p, n) float64 contiguous on the device, rows = right-hand sides."""
+        if not self._inertia.valid:
+            raise RuntimeError("solve without a usable factorization")
+        if x.dim() == 1:
+            x2 = x.view(1, -1)
+        else:
+            x2 = x
+        if x2.dtype != torch.float64 or not x2.is_contiguous() or x2.shape[1] != self.pencil.n:
+            raise InvalidInputError("solve expects a contiguous float64 (p, n) device tensor")
+        t0 = time.perf_counter()
+        _lib.call("hsg_solve", self._handle, ptr(x2), x2.shape[0], self.pencil.n, stream_ptr())
+        self.stats.solve_s += time.perf_counter() - t0
+        self.stats.solves += 1
+        self.stats.rhs_solved += int(x2.shape[0])
+        return x
+
+    def release_factors(self) -> bool:
+        _lib.call("hsg_release_factors", self._handle)
+        self._inertia = Inertia(self._inertia.neg, self._inertia.zero, self._inertia.pos, False)
+        self.sigma_held = None
+        return True
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().hsg_factor_destroy(h)
+            except Exception:
+                pass
+            self._handle = C.c_void_p(0)

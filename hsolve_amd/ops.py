@@ -1,0 +1,246 @@
+"""Device kernels of the operator and Krylov layers, and the shift-invert operator.
+
+Blocks of vectors are torch float64 tensors of shape (p, n) on the GPU (row c = column c of the
+column-major n x p block of the C ABI). Every arithmetic kernel is a libhsg entry point; torch
+only allocates, copies and indexes.
+
+ShiftInvertOperator restates src/ops/shift_invert_operator.cpp:83-177: Y = K^-1 B X with
+K = A - sigma B factorized by GpuFactorization; with an inexact (single-precision) factor every
+apply is refined in FP64 against the exact A and B (r = B (x + sigma y) - A y in one fused sweep),
+per column until ||r|| <= rel_tol ||B x|| or the componentwise backward error reaches 8 eps, a
+correction that does not reduce the residual is rejected (best iterate kept), at most max_steps
+block corrections, only the active columns are solved.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .factor import GpuFactorization, Pencil, device, ptr, stream_ptr
+
+EPS = float(np.finfo(np.float64).eps)
+ROUNDING_LEVEL = 8.0 * EPS  # shift_invert_operator.cpp:37
+
+
+class DeviceOps:
+    """Kernels bound to one pencil (A, B on the device)."""
+
+    def __init__(self, pencil: Pencil):
+        self.pencil = pencil
+        self.n = pencil.n
+        self.dev = device()
+        self.d = pencil.to_device()
+
+    # --------------------------------------------------------------- helpers
+    def empty(self, p: int) -> torch.Tensor:
+        return torch.empty((p, self.n), dtype=torch.float64, device=self.dev)
+
+    def zeros(self, p: int) -> torch.Tensor:
+        return torch.zeros((p, self.n), dtype=torch.float64, device=self.dev)
+
+    # --------------------------------------------------------------- products
+    def apply_a(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha=1.0, beta=0.0) -> torch.Tensor:
+        p = x.shape[0]
+        if y is None:
+            y = self.empty(p)
+        d = self.d
+        _lib.call("hsg_csr_spmm", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]), ptr(x), self.n,
+                  ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
+        return y
+
+    def apply_b(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha=1.0, beta=0.0) -> torch.Tensor:
+        p = x.shape[0]
+        if y is None:
+            y = self.empty(p)
+        d = self.d
+        if self.pencil.b_is_diagonal:
+            # B = I (b_diag holds ones) or a diagonal mass matrix
+            _lib.call("hsg_diag_spmm", self.n, ptr(d["b_diag"]), ptr(x), self.n, ptr(y), self.n, p, float(alpha),
+                      float(beta), stream_ptr())
+        else:
+            _lib.call("hsg_csr_spmm", self.n, ptr(d["b_rowptr"]), ptr(d["b_col"]), ptr(d["b_val"]), ptr(x), self.n,
+                      ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
+        return y
+
+    def b_weight(self):
+        """Diagonal weight of the B inner product (None for B = I)."""
+        if self.pencil.b_is_identity:
+            return None
+        if self.pencil.b_is_diagonal:
+            return self.d["b_diag"]
+        raise NotImplementedError
+
+    # --------------------------------------------------------------- dense
+    def gram(self, v: torch.Tensor, w: torch.Tensor, weight=None) -> torch.Tensor:
+        """C = V^T diag(weight) W as a (p, k) tensor (column-major k x p)."""
+        k, p = v.shape[0], w.shape[0]
+        c = torch.empty((p, k), dtype=torch.float64, device=self.dev)
+        if k == 0 or p == 0:
+            return c
+        _lib.call("hsg_gram", self.n, ptr(v), self.n, k, ptr(weight), ptr(w), self.n, p, ptr(c), k, stream_ptr())
+        return c
+
+    def b_gram(self, v: torch.Tensor, w: torch.Tensor, bw: torch.Tensor | None = None) -> torch.Tensor:
+        """V^T B W; bw = B W when already formed (general B)."""
+        if self.pencil.b_is_diagonal:
+            return self.gram(v, w, self.b_weight())
+        if bw is None:
+            bw = self.apply_b(w)
+        return self.gram(v, bw, None)
+
+    def update(self, v: torch.Tensor, c: torch.Tensor, w: torch.Tensor, alpha=1.0, beta=1.0) -> torch.Tensor:
+        """W = beta W + alpha V C with C a (p, k) tensor (column-major k x p)."""
+        k, p = v.shape[0], w.shape[0]
+        if p == 0:
+            return w
+        _lib.call("hsg_block_update", self.n, ptr(v), self.n, k, ptr(c), k, ptr(w), self.n, p, float(alpha),
+                  float(beta), stream_ptr())
+        return w
+
+    def dots(self, x: torch.Tensor, y: torch.Tensor, weight=None) -> torch.Tensor:
+        p = x.shape[0]
+        out = torch.empty(p, dtype=torch.float64, device=self.dev)
+        if p:
+            _lib.call("hsg_column_dots", self.n, ptr(x), self.n, ptr(y), self.n, p, ptr(weight), ptr(out),
+                      stream_ptr())
+        return out
+
+    def b_dots(self, x, y, by=None):
+        if self.pencil.b_is_diagonal:
+            return self.dots(x, y, self.b_weight())
+        if by is None:
+            by = self.apply_b(y)
+        return self.dots(x, by)
+
+    def pair_sums(self, ax: torch.Tensor, bx: torch.Tensor, x: torch.Tensor, lam: np.ndarray) -> np.ndarray:
+        """(p, 3) host array: ||ax - lambda bx||^2, ||bx||^2, ||x||^2 per column."""
+        p = x.shape[0]
+        if p == 0:
+            return np.zeros((0, 3))
+        ld = torch.from_numpy(np.ascontiguousarray(lam, dtype=np.float64)).to(self.dev)
+        out = torch.empty(3 * p, dtype=torch.float64, device=self.dev)
+        _lib.call("hsg_pair_sums", self.n, ptr(ax), ptr(bx), ptr(x), self.n, p, ptr(ld), ptr(out), stream_ptr())
+        return out.cpu().numpy().reshape(p, 3)
+
+    def axpby(self, a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
+        p = x.shape[0]
+        if p:
+            _lib.call("hsg_axpby", self.n, float(a), ptr(x), self.n, float(b), ptr(y), self.n, p, stream_ptr())
+        return y
+
+    def shifted_residual(self, x: torch.Tensor, y: torch.Tensor, sigma: float, r: torch.Tensor | None = None):
+        """R = B (X + sigma Y) - A Y, with ||R_c||^2 and omega_c per column (host arrays)."""
+        p = x.shape[0]
+        if r is None:
+            r = self.empty(p)
+        rn = torch.empty(p, dtype=torch.float64, device=self.dev)
+        om = torch.empty(p, dtype=torch.float64, device=self.dev)
+        d = self.d
+        _lib.call("hsg_shifted_residual", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]),
+                  ptr(d["b_rowptr"]), ptr(d["b_col"]), ptr(d["b_val"]), ptr(d["b_diag"]), float(sigma), ptr(x),
+                  self.n, ptr(y), self.n, ptr(r), self.n, p, ptr(rn), ptr(om), stream_ptr())
+        return r, rn, om
+
+
+@dataclass
+class SolveRefinement:
+    """include/hsolve/operator.hpp:51-57."""
+    max_steps: int = 0
+    rel_tol: float = 0.0
+    stop_at_rounding_level: bool = True
+
+
+class InexactFactorFailure(RuntimeError):
+    """A refined apply with an inexact factor did not converge (src/driver/steps.cpp:132-143)."""
+
+
+class ShiftInvertOperator:
+    """Op = K^{-1} B, K = A - sigma B (src/ops/shift_invert_operator.cpp:83-177)."""
+
+    def __init__(self, factor: GpuFactorization, ops: DeviceOps, sigma: float, refinement: SolveRefinement | None = None,
+                 checked: bool = False, accept_rel: float = math.inf):
+        if not factor.inertia().valid:
+            from .errors import InvalidInputError
+            raise InvalidInputError("ShiftInvertOperator: the factor holds no usable factorization")
+        self.factor = factor
+        self.ops = ops
+        self.sigma = float(sigma)
+        self.ref = refinement or SolveRefinement()
+        self.checked = checked
+        self.accept_rel = accept_rel
+        self.applications = 0
+        self.refinement_steps = 0
+        self.last_rel_residual = 0.0
+        self.last_backward_error = 0.0
+        self.last_converged = True
+        self._factorizations = factor.stats.factorizations
+
+    def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        if self.factor.stats.factorizations != self._factorizations:
+            from .errors import InternalError
+            raise InternalError("ShiftInvertOperator: the factorization was replaced after the operator was built")
+        ops = self.ops
+        p = x.shape[0]
+        self.applications += p
+        y = ops.apply_b(x, y)
+        if self.ref.max_steps == 0:
+            self.factor.solve(y)
+            self.last_converged = True
+            return y
+        bxn = torch.sqrt(ops.dots(y, y)).cpu().numpy()
+        self.factor.solve(y)
+        r, rn2, om = ops.shifted_residual(x, y, self.sigma)
+        rn2 = rn2.cpu().numpy()
+        omega = om.cpu().numpy()
+
+        def rel_of(rr2, k):
+            rnk = math.sqrt(max(rr2[k], 0.0))
+            if rnk == 0.0:
+                return 0.0
+            return rnk / bxn[k] if bxn[k] > 0 else math.inf
+
+        rel = np.array([rel_of(rn2, k) for k in range(p)])
+        floor = ROUNDING_LEVEL if self.ref.stop_at_rounding_level else -1.0
+        active = (rel > self.ref.rel_tol) & (omega > floor)
+        for _ in range(self.ref.max_steps):
+            if not active.any():
+                break
+            cols = np.flatnonzero(active)
+            idx = torch.from_numpy(cols).to(ops.dev)
+            c = r.index_select(0, idx).contiguous()
+            self.factor.solve(c)
+            self.refinement_steps += 1
+            cand = y.index_select(0, idx).contiguous()
+            ops.axpby(1.0, c, 1.0, cand)  # cand = y + correction
+            xs = x.index_select(0, idx).contiguous()
+            rc, rc2, omc = ops.shifted_residual(xs, cand, self.sigma)
+            rc2 = rc2.cpu().numpy()
+            omc = omc.cpu().numpy()
+            accept = []
+            for j, k in enumerate(cols):
+                cr = rel_of(rc2, j) if bxn[k] > 0 else math.inf
+                if cr < rel[k]:
+                    accept.append(j)
+                    rel[k] = cr
+                    omega[k] = omc[j]
+                    if cr > self.ref.rel_tol and omega[k] > floor:
+                        continue
+                active[k] = False
+            if accept:
+                a_idx = torch.from_numpy(np.asarray(accept)).to(ops.dev)
+                dst = torch.from_numpy(cols[np.asarray(accept)]).to(ops.dev)
+                y.index_copy_(0, dst, cand.index_select(0, a_idx))
+                r.index_copy_(0, dst, rc.index_select(0, a_idx))
+        self.last_rel_residual = float(rel.max()) if p else 0.0
+        self.last_backward_error = float(omega.max()) if p else 0.0
+        self.last_converged = bool(np.all((rel <= self.ref.rel_tol) | (omega <= floor)))
+        if self.checked and not self.last_converged and not (self.last_rel_residual <= self.accept_rel):
+            raise InexactFactorFailure(
+                f"the refinement of an operator application at sigma = {self.sigma:.6g} did not converge "
+                f"(relative residual {self.last_rel_residual:.3g}, backward error {self.last_backward_error:.3g}): "
+                "eps_factor * cond(K) is too large at this shift")
+        return y

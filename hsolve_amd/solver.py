@@ -1,0 +1,200 @@
+"""hs::solve for HEP / GHEP (B Hermitian positive definite) on the GPU engine.
+
+Restates driver::solve_with_helpers (src/driver/solver.cpp:523-694) for the definite path:
+  * hermitian_input: finite values, relative Hermitian defect <= hermitian_tol, then the exact
+    Hermitian part (solver.cpp:70-114);
+  * norm estimates ||A||_2, ||B||_2 (30-step Lanczos, seed 2026; B = I -> 1) and scale
+    (solver.cpp:558-562);
+  * problem type: B absent -> standard; B diagonal with positive entries -> generalized definite;
+    a general B is tested by the Sylvester count of B itself on the GPU (nu(B) = 0 <=> B HPD,
+    solver.cpp:231-265); indefinite B (GHIEP) is unsupported here;
+  * analysis, then solve_definite: nearest -> nearest.solve_nearest, interval / smallest /
+    largest -> windows.WindowSolver (definite.cpp:2277-2335);
+  * single-precision working factors with the FP64 fallback (solver.cpp:615-675): an attempt whose
+    refinement fails, whose pairs miss tol or whose certificate fails restarts with double
+    factors on the same analysis (report.precision.fallback);
+  * report: SolveReport with the reference's keys (serialization.cpp:283-413), exclusive phase
+    times that add up to time_total_s.
+"""
+from __future__ import annotations
+
+import math
+import resource
+import time
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import config as cfgmod
+from .driver_core import Ctx, PhaseClock, estimate_norm2
+from .errors import InvalidInputError, NotHermitianError, UnsupportedError
+from .factor import GpuFactorization, Pencil
+from .ops import DeviceOps, InexactFactorFailure
+
+
+def hermitian_input(M, name: str, tol: float):
+    """Checks and Hermitian part (solver.cpp:70-114): returns (csr, defect)."""
+    M = sp.csr_matrix(M)
+    if M.shape[0] != M.shape[1]:
+        raise InvalidInputError(f"{name} must be square, got {M.shape}")
+    if M.nnz and not np.all(np.isfinite(M.data)):
+        raise InvalidInputError(f"{name} has non-finite entries")
+    if np.iscomplexobj(M.data):
+        if M.nnz and np.all(M.data.imag == 0):
+            M = sp.csr_matrix(M.real)
+        else:
+            raise UnsupportedError(f"{name}: complex Hermitian pencils are not supported by this engine yet")
+    M = M.astype(np.float64)
+    fro = sp.linalg.norm(M) if M.nnz else 0.0
+    if fro == 0.0:
+        raise InvalidInputError(f"{name} is zero")
+    D = (M - M.T).tocsr()
+    defect = (sp.linalg.norm(D) / fro) if D.nnz else 0.0
+    if defect > tol:
+        raise NotHermitianError(f"{name} is not Hermitian: ||M - M^H||_F / ||M||_F = {defect:.3g} > {tol:.3g}")
+    H = ((M + M.T) * 0.5).tocsr()
+    H.sum_duplicates()
+    H.eliminate_zeros()
+    H.sort_indices()
+    return H, float(defect)
+
+
+def _b_kind(B):
+    """'identity' / 'diagonal' / 'general'."""
+    if B is None:
+        return "identity"
+    Bc = sp.csr_matrix(B)
+    n = Bc.shape[0]
+    if Bc.nnz == n and np.array_equal(Bc.indices, np.arange(n)) and np.array_equal(Bc.indptr, np.arange(n + 1)):
+        return "diagonal"
+    off = Bc - sp.diags(Bc.diagonal())
+    off.eliminate_zeros()
+    return "diagonal" if off.nnz == 0 else "general"
+
+
+def solve(A, B=None, config: dict | None = None):
+    """Returns (eigenvalues ndarray, eigenvectors ndarray (n, m) Fortran order, report dict, config dict)."""
+    cfg = cfgmod.parse_config(config or {})
+    rep = cfgmod.new_report()
+    clock = PhaseClock()
+    clock.push("setup")
+    t_start = time.perf_counter()
+    A, def_a = hermitian_input(A, "A", cfg["hermitian_tol"])
+    rep["hermitian_defect_a"] = def_a
+    bk = _b_kind(B)
+    if B is not None:
+        B, def_b = hermitian_input(B, "B", cfg["hermitian_tol"])
+        if B.shape != A.shape:
+            raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {B.shape}")
+        rep["hermitian_defect_b"] = def_b
+        if bk == "diagonal":
+            B = sp.diags(B.diagonal(), format="csr")
+    n = A.shape[0]
+    rep["n"] = n
+    rep["nnz_a"] = int(A.nnz)
+    rep["nnz_b"] = 0 if B is None else int(B.nnz)
+    rep["sigma_requested"] = cfg["target"]["sigma"]
+    if cfg["problem"] == "generalized_indefinite":
+        raise UnsupportedError("indefinite B (GHIEP) is not supported by this engine")
+    if B is not None and bk == "diagonal" and np.any(B.diagonal() <= 0):
+        raise UnsupportedError("B is diagonal but not positive definite (GHIEP): not supported by this engine")
+    pencil = Pencil(A, B)
+    ops = DeviceOps(pencil)
+    n2a = estimate_norm2(ops, ops.apply_a, 30, cfg["seed"])
+    n2b = 1.0 if B is None else estimate_norm2(ops, ops.apply_b, 30, cfg["seed"])
+    rep["norm2_a"], rep["norm2_b"] = n2a, n2b
+    scale = n2a / n2b if n2b > 0 else n2a
+    rep["scale"] = scale
+    rep["problem"] = "standard" if B is None else "generalized_definite"
+    rep["method"] = "shift_invert_lanczos"
+    clock.pop()
+    with clock.phase("factor"):
+        fac = GpuFactorization(pencil)
+    if B is not None and bk == "general":
+        with clock.phase("setup"):
+            fac.set_null_pivot_detection(True)
+            ok = fac.factorize(0.0, 1.0, precision="double")
+            inn = fac.inertia()
+            if not ok or inn.neg > 0:
+                raise UnsupportedError("B is not positive definite (GHIEP): not supported by this engine")
+    ctx = Ctx(cfg=cfg, rep=rep, clock=clock, pencil=pencil, ops=ops, fac=fac, n2a=n2a, n2b=n2b, scale=scale)
+    requested = cfg["factor_precision"]
+    ctx.precision = "single" if requested == "single" else "double"
+    rep["precision"]["requested"] = requested
+    rep["precision"]["decision"] = ("double analysis expected" if requested != "auto" else
+                                    "double: in-core estimate fits the device memory")
+    mode = cfg["target"]["mode"]
+    result = None
+    if ctx.precision == "single":
+        t_attempt = time.perf_counter()
+        try:
+            result = _solve_definite(ctx, mode)
+            if cfg["inexact_fault_reject"] and result[0].size:
+                result[1][0].backward_error = math.inf
+                result = (result[0], result[1], False)
+            if not result[2]:
+                raise InexactFactorFailure("pairs not accepted or certificate failed")
+        except InexactFactorFailure as e:
+            rep["precision"]["fallback"] = True
+            rep["precision"]["fallback_reason"] = str(e)
+            rep["precision"]["attempt_s"] = time.perf_counter() - t_attempt
+            ctx.warn(f"single-precision attempt abandoned ({e}); solving again with double factors")
+            keep = {k: rep[k] for k in ("warnings", "precision", "hermitian_defect_a", "hermitian_defect_b", "n",
+                                        "nnz_a", "nnz_b", "norm2_a", "norm2_b", "scale", "problem", "method",
+                                        "sigma_requested")}
+            rep.clear()
+            rep.update(cfgmod.new_report())
+            rep.update(keep)
+            ctx.precision = "double"
+            ctx.held = type(ctx.held)()
+            result = None
+    if result is None:
+        result = _solve_definite(ctx, mode)
+    sel, metrics, ok = result
+    rep["precision"]["used"] = ctx.precision
+    # report
+    with clock.phase("verify"):
+        lam = np.asarray(sel.lam, dtype=np.float64)
+        vecs = sel.x.detach().cpu().numpy().T.copy(order="F") if sel.size else np.zeros((n, 0), order="F")
+    all_ok = ok
+    for j, m in enumerate(metrics):
+        acc = m.backward_error <= cfg["tol"]
+        all_ok = all_ok and acc
+        rep["pairs"].append({"lambda_re": float(lam[j]), "lambda_im": 0.0, "abs_residual": m.abs_residual,
+                             "rel_residual": m.rel_residual, "backward_error": m.backward_error, "accepted": acc,
+                             "subspace_sin_bound": None})
+    rep["success"] = bool(all_ok)
+    fs = fac.stats
+    rep["factor_options"]["precision"] = ctx.precision
+    rep["factor_options"]["null_pivot_tol"] = fac.NULL_PIVOT_TOL[0 if ctx.precision == "double" else 1]
+    est = fac.estimate()
+    rep["factor_stats"].update({
+        "analyze_s": fs.analyze_s, "factorize_s": fs.factorize_s, "solve_s": fs.solve_s,
+        "factorizations": fs.factorizations, "solves": fs.solves, "rhs_solved": fs.rhs_solved,
+        "factor_entries": int(fs.factor_entries), "null_pivots": fs.null_pivots,
+        "factor_mb_used": est["incore_mb"],
+    })
+    rep["factor_stats"]["estimate"].update({"factor_entries": est["factor_entries"], "incore_mb": est["incore_mb"],
+                                            "flops": est["flops"]})
+    total = clock.total()
+    rep["time_total_s"] = total
+    for ph in clock.PHASES:
+        rep[f"time_{ph}_s"] = clock.t[ph]
+    rep["peak_rss_bytes"] = int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss) * 1024
+    try:
+        free, tot = torch.cuda.mem_get_info()
+        rep["memory"].update({"budget_gb": tot / 2 ** 30, "source": "device", "limit_gb": tot / 2 ** 30,
+                              "limit_source": "hipMemGetInfo"})
+    except Exception:
+        pass
+    rep["_events"] = ctx.events
+    return lam, vecs, rep, cfg
+
+
+def _solve_definite(ctx: Ctx, mode: str):
+    if mode == "nearest":
+        from .nearest import solve_nearest
+        return solve_nearest(ctx)
+    from .windows import solve_window
+    return solve_window(ctx)

@@ -1,0 +1,198 @@
+/* hsg: MI355X-native shift-invert engine, C ABI.
+ *
+ * Plain C, plain pointers and sizes. Device pointers are HIP device addresses (for example the
+ * data_ptr() of a torch tensor on a ROCm device); streams are hipStream_t passed as void*
+ * (NULL = the default stream). Every function returns an error code; the message of the last
+ * error of the calling thread is hsg_last_error_message().
+ *
+ * Two layers, each replacing an interface of the reference (hsolve, file:line given per entry):
+ *   1. The factorization seam SparseFactorization<T> (include/hsolve/factor.hpp:341-443), which
+ *      the reference fills with MUMPS: analyze / factorize / count / solve / inertia.
+ *   2. The kernels that the reference's shift-invert operator and Krylov solver run on the CPU:
+ *      spmm (include/hsolve/sparse.hpp:151-155), shifted_residual (sparse.hpp:157-173), the
+ *      block Gram / update kernels of CGS2 (src/linalg/dense_ops.cpp:752, :883).
+ * The Python package hsolve_amd binds these with ctypes and provides hsolve.solve() above them.
+ */
+#ifndef HSG_H
+#define HSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: 0 = success, then hs::Error::Code in declaration order plus one
+ * (include/hsolve/error.hpp:15-25), matching the Python error classes
+ * (python/hsolve/__init__.py:48-130). */
+enum {
+  HSG_OK = 0,
+  HSG_INVALID_INPUT = 1,
+  HSG_IO = 2,
+  HSG_NOT_HERMITIAN = 3,
+  HSG_UNSUPPORTED = 4,
+  HSG_FACTORIZATION_FAILED = 5,
+  HSG_SINGULAR = 6,
+  HSG_OUT_OF_MEMORY_BUDGET = 7,
+  HSG_NOT_CONVERGED = 8,
+  HSG_INTERNAL = 9
+};
+
+/* Arithmetic of a numeric factorization (FactorPrecision, include/hsolve/factor.hpp). */
+enum { HSG_DOUBLE = 0, HSG_SINGLE = 1 };
+
+/* Library version string, e.g. "0.1.0" (hs::version, python/hsolve/__init__.py:437). */
+const char* hsg_version(void);
+
+/* Message of the last failing call on this thread (empty if none). */
+const char* hsg_last_error_message(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Layer 1: sparse LDL^T factorization (replaces SparseFactorization<T>, factor.hpp:341-443).   */
+
+typedef struct hsg_factor hsg_factor;
+
+/* Analysis options (FactorOptions, factor.hpp:248-293; only what this engine uses). */
+typedef struct hsg_analysis_options {
+  int32_t leaf_size;          /* nested-dissection leaves (default 64) */
+  int32_t panel_width;        /* column block of the large-front factorization (default 64) */
+  int32_t small_front_max;    /* fronts of order <= this are factorized in LDS (0 = default) */
+  int32_t reserved[5];
+} hsg_analysis_options;
+
+/* Statistics of an analysis (FactorEstimate, factor.hpp:302-310, plus the supernodal shape). */
+typedef struct hsg_analysis_info {
+  int64_t n;
+  int64_t nnz;               /* pattern entries (both triangles) */
+  int64_t supernodes;
+  int64_t levels;
+  int64_t max_front;
+  int64_t max_pivots;        /* largest number of pivot columns of one front */
+  double factor_entries;     /* lower-trapezoid entries of L incl. D (INFOG(20)) */
+  double flops;              /* sum over fronts of p^3/3 + p^2 c + p c^2 (RINFOG(1) analogue) */
+  double panel_bytes_double; /* device bytes of the FP64 factor panels */
+  double cb_bytes_double;    /* device bytes of the FP64 contribution blocks */
+  double analyze_s;          /* host wall time of ordering + symbolic analysis */
+  double ordering_s;
+  int64_t small_fronts;      /* fronts handled by the one-workgroup LDS kernel */
+  int64_t large_fronts;
+  int64_t front_rows;        /* sum over fronts of their order (length of the row lists) */
+} hsg_analysis_info;
+
+/* Inertia of the last factorization (Inertia, factor.hpp:295-300). valid == 0 after a singular
+ * factorization (null pivots), as in the reference. */
+typedef struct hsg_inertia {
+  int64_t neg;
+  int64_t zero;
+  int64_t pos;
+  int32_t valid;
+  int32_t two_by_two;        /* 2x2 pivots chosen */
+} hsg_inertia;
+
+/* Analyse the pattern of the pencil K(alpha, beta) = alpha A + beta B (replaces
+ * SparseFactorization::analyze, factor.hpp:363-371; MUMPS JOB=1 at
+ * src/factor/mumps_factorization.cpp:273). The pattern is the union of A's and B's patterns in
+ * full symmetric CSR form (host pointers, n rows, rowptr[n] entries). For every pattern entry e,
+ * a_index[e] / b_index[e] give the position of that entry in the value arrays of A / B passed to
+ * hsg_factorize (or -1 where the matrix has no entry). Symbolic work (nested-dissection ordering,
+ * elimination tree, supernodes, front maps) runs on the host; the maps are then uploaded.
+ * order (may be NULL): a given elimination order, new -> old (set_pivot_order, factor.hpp:432). */
+int hsg_analyze(int64_t n, const int64_t* rowptr, const int32_t* colind, const int64_t* a_index,
+                const int64_t* b_index, const int32_t* order, const hsg_analysis_options* opt,
+                hsg_factor** out);
+
+int hsg_analysis_get_info(const hsg_factor* f, hsg_analysis_info* info);
+
+/* Elimination order new -> old (n entries) of the analysis (pivot_order, factor.hpp:424-431). */
+int hsg_analysis_get_order(const hsg_factor* f, int32_t* order);
+
+/* Front structure of the analysis, for inspection and tests (no counterpart in the reference,
+ * whose supernodal structure stays inside MUMPS). Arrays sized by hsg_analysis_info:
+ * supernodes entries for first / ncol / m / parent (-1 = root), supernodes + 1 for rowoff, and
+ * front_rows for rows (new numbering, ascending per front; the first ncol are the front's own
+ * pivot columns). */
+int hsg_analysis_get_fronts(const hsg_factor* f, int32_t* first, int32_t* ncol, int32_t* m, int32_t* parent,
+                            int64_t* rowoff, int32_t* rows);
+
+/* Numeric multifrontal LDL^T of K = alpha A + beta B on the GPU (replaces factorize,
+ * factor.hpp:373-392; MUMPS JOB=2 at mumps_factorization.cpp:326): Bunch-Kaufman 1x1 / 2x2
+ * pivots inside each front, inertia read off D. a_values / b_values: device FP64 arrays indexed
+ * by a_index / b_index (b_values may be NULL when no b_index is >= 0). precision: HSG_DOUBLE or
+ * HSG_SINGLE. null_pivot_tol > 0 enables null-pivot detection (a pivot block with an eigenvalue
+ * of magnitude <= null_pivot_tol * max|K_ij| is a null pivot); 0 disables it (then only exactly
+ * zero pivots count). *singular receives 1 when null pivots were found (the reference's
+ * factorize() == false); the inertia is written to *inertia in either case. Synchronizes the
+ * stream before returning. */
+int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_values, const double* b_values,
+                  int32_t precision, double null_pivot_tol, void* stream, hsg_inertia* inertia,
+                  int32_t* singular);
+
+/* Solve K X = X in place for nrhs columns (column-major, leading dimension ldx >= n) on the
+ * device with the last successful factorization (replaces solve, factor.hpp:394-399; MUMPS JOB=3
+ * at mumps_factorization.cpp:506). x is FP64; a single-precision factor rounds it to FP32 for
+ * the triangular solves (as the reference's single factorization does). Asynchronous on stream. */
+int hsg_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream);
+
+/* Frees the numerical factors, keeps the analysis (release_factors, factor.hpp:411-418). */
+int hsg_release_factors(hsg_factor* f);
+
+void hsg_factor_destroy(hsg_factor* f);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Layer 2: kernels of the shift-invert operator and the block Krylov solver.                  */
+
+/* Y = alpha * M X + beta * Y for CSR M (n x n, device arrays, int64 row pointers, int32 column
+ * indices, FP64 values); X, Y: n x p column-major (spmm, sparse.hpp:151-155). */
+int hsg_csr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values,
+                 const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta,
+                 void* stream);
+
+/* Y = alpha * diag(d) X + beta * Y (the diagonal-B case of spmm). */
+int hsg_diag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                  double alpha, double beta, void* stream);
+
+/* Shifted residual R = B X - (A - sigma B) Y = B (X + sigma Y) - A Y in one sweep of A and B,
+ * with per-column FP64 sums |R|^2 and the componentwise backward-error quotient
+ * max_i |R_i| / (|B| |X + sigma Y| + |A - sigma B| |Y|)_i  (shifted_residual,
+ * sparse.hpp:157-173). B is CSR (b_rowptr != NULL) or diagonal (b_diag != NULL) or identity
+ * (both NULL). rnorm2 / omega: device arrays of p doubles (either may be NULL). */
+int hsg_shifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_colind, const double* a_values,
+                         const int64_t* b_rowptr, const int32_t* b_colind, const double* b_values,
+                         const double* b_diag, double sigma, const double* x, int64_t ldx, const double* y,
+                         int64_t ldy, double* r, int64_t ldr, int64_t p, double* rnorm2, double* omega,
+                         void* stream);
+
+/* C = V^T diag(w) W (k x p, column-major, ldc >= k) for V: n x k, W: n x p (w may be NULL =
+ * identity). The Gram / projection kernel of block CGS2 and Rayleigh-Ritz in the B inner product
+ * (gram, dense_ops.cpp:883; orthogonalize_block_cgs2, :752). Deterministic: fixed reduction tree. */
+int hsg_gram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w,
+             int64_t ldw, int64_t p, double* c, int64_t ldc, void* stream);
+
+/* W = beta W + alpha V C for V: n x k, C: k x p (device, column-major), W: n x p. The update of
+ * block CGS2 and the basis rotation of a thick restart (rotate_in_place,
+ * src/eigen/block_lanczos.hpp:96-98). W must not overlap V. */
+int hsg_block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                     int64_t ldw, int64_t p, double alpha, double beta, void* stream);
+
+/* out[c] = x_c^T diag(w) y_c for the p columns (w may be NULL = identity); out: device, p doubles.
+ * Column norms and Rayleigh quotients of the driver (linalg::dot / norm2 in
+ * src/driver/definite.cpp:152-188 and src/ops/shift_invert_operator.cpp:110-113). Deterministic. */
+int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                    const double* w_diag, double* out, void* stream);
+
+/* Residual sums of eigenpair candidates (pair_metrics, src/driver/projection.cpp:53-70): for each of
+ * the p columns, out[3c] = ||ax_c - lambda_c bx_c||^2, out[3c+1] = ||bx_c||^2, out[3c+2] = ||x_c||^2,
+ * all blocks n x p with leading dimension ld; lambda and out are device arrays. Deterministic. */
+int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
+                  const double* lambda, double* out, void* stream);
+
+/* Y = a X + b Y for p columns (linalg::axpy, used by the refinement and rescue steps,
+ * src/ops/shift_invert_operator.cpp:154-159). */
+int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSG_H */
