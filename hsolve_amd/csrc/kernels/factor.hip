@@ -81,9 +81,10 @@ struct PivotCounts {
 // the pivot data. Called by all threads of the workgroup.
 template <class T>
 __device__ void bk_factor_lds(T* F, int ld, int ncand, int mrows, int* perm, T* dv, T* dof, int8_t* pt, T* t1,
-                              T* t2, double tau, bool detect, PivotCounts& pc) {
+                              T* t2, double tau, bool detect, PivotCounts& pc, const double* below) {
   __shared__ int s_kind, s_kp;
   __shared__ T s_inv[3];
+  __shared__ double s_colmax[2];
   const int tid = threadIdx.x;
   const int nth = blockDim.x;
   for (int i = tid; i < ncand; i += nth) perm[i] = i;
@@ -173,12 +174,38 @@ __device__ void bk_factor_lds(T* F, int ld, int ncand, int mrows, int* perm, T* 
       }
       __syncthreads();
     }
+    // Null-pivot test on the whole remaining column (rows of the front below the pivot block,
+    // contribution rows included, plus the rows below this LDS block for the blocked path): the
+    // reference's rule flags a pivot whose remaining row is below tau (CNTL(3)), so a tiny pivot
+    // of a nearly singular fully-summed block that still couples to its contribution rows is a
+    // valid (if large-growth) pivot, not a singularity.
+    if (tid < 64) {
+      const int s0 = kind == 1 ? k + 1 : k + 2;
+      double m0 = 0.0, m1 = 0.0;
+      for (int i = s0 + tid; i < mrows; i += 64) {
+        m0 = fmax(m0, fabs((double)F[i + k * ld]));
+        if (kind == 2) m1 = fmax(m1, fabs((double)F[i + (k + 1) * ld]));
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmax(m0, __shfl_xor(m0, off));
+        m1 = fmax(m1, __shfl_xor(m1, off));
+      }
+      if (tid == 0) {
+        if (below) {
+          m0 = fmax(m0, below[perm[k]]);
+          if (kind == 2) m1 = fmax(m1, below[perm[k + 1]]);
+        }
+        s_colmax[0] = m0;
+        s_colmax[1] = m1;
+      }
+    }
     if (kind == 1) {
       if (tid == 0) {
         const double d = (double)F[k + k * ld];
+        const bool coupled = s_colmax[0] > tau;
         bool null_piv = d == 0.0;
         if (d == 0.0) ++pc.exact_zero;
-        if (detect && fabs(d) <= tau) null_piv = true;
+        if (detect && fabs(d) <= tau && !coupled) null_piv = true;
         if (null_piv) {
           ++pc.zero;
           pt[k] = kPivNull;
@@ -217,8 +244,9 @@ __device__ void bk_factor_lds(T* F, int ld, int ncand, int mrows, int* perm, T* 
         const double e1 = mean - rad, e2 = mean + rad;  // eigenvalues, e1 <= e2
         int nz = 0;
         if (det == 0.0) ++pc.exact_zero;
+        const bool coupled = fmax(s_colmax[0], s_colmax[1]) > tau;
         for (double e : {e1, e2}) {
-          const bool z = (detect && fabs(e) <= tau) || e == 0.0;
+          const bool z = (detect && fabs(e) <= tau && !coupled) || e == 0.0;
           if (z) ++nz;
           else if (e < 0) ++pc.neg;
           else ++pc.pos;
@@ -317,7 +345,7 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
   }
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   PivotCounts pc{0, 0, 0, 0, 0};
-  bk_factor_lds<T>(F, m, ncol, m, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc);
+  bk_factor_lds<T>(F, m, ncol, m, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc, nullptr);
   for (int idx = tid; idx < m * ncol; idx += blockDim.x) N.L[poff + idx] = F[idx];
   T* cbs = N.CB + S.cb_off[s];
   for (int idx = tid; idx < cb * cb; idx += blockDim.x) {
@@ -404,14 +432,25 @@ __global__ void __launch_bounds__(kThreads) k_bk_diag(DevSym S, DevNum<T> N, con
   int* perm = reinterpret_cast<int*>(dof + nb);
   int8_t* pt = reinterpret_cast<int8_t*>(perm + nb);
   const int tid = threadIdx.x;
+  __shared__ double below[64];
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     const int i = idx % w, j = idx / w;
     F[i + j * w] = i >= j ? P[(int64_t)(c0 + j) * m + c0 + i] : (T)0;
   }
+  // column maxima of the panel rows below the block (the null-pivot test's remaining rows)
+  {
+    const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int j = wave; j < w; j += nw) {
+      double mx = 0.0;
+      for (int r = c0 + w + lane; r < m; r += 64) mx = fmax(mx, fabs((double)P[(int64_t)(c0 + j) * m + r]));
+      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+      if (lane == 0) below[j] = mx;
+    }
+  }
   __syncthreads();
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   PivotCounts pc{0, 0, 0, 0, 0};
-  bk_factor_lds<T>(F, w, w, w, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc);
+  bk_factor_lds<T>(F, w, w, w, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc, below);
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     const int i = idx % w, j = idx / w;
     if (i >= j) P[(int64_t)(c0 + j) * m + c0 + i] = F[i + j * w];
