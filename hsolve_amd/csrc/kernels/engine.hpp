@@ -57,6 +57,7 @@ class Engine {
     int large_maxm = 0;
     int large_max_ncol = 0;
     int large_max_cb = 0;
+    int max_m = 0, max_cb = 0, nblk = 0;  // over all fronts of the level (solve launches)
   };
   template <class T>
   InertiaResult factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
@@ -69,7 +70,14 @@ class Engine {
 
   Symbolic sym_;
   EngineOptions opt_;
-  std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_;
+  std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_, h_linv_off_;
+  std::vector<int32_t> h_pairs_;
+  int64_t linv_total_ = 0;
+  int n_pairs_ = 0;
+  int64_t* d_linv_off_ = nullptr;
+  int32_t* d_pairs_ = nullptr;
+  void* d_Linv_ = nullptr;
+  void* d_pinv_ = nullptr;  // panel inverses of the large fronts of one level (panel solve)
   bool on_device_ = false;
   int64_t nnz_ = 0;
   int64_t n_small_ = 0, n_large_ = 0;
@@ -88,7 +96,7 @@ class Engine {
   int8_t* d_ptype_ = nullptr;
   int32_t *d_piv_ = nullptr, *d_ipiv_ = nullptr, *d_bperm_ = nullptr;
   unsigned long long* d_cnt_ = nullptr;
-  void *d_Z_ = nullptr, *d_U_ = nullptr;
+  void *d_Z_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;
   int64_t ws_cols_ = 0;
 };
 

@@ -26,6 +26,7 @@
 
 #include "engine.hpp"
 #include "front_common.hpp"
+#include "solve.hpp"
 
 namespace hsg {
 
@@ -79,9 +80,27 @@ struct PivotCounts {
 // mrows. On exit F holds L (unit, strictly lower; the (k+1, k) entry of a 2x2 pivot holds D's
 // off-diagonal), D on the diagonal; perm[i] = original index of pivot position i; dv / dof / pt
 // the pivot data. Called by all threads of the workgroup.
+// Rows of the blocked path below the LDS block: column j (original local index) of the block
+// continues at base[j * ld + row] for rows [row0, row1) in global memory.
+template <class T>
+struct BelowRows {
+  const T* base = nullptr;
+  int64_t ld = 0;
+  int row0 = 0, row1 = 0;
+};
+
+template <class T>
+__device__ double below_max(const BelowRows<T>& b, int j, int lane) {
+  double mx = 0.0;
+  if (b.base)
+    for (int r = b.row0 + lane; r < b.row1; r += 64) mx = fmax(mx, fabs((double)b.base[(int64_t)j * b.ld + r]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  return mx;
+}
+
 template <class T>
 __device__ void bk_factor_lds(T* F, int ld, int ncand, int mrows, int* perm, T* dv, T* dof, int8_t* pt, T* t1,
-                              T* t2, double tau, bool detect, PivotCounts& pc, const double* below) {
+                              T* t2, double tau, bool detect, PivotCounts& pc, BelowRows<T> below) {
   __shared__ int s_kind, s_kp;
   __shared__ T s_inv[3];
   __shared__ double s_colmax[2];
@@ -190,11 +209,16 @@ __device__ void bk_factor_lds(T* F, int ld, int ncand, int mrows, int* perm, T* 
         m0 = fmax(m0, __shfl_xor(m0, off));
         m1 = fmax(m1, __shfl_xor(m1, off));
       }
-      if (tid == 0) {
-        if (below) {
-          m0 = fmax(m0, below[perm[k]]);
-          if (kind == 2) m1 = fmax(m1, below[perm[k + 1]]);
+      // the rows below an LDS block matter only for a pivot that is itself tiny: scan them lazily
+      if (detect && below.base) {
+        const double d0 = fabs((double)F[k + k * ld]);
+        bool tiny = kind == 1 ? (d0 <= tau && m0 <= tau) : (fmax(m0, m1) <= tau);
+        if (tiny) {
+          m0 = fmax(m0, below_max(below, perm[k], tid));
+          if (kind == 2) m1 = fmax(m1, below_max(below, perm[k + 1], tid));
         }
+      }
+      if (tid == 0) {
         s_colmax[0] = m0;
         s_colmax[1] = m1;
       }
@@ -345,7 +369,7 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
   }
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   PivotCounts pc{0, 0, 0, 0, 0};
-  bk_factor_lds<T>(F, m, ncol, m, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc, nullptr);
+  bk_factor_lds<T>(F, m, ncol, m, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc, BelowRows<T>{});
   for (int idx = tid; idx < m * ncol; idx += blockDim.x) N.L[poff + idx] = F[idx];
   T* cbs = N.CB + S.cb_off[s];
   for (int idx = tid; idx < cb * cb; idx += blockDim.x) {
@@ -411,12 +435,20 @@ __global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<
   }
 }
 
-// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front (one workgroup per front).
+// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, one 64-lane wave per
+// front: lane i owns row i of the block (LDS, padded leading dimension), pivot searches are wave
+// reductions, the multipliers of a rank-1 / rank-2 update are broadcast by shuffles. Also forms
+// the inverse of the block's unit lower L11 (per-front scratch, read by the panel solve).
+constexpr int kBkLd = 65;
+
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int p,
-                                                     int nb, int32_t* __restrict__ bperm, double null_tol,
-                                                     int detect) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+__global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int p,
+                                                int nb, int32_t* __restrict__ bperm, double null_tol, int detect,
+                                                T* __restrict__ Linv) {
+  __shared__ T F[64 * kBkLd];
+  __shared__ int perm[64];
+  __shared__ unsigned long long s_key[3];  // maxima as bit patterns of non-negative doubles
+  __shared__ int s_idx;
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -424,47 +456,219 @@ __global__ void __launch_bounds__(kThreads) k_bk_diag(DevSym S, DevNum<T> N, con
   if (c0 >= ncol) return;
   const int w = min(nb, ncol - c0);
   T* P = N.L + S.panel_off[s];
-  T* F = reinterpret_cast<T*>(smem_raw);
-  T* t1 = F + nb * nb;
-  T* t2 = t1 + nb;
-  T* dv = t2 + nb;
-  T* dof = dv + nb;
-  int* perm = reinterpret_cast<int*>(dof + nb);
-  int8_t* pt = reinterpret_cast<int8_t*>(perm + nb);
-  const int tid = threadIdx.x;
-  __shared__ double below[64];
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int i = idx % w, j = idx / w;
-    F[i + j * w] = i >= j ? P[(int64_t)(c0 + j) * m + c0 + i] : (T)0;
+  const int lane = threadIdx.x;
+  for (int j = 0; j < w; ++j) F[lane + j * kBkLd] = (lane >= j && lane < w) ? P[(int64_t)(c0 + j) * m + c0 + lane] : (T)0;
+  perm[lane] = lane;
+  __syncthreads();
+  const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
+  const bool det = detect != 0;
+  int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // lane 0's tallies
+  int8_t my_pt = kPiv1;
+  T my_dv = (T)0, my_dof = (T)0;
+  // Wave maxima through LDS atomics on the bit patterns of non-negative doubles (integer order =
+  // value order); the pivot loop uses no cross-lane shuffles.
+  auto key_of = [](double v) { return (unsigned long long)__double_as_longlong(v); };
+  auto val_of = [](unsigned long long k2) { return __longlong_as_double((long long)k2); };
+  int k = 0;
+  while (k < w) {
+    // ---- pivot search (column k, candidates k..w-1)
+    if (lane == 0) {
+      s_key[0] = 0ull;
+      s_key[1] = 0ull;
+      s_idx = 64;
+    }
+    __syncthreads();
+    const double absakk = fabs((double)F[k + k * kBkLd]);
+    const double v = (lane > k && lane < w) ? fabs((double)F[lane + k * kBkLd]) : 0.0;
+    if (lane > k && lane < w) atomicMax(&s_key[0], key_of(v));
+    __syncthreads();
+    const double colmax = val_of(s_key[0]);
+    if (lane > k && lane < w && v == colmax && colmax > 0.0) atomicMin(&s_idx, lane);
+    __syncthreads();
+    const int imax = colmax > 0.0 ? s_idx : -1;
+    int kind = 1, kp = k;
+    if (!(absakk >= kBkAlpha * colmax) && imax > k) {
+      if (lane >= k && lane < w && lane != imax) {
+        const double r = lane < imax ? fabs((double)F[imax + lane * kBkLd]) : fabs((double)F[lane + imax * kBkLd]);
+        atomicMax(&s_key[1], key_of(r));
+      }
+      __syncthreads();
+      const double rowmax = val_of(s_key[1]);
+      if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
+        kind = 1;
+        kp = k;
+      } else if (fabs((double)F[imax + imax * kBkLd]) >= kBkAlpha * rowmax) {
+        kind = 1;
+        kp = imax;
+      } else {
+        kind = 2;
+        kp = imax;
+      }
+    }
+    const int kk = kind == 1 ? k : k + 1;
+    if (kp != kk) {
+      const int pp = kk, q = kp;
+      if (lane < pp) {  // rows p, q of the columns before p
+        const T a = F[pp + lane * kBkLd];
+        F[pp + lane * kBkLd] = F[q + lane * kBkLd];
+        F[q + lane * kBkLd] = a;
+      } else if (lane > pp && lane < q) {  // A(j, p) <-> A(q, j)
+        const T a = F[lane + pp * kBkLd];
+        F[lane + pp * kBkLd] = F[q + lane * kBkLd];
+        F[q + lane * kBkLd] = a;
+      } else if (lane > q && lane < w) {  // A(i, p) <-> A(i, q)
+        const T a = F[lane + pp * kBkLd];
+        F[lane + pp * kBkLd] = F[lane + q * kBkLd];
+        F[lane + q * kBkLd] = a;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const T a = F[pp + pp * kBkLd];
+        F[pp + pp * kBkLd] = F[q + q * kBkLd];
+        F[q + q * kBkLd] = a;
+        const int t = perm[pp];
+        perm[pp] = perm[q];
+        perm[q] = t;
+      }
+      __syncthreads();
+    }
+    // ---- remaining column maxima for the null-pivot test (rows of the block below the pivot,
+    //      then lazily the panel rows below the block)
+    const int s0 = kind == 1 ? k + 1 : k + 2;
+    if (lane == 0) s_key[2] = 0ull;
+    __syncthreads();
+    if (lane >= s0 && lane < w) {
+      double mv = fabs((double)F[lane + k * kBkLd]);
+      if (kind == 2) mv = fmax(mv, fabs((double)F[lane + (k + 1) * kBkLd]));
+      atomicMax(&s_key[2], key_of(mv));
+    }
+    __syncthreads();
+    const double m01 = val_of(s_key[2]);
+    if (kind == 1) {
+      const double d = (double)F[k + k * kBkLd];
+      bool null_piv = d == 0.0;
+      if (det && fabs(d) <= tau && m01 <= tau) {
+        double mb = 0.0;
+        const int jc = perm[k];
+        for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs((double)P[(int64_t)(c0 + jc) * m + r2]));
+        for (int off = 32; off > 0; off >>= 1) mb = fmax(mb, __shfl_xor(mb, off));
+        if (mb <= tau) null_piv = true;
+      }
+      const T inv = null_piv ? (T)0 : (T)(1.0 / d);
+      if (lane == 0) {
+        if (d == 0.0) ++exz;
+        if (null_piv) ++zero;
+        else if (d < 0) ++neg;
+        else ++pos;
+      }
+      if (lane == k) {
+        my_pt = null_piv ? kPivNull : kPiv1;
+        my_dv = (T)d;
+        my_dof = (T)0;
+      }
+      const T wi = (lane > k && lane < w) ? F[lane + k * kBkLd] : (T)0;
+      // the multipliers replace column k first; the update then reads them by broadcast
+      if (lane > k && lane < w) F[lane + k * kBkLd] = wi * inv;
+      __syncthreads();
+      if (inv != (T)0) {
+        for (int j = k + 1; j < w; ++j) {
+          const T lj = F[j + k * kBkLd];
+          if (lane >= j && lane < w) F[lane + j * kBkLd] -= wi * lj;
+        }
+      }
+      __syncthreads();
+      k += 1;
+    } else {
+      const double a = (double)F[k + k * kBkLd], b = (double)F[k + 1 + k * kBkLd], c = (double)F[k + 1 + (k + 1) * kBkLd];
+      const double dt = a * c - b * b;
+      const double mean = 0.5 * (a + c), rad = sqrt(0.25 * (a - c) * (a - c) + b * b);
+      const double e1 = mean - rad, e2 = mean + rad;
+      const bool coupled = m01 > tau;
+      int nz = 0, nn = 0, np = 0;
+      for (double e : {e1, e2}) {
+        const bool z = (det && fabs(e) <= tau && !coupled) || e == 0.0;
+        if (z) ++nz;
+        else if (e < 0) ++nn;
+        else ++np;
+      }
+      const bool dead = nz > 0 || dt == 0.0;
+      const T i11 = dead ? (T)0 : (T)(c / dt), i12 = dead ? (T)0 : (T)(-b / dt), i22 = dead ? (T)0 : (T)(a / dt);
+      if (lane == 0) {
+        if (dt == 0.0) ++exz;
+        zero += nz;
+        neg += nn;
+        pos += np;
+        ++two;
+      }
+      if (lane == k) {
+        my_pt = dead ? kPivNull : kPiv2First;
+        my_dv = (T)a;
+        my_dof = (T)b;
+      }
+      if (lane == k + 1) {
+        my_pt = dead ? kPivNull : kPiv2Second;
+        my_dv = (T)c;
+        my_dof = (T)0;
+      }
+      const T w1 = (lane > k + 1 && lane < w) ? F[lane + k * kBkLd] : (T)0;
+      const T w2 = (lane > k + 1 && lane < w) ? F[lane + (k + 1) * kBkLd] : (T)0;
+      const T l1 = w1 * i11 + w2 * i12, l2 = w1 * i12 + w2 * i22;
+      if (lane > k + 1 && lane < w) {
+        F[lane + k * kBkLd] = l1;
+        F[lane + (k + 1) * kBkLd] = l2;
+      }
+      __syncthreads();
+      if (!dead) {
+        for (int j = k + 2; j < w; ++j) {
+          const T l1j = F[j + k * kBkLd], l2j = F[j + (k + 1) * kBkLd];
+          if (lane >= j && lane < w) F[lane + j * kBkLd] -= w1 * l1j + w2 * l2j;
+        }
+      }
+      __syncthreads();
+      k += 2;
+    }
   }
-  // column maxima of the panel rows below the block (the null-pivot test's remaining rows)
-  {
-    const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    for (int j = wave; j < w; j += nw) {
-      double mx = 0.0;
-      for (int r = c0 + w + lane; r < m; r += 64) mx = fmax(mx, fabs((double)P[(int64_t)(c0 + j) * m + r]));
-      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-      if (lane == 0) below[j] = mx;
+  // ---- write back L11 and D, pivot data
+  for (int j = 0; j < w; ++j)
+    if (lane >= j && lane < w) P[(int64_t)(c0 + j) * m + c0 + lane] = F[lane + j * kBkLd];
+  if (lane < w) {
+    N.dval[first + c0 + lane] = my_dv;
+    N.doff[first + c0 + lane] = my_dof;
+    N.ptype[first + c0 + lane] = my_pt;
+    N.piv[first + c0 + lane] = c0 + perm[lane];
+    N.ipiv[first + c0 + perm[lane]] = c0 + lane;
+    bperm[(int64_t)f * nb + lane] = perm[lane];
+  }
+  // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L): lane j
+  //      forms column j in LDS (rows of F already final), then the block is stored
+  __shared__ T Xi[64 * kBkLd];
+  __shared__ int8_t ptl[64];
+  if (lane < w) ptl[lane] = my_pt;
+  __syncthreads();
+  if (lane < w) {
+    const int j = lane;
+    for (int i = 0; i < w; ++i) Xi[i + j * kBkLd] = i == j ? (T)1 : (T)0;
+    for (int i = j + 1; i < w; ++i) {
+      T v2 = (T)0;
+      for (int t = j; t < i; ++t) {
+        const bool skip = (i == t + 1) && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
+        if (!skip) v2 -= F[i + t * kBkLd] * Xi[t + j * kBkLd];
+      }
+      Xi[i + j * kBkLd] = v2;
     }
   }
   __syncthreads();
-  const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
-  PivotCounts pc{0, 0, 0, 0, 0};
-  bk_factor_lds<T>(F, w, w, w, perm, dv, dof, pt, t1, t2, tau, detect != 0, pc, below);
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int i = idx % w, j = idx / w;
-    if (i >= j) P[(int64_t)(c0 + j) * m + c0 + i] = F[i + j * w];
+  T* out = Linv + (int64_t)f * nb * nb;
+  for (int j = 0; j < w; ++j)
+    if (lane < w) out[(int64_t)j * w + lane] = Xi[lane + j * kBkLd];
+  if (lane == 0) {
+    unsigned long long* cnt = N.cnt;
+    if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
+    if (zero) atomicAdd(&cnt[kCntZero], (unsigned long long)zero);
+    if (pos) atomicAdd(&cnt[kCntPos], (unsigned long long)pos);
+    if (two) atomicAdd(&cnt[kCnt2x2], (unsigned long long)two);
+    if (exz) atomicAdd(&cnt[kCntExactZero], (unsigned long long)exz);
   }
-  for (int i = tid; i < w; i += blockDim.x) {
-    N.dval[first + c0 + i] = dv[i];
-    N.doff[first + c0 + i] = dof[i];
-    N.ptype[first + c0 + i] = pt[i];
-    N.piv[first + c0 + i] = c0 + perm[i];
-    N.ipiv[first + c0 + perm[i]] = c0 + i;
-    bperm[(int64_t)f * nb + i] = perm[i];
-  }
-  (void)m;
-  flush_counts<T>(pc, N.cnt);
 }
 
 constexpr int kPanelRows = 64;  // rows per panel-solve workgroup
@@ -475,7 +679,8 @@ constexpr int kPanelRows = 64;  // rows per panel-solve workgroup
 // [c0, c0 + w) of the earlier panel columns [0, c0).
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                         int p, int nb, const int32_t* __restrict__ bperm) {
+                                                         int p, int nb, const int32_t* __restrict__ bperm,
+                                                         const T* __restrict__ Linv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int f = blockIdx.y;
   const int s = list[f];
@@ -505,20 +710,14 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
   }
   const int r0 = c1 + blockIdx.x * kPanelRows;
   const int rn = min(kPanelRows, m - r0);
-  T* L11 = reinterpret_cast<T*>(smem_raw);  // w x w
-  T* X = L11 + nb * nb;                      // rn x w, ld kPanelRows
+  T* Li = reinterpret_cast<T*>(smem_raw);   // w x w inverse of L11, column-major
+  T* X = Li + nb * nb;                       // rn x w, ld kPanelRows
   __shared__ T dinv[3 * 64];
   __shared__ int8_t ptl[64];
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int i = idx % w, j = idx / w;
-    T v = i > j ? P[(int64_t)(c0 + j) * m + c0 + i] : (T)0;
-    L11[idx] = v;
-  }
+  const T* lsrc = Linv + (int64_t)f * nb * nb;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) Li[idx] = lsrc[idx];
   for (int i = tid; i < w; i += blockDim.x) ptl[i] = N.ptype[first + c0 + i];
   __syncthreads();
-  // the (j+1, j) entry of a 2x2 pivot is D's off-diagonal, not part of L11
-  for (int j = tid; j < w; j += blockDim.x)
-    if ((ptl[j] == kPiv2First || ptl[j] == kPivNull) && j + 1 < w) L11[(j + 1) + j * w] = (T)0;
   if (tid < w) {
     const int j = tid;
     const int8_t t = ptl[j];
@@ -542,35 +741,44 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
     X[r + j * kPanelRows] = P[(int64_t)(c0 + bp[j]) * m + r0 + r];
   }
   __syncthreads();
-  // W = A21 L11^-T, right-looking over the w columns
-  for (int j = 0; j < w; ++j) {
-    const int nt = w - j - 1;
-    for (int idx = tid; idx < rn * nt; idx += blockDim.x) {
-      const int r = idx % rn, t = j + 1 + idx / rn;
-      X[r + t * kPanelRows] -= X[r + j * kPanelRows] * L11[t + j * w];
-    }
-    __syncthreads();
+  // W = A21 inv(L11)^T: W(r, j) = sum_{t <= j} A21(r, t) inv(L11)(j, t); 4 groups of columns
+  T Wr[16];
+  const int r = tid % kPanelRows, jg = tid / kPanelRows;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int j = jg + 4 * q;
+    T acc = (T)0;
+    if (j < w && r < rn)
+      for (int t = 0; t <= j; ++t) acc += X[r + t * kPanelRows] * Li[j + t * w];
+    Wr[q] = acc;
   }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int j = jg + 4 * q;
+    if (j < w && r < rn) X[r + j * kPanelRows] = Wr[q];
+  }
+  __syncthreads();
   T* Wp = N.W + S.w_off[s];
   for (int idx = tid; idx < rn * w; idx += blockDim.x) {
-    const int r = idx % rn, j = idx / rn;
-    Wp[(int64_t)(c0 + j) * m + r0 + r] = X[r + j * kPanelRows];
+    const int rr = idx % rn, j = idx / rn;
+    Wp[(int64_t)(c0 + j) * m + r0 + rr] = X[rr + j * kPanelRows];
   }
   // L21 = W D^-1
   for (int idx = tid; idx < rn * w; idx += blockDim.x) {
-    const int r = idx % rn, j = idx / rn;
+    const int rr = idx % rn, j = idx / rn;
     const int8_t t = ptl[j];
     T v;
     if (t == kPiv1) {
-      v = X[r + j * kPanelRows] * dinv[3 * j];
+      v = X[rr + j * kPanelRows] * dinv[3 * j];
     } else if (t == kPiv2First) {
-      v = X[r + j * kPanelRows] * dinv[3 * j] + X[r + (j + 1) * kPanelRows] * dinv[3 * j + 1];
+      v = X[rr + j * kPanelRows] * dinv[3 * j] + X[rr + (j + 1) * kPanelRows] * dinv[3 * j + 1];
     } else if (t == kPiv2Second) {
-      v = X[r + (j - 1) * kPanelRows] * dinv[3 * (j - 1) + 1] + X[r + j * kPanelRows] * dinv[3 * (j - 1) + 2];
+      v = X[rr + (j - 1) * kPanelRows] * dinv[3 * (j - 1) + 1] + X[rr + j * kPanelRows] * dinv[3 * (j - 1) + 2];
     } else {
       v = (T)0;
     }
-    P[(int64_t)(c0 + j) * m + r0 + r] = v;
+    P[(int64_t)(c0 + j) * m + r0 + rr] = v;
   }
 }
 
@@ -662,164 +870,6 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
     }
 }
 
-// ------------------------------------------------------------------------------------ solves
-//
-// Multifrontal forward / backward substitution with p right-hand sides (the reference's MUMPS
-// JOB = 3, src/factor/mumps_factorization.cpp:458-527). Forward (levels bottom-up): a front
-// gathers its own right-hand-side rows (pivot order), adds its children's update vectors, solves
-// with the unit lower L11 and passes U = v_cb - L21 y up; the pivot-order intermediate lives in Z
-// (n x p). D^-1 is applied per pivot block. Backward (levels top-down): a front subtracts
-// L21^T x_cb (ancestor rows, already final), solves with L11^T and scatters its rows into X.
-// R is the right-hand-side type (double), T the factor type.
-
-// L(i, j) of the pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair.
-template <class T>
-__device__ __forceinline__ T l11_at(const T* P, int m, int i, int j, const int8_t* pt) {
-  if (i == j + 1 && (pt[j] == kPiv2First || pt[j] == kPivNull)) return (T)0;
-  return P[(int64_t)j * m + i];
-}
-
-template <class T, class R>
-__global__ void __launch_bounds__(kThreads) k_solve_forward(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                           const R* __restrict__ X, int64_t ldx, int p, T* Z,
-                                                           T* U, int64_t ldu) {
-  const int s = list[blockIdx.x];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int cb = m - ncol;
-  const int n = S.n;
-  const int64_t roff = S.sn_rowoff[s];
-  const int32_t* rows = S.rows_orig + roff;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* zs = Z + first;
-  T* us = U + roff + ncol;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < ncol * p; idx += blockDim.x) {
-    const int i = idx % ncol, c = idx / ncol;
-    zs[i + (int64_t)c * n] = (T)X[rows[N.piv[first + i]] + (int64_t)c * ldx];
-  }
-  for (int idx = tid; idx < cb * p; idx += blockDim.x) {
-    const int i = idx % cb, c = idx / cb;
-    us[i + (int64_t)c * ldu] = (T)0;
-  }
-  __syncthreads();
-  for (int q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
-    const int ch = S.child_list[q];
-    const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
-    const int64_t coff = S.sn_rowoff[ch] + S.sn_ncol[ch];
-    const int32_t* rel = S.relmap + coff;
-    const T* uc = U + coff;
-    for (int idx = tid; idx < cbc * p; idx += blockDim.x) {
-      const int i = idx % cbc, c = idx / cbc;
-      const int loc = rel[i];
-      const T v = uc[i + (int64_t)c * ldu];
-      if (loc < ncol) zs[N.ipiv[first + loc] + (int64_t)c * n] += v;
-      else us[loc - ncol + (int64_t)c * ldu] += v;
-    }
-    __syncthreads();
-  }
-  // L11 y = v, column by column
-  for (int j = 0; j < ncol - 1; ++j) {
-    const int nt = ncol - j - 1;
-    for (int idx = tid; idx < nt * p; idx += blockDim.x) {
-      const int i = j + 1 + idx % nt, c = idx / nt;
-      const T l = l11_at(P, m, i, j, pt);
-      if (l != (T)0) zs[i + (int64_t)c * n] -= l * zs[j + (int64_t)c * n];
-    }
-    __syncthreads();
-  }
-  // U = v_cb - L21 y
-  for (int idx = tid; idx < cb * p; idx += blockDim.x) {
-    const int i = idx % cb, c = idx / cb;
-    T acc = (T)0;
-    for (int j = 0; j < ncol; ++j) acc += P[(int64_t)j * m + ncol + i] * zs[j + (int64_t)c * n];
-    us[i + (int64_t)c * ldu] -= acc;
-  }
-}
-
-template <class T>
-__global__ void k_solve_diag(int n, const T* __restrict__ dval, const T* __restrict__ doff,
-                             const int8_t* __restrict__ ptype, T* Z, int p) {
-  const int64_t total = (int64_t)n * p;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % n);
-    const int64_t c = idx / n;
-    const int8_t t = ptype[i];
-    T* z = Z + c * n;
-    if (t == kPiv1) {
-      z[i] = z[i] / dval[i];
-    } else if (t == kPiv2First) {
-      const T a = dval[i], b = doff[i], d = dval[i + 1];
-      const T det = a * d - b * b;
-      const T z1 = z[i], z2 = z[i + 1];
-      z[i] = (d * z1 - b * z2) / det;
-      z[i + 1] = (a * z2 - b * z1) / det;
-    } else if (t == kPivNull) {
-      z[i] = (T)0;
-    }
-  }
-}
-
-template <class T, class R>
-__global__ void __launch_bounds__(kThreads) k_solve_backward(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                            R* X, int64_t ldx, int p, T* Z) {
-  __shared__ T red[kThreads / 64][16];
-  const int s = list[blockIdx.x];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int cb = m - ncol;
-  const int n = S.n;
-  const int64_t roff = S.sn_rowoff[s];
-  const int32_t* rows = S.rows_orig + roff;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* zs = Z + first;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-  // y_j -= sum_i L21(i, j) x_cb(i): one wave per column
-  if (cb > 0) {
-    for (int j = wave; j < ncol; j += nw) {
-      for (int c = 0; c < p; ++c) {
-        T acc = (T)0;
-        for (int i = lane; i < cb; i += 64)
-          acc += P[(int64_t)j * m + ncol + i] * (T)X[rows[ncol + i] + (int64_t)c * ldx];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) zs[j + (int64_t)c * n] -= acc;
-      }
-    }
-    __syncthreads();
-  }
-  // L11^T x = y, from the last pivot up: x_j = y_j - sum_{i > j} L(i, j) x_i
-  for (int j = ncol - 2; j >= 0; --j) {
-    const int nt = ncol - j - 1;
-    for (int c0 = 0; c0 < p; c0 += 16) {
-      const int pc = min(16, p - c0);
-      T acc[16];
-      for (int c = 0; c < pc; ++c) acc[c] = (T)0;
-      for (int t = tid; t < nt; t += blockDim.x) {
-        const int i = j + 1 + t;
-        const T l = l11_at(P, m, i, j, pt);
-        for (int c = 0; c < pc; ++c) acc[c] += l * zs[i + (int64_t)(c0 + c) * n];
-      }
-      for (int c = 0; c < pc; ++c) {
-        T v = acc[c];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) red[wave][c] = v;
-      }
-      __syncthreads();
-      if (tid < pc) {
-        T v = (T)0;
-        for (int w = 0; w < nw; ++w) v += red[w][tid];
-        zs[j + (int64_t)(c0 + tid) * n] -= v;
-      }
-      __syncthreads();
-    }
-  }
-  for (int idx = tid; idx < ncol * p; idx += blockDim.x) {
-    const int i = idx % ncol, c = idx / ncol;
-    X[rows[N.piv[first + i]] + (int64_t)c * ldx] = (R)zs[i + (int64_t)c * n];
-  }
-}
-
 // ------------------------------------------------------------------------------ host engine
 
 void hip_check(hipError_t e, const char* what) {
@@ -881,6 +931,11 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
         L.large_max_cb = std::max(L.large_max_cb, sym_.sn_m[s] - sym_.sn_ncol[s]);
       }
     }
+    for (int s2 : L.h_all) {
+      L.max_m = std::max(L.max_m, sym_.sn_m[s2]);
+      L.max_cb = std::max(L.max_cb, sym_.sn_m[s2] - sym_.sn_ncol[s2]);
+      L.nblk = std::max(L.nblk, (sym_.sn_ncol[s2] + kSolveBlock - 1) / kSolveBlock);
+    }
     L.n_all = (int)L.h_all.size();
     L.n_small = (int)L.h_small.size();
     L.n_large = (int)L.h_large.size();
@@ -888,6 +943,17 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
     n_large_ += L.n_large;
     max_large_per_level_ = std::max(max_large_per_level_, L.n_large);
   }
+  h_linv_off_.assign(nsn, 0);
+  for (int s2 = 0; s2 < nsn; ++s2) {
+    h_linv_off_[s2] = linv_total_;
+    const int nb2 = (sym_.sn_ncol[s2] + kSolveBlock - 1) / kSolveBlock;
+    for (int b = 0; b < nb2; ++b) {
+      h_pairs_.push_back(s2);
+      h_pairs_.push_back(b);
+    }
+    linv_total_ += (int64_t)nb2 * kSolveBlock * kSolveBlock;
+  }
+  n_pairs_ = (int)(h_pairs_.size() / 2);
 }
 
 void Engine::ensure_device() {
@@ -908,6 +974,8 @@ void Engine::ensure_device() {
   d_asm_ = upload(h_asm_);
   d_ia_ = upload(h_ia_);
   d_ib_ = upload(h_ib_);
+  d_linv_off_ = upload(h_linv_off_);
+  d_pairs_ = upload(h_pairs_);
   for (Level& L : levels_) {
     L.d_all = upload(L.h_all);
     if (L.n_small) L.d_small = upload(L.h_small);
@@ -920,7 +988,7 @@ Engine::~Engine() {
   free_numeric();
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
-                  (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_})
+                  (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_linv_off_, (void*)d_pairs_})
     dfree(p);
   for (auto& L : levels_) {
     dfree(L.d_all);
@@ -931,9 +999,9 @@ Engine::~Engine() {
 
 void Engine::free_numeric() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
-                  (void*)d_bperm_, (void*)d_cnt_, d_Z_, d_U_})
+                  (void*)d_bperm_, (void*)d_cnt_, d_Z_, d_Y_, d_U_, d_Linv_, d_pinv_})
     dfree(p);
-  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Z_ = d_U_ = nullptr;
+  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Z_ = d_Y_ = d_U_ = d_Linv_ = d_pinv_ = nullptr;
   d_ptype_ = nullptr;
   d_piv_ = d_ipiv_ = d_bperm_ = nullptr;
   d_cnt_ = nullptr;
@@ -960,6 +1028,9 @@ void Engine::alloc_numeric(int precision) {
   hip_check(hipMalloc(&d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t)),
             "hipMalloc (pivots)");
   hip_check(hipMalloc(&d_cnt_, kCntSlots * sizeof(unsigned long long)), "hipMalloc (counters)");
+  hip_check(hipMalloc(&d_Linv_, std::max<int64_t>(1, linv_total_) * ts), "hipMalloc (diagonal block inverses)");
+  hip_check(hipMalloc(&d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts),
+            "hipMalloc (panel inverses)");
   precision_ = precision;
 }
 
@@ -995,10 +1066,10 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
-        hipLaunchKernelGGL(k_bk_diag<T>, dim3(L.n_large), dim3(kThreads), shm_bk, st, S, N, L.d_large, p, nb, d_bperm_,
-                           null_tol, detect);
+        hipLaunchKernelGGL(k_bk_diag<T>, dim3(L.n_large), dim3(64), 0, st, S, N, L.d_large, p, nb, d_bperm_, null_tol,
+                           detect, (T*)d_pinv_);
         hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
-                           nb, (const int32_t*)d_bperm_);
+                           nb, (const int32_t*)d_bperm_, (const T*)d_pinv_);
         const int rem = L.large_maxm - (p + 1) * nb;
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
@@ -1013,6 +1084,8 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
       }
     }
   }
+  if (n_pairs_ > 0)
+    hipLaunchKernelGGL(k_block_inverse<T>, dim3(n_pairs_), dim3(256), 0, st, S, N, d_pairs_, d_linv_off_, (T*)d_Linv_);
   hip_check(hipGetLastError(), "factorization kernels");
   unsigned long long h[kCntSlots];
   hip_check(hipMemcpyAsync(h, d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
@@ -1046,25 +1119,46 @@ void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
   if (ws_cols_ < nrhs) {
     dfree(d_Z_);
+    dfree(d_Y_);
     dfree(d_U_);
-    d_Z_ = d_U_ = nullptr;
+    d_Z_ = d_Y_ = d_U_ = nullptr;
     hip_check(hipMalloc(&d_Z_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
+    hip_check(hipMalloc(&d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
     hip_check(hipMalloc(&d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
               "hipMalloc (solve workspace)");
     ws_cols_ = nrhs;
   }
   T* Z = (T*)d_Z_;
+  T* Y = (T*)d_Y_;
   T* U = (T*)d_U_;
+  const T* Linv = (const T*)d_Linv_;
   const int p = (int)nrhs;
-  for (const Level& L : levels_)
-    hipLaunchKernelGGL((k_solve_forward<T, double>), dim3(L.n_all), dim3(kThreads), 0, st, S, N, L.d_all,
-                       (const double*)x, ldx, p, Z, U, rows_total_);
+  // forward: Z (accumulated) -> Y (final L^-1 rows, pivot order)
+  for (const Level& L : levels_) {
+    hipLaunchKernelGGL((k_fwd_gather<T, double>), dim3(L.n_all), dim3(256), 0, st, S, N, L.d_all, (const double*)x,
+                       ldx, p, Z, U, rows_total_);
+    for (int b = 0; b < L.nblk; ++b) {
+      const int below = std::max(1, L.max_m - b * kSolveBlock);
+      hipLaunchKernelGGL(k_fwd_step<T>, dim3((below + kSolveRows - 1) / kSolveRows, L.n_all), dim3(256), 0, st, S, N,
+                         L.d_all, b, d_linv_off_, Linv, p, Z, Y, U, rows_total_);
+    }
+  }
   hipLaunchKernelGGL(k_solve_diag<T>, dim3(blocks_for((int64_t)sym_.n * p)), dim3(256), 0, st, sym_.n, N.dval,
-                     N.doff, N.ptype, Z, p);
+                     N.doff, N.ptype, Y, p);
+  // backward: Y (accumulated) -> Z (final rows, pivot order) -> X
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
-    hipLaunchKernelGGL((k_solve_backward<T, double>), dim3(L.n_all), dim3(kThreads), 0, st, S, N, L.d_all, x, ldx, p,
-                       Z);
+    if (L.max_cb > 0) {
+      hipLaunchKernelGGL((k_bwd_gather<T, double>), dim3(L.n_all), dim3(256), 0, st, S, L.d_all, (const double*)x,
+                         ldx, p, U, rows_total_);
+      hipLaunchKernelGGL(k_bwd_cb<T>, dim3(L.nblk, L.n_all), dim3(256), 0, st, S, N, L.d_all, p, Y, (const T*)U,
+                         rows_total_);
+    }
+    for (int b = L.nblk - 1; b >= 0; --b)
+      hipLaunchKernelGGL(k_bwd_step<T>, dim3(1 + b, L.n_all), dim3(256), 0, st, S, N, L.d_all, b, d_linv_off_, Linv, p,
+                         Y, Z);
+    hipLaunchKernelGGL((k_bwd_scatter<T, double>), dim3(L.n_all), dim3(256), 0, st, S, N, L.d_all, x, ldx, p,
+                       (const T*)Z);
   }
   hip_check(hipGetLastError(), "solve kernels");
 }

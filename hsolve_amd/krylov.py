@@ -215,7 +215,7 @@ class BlockLanczos:
                 if d.min() <= 1e-7 * d.max():  # too ill-conditioned for Cholesky-QR
                     ok = False
                     break
-                Rinv = sla.solve_triangular(Rk, np.eye(p), lower=False)
+                Rinv = np.linalg.inv(Rk)
                 tmp = ops.empty(p)
                 ops.update(w, torch.from_numpy(np.ascontiguousarray(Rinv.T)).to(ops.dev), tmp, 1.0, 0.0)
                 w.copy_(tmp)
