@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../host/analysis.hpp"
+#include "solve.hpp"
 
 namespace hsg {
 
@@ -57,26 +58,40 @@ class Engine {
     int large_maxm = 0;
     int large_max_ncol = 0;
     int large_max_cb = 0;
-    int max_m = 0, max_cb = 0, nblk = 0;  // over all fronts of the level (solve launches)
+    int max_m = 0, max_ncol = 0;  // over all fronts of the level (solve launches)
+    int fwd_tiles = 0, fwd_chunks = 0, bwd_tiles = 0, bwd_chunks = 0;  // split GEMV grids
+  };
+  struct TaskRange {
+    int64_t begin = 0, count = 0;
   };
   template <class T>
   InertiaResult factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
                             hipStream_t stream);
   template <class T>
   void solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
+  template <class T>
+  void build_inverse_panels(hipStream_t stream);  // G and Gt from L (first solve after a factorization)
+  void plan_solve();                              // host task lists of the solve phase (symbolic)
+  void validate_solve_plan() const;
   void alloc_numeric(int precision);
   void ensure_device();  // uploads the analysis on first use (the analysis itself is host-only)
   void free_numeric();
 
   Symbolic sym_;
   EngineOptions opt_;
-  std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_, h_linv_off_;
-  std::vector<int32_t> h_pairs_;
-  int64_t linv_total_ = 0;
-  int n_pairs_ = 0;
-  int64_t* d_linv_off_ = nullptr;
-  int32_t* d_pairs_ = nullptr;
-  void* d_Linv_ = nullptr;
+  std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_;
+  // solve phase plan: diagonal 64-block pairs, tile-product stages building G, transpose tiles,
+  // and the pull map of the forward gather (per front row, the children's U rows mapped onto it)
+  std::vector<int32_t> h_pairs_, h_ttiles_;
+  std::vector<GemmTask> h_tasks_;
+  std::vector<TaskRange> gstages_;
+  std::vector<int64_t> h_cptr_;
+  std::vector<int32_t> h_cidx_;
+  int64_t t_total_ = 0;
+  int n_pairs_ = 0, n_ttiles_ = 0;
+  int32_t *d_pairs_ = nullptr, *d_ttiles_ = nullptr, *d_cidx_ = nullptr;
+  int64_t* d_cptr_ = nullptr;
+  GemmTask* d_tasks_ = nullptr;
   void* d_pinv_ = nullptr;  // panel inverses of the large fronts of one level (panel solve)
   bool on_device_ = false;
   int64_t nnz_ = 0;
@@ -98,6 +113,11 @@ class Engine {
   unsigned long long* d_cnt_ = nullptr;
   void *d_Z_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;
   int64_t ws_cols_ = 0;
+  void *d_G_ = nullptr, *d_Gt_ = nullptr, *d_T_ = nullptr;  // inverse panels (solve phase)
+  void* d_part_ = nullptr;      // partial sums of split GEMV tiles
+  int* d_gv_count_ = nullptr;   // arrival counters of split GEMV tiles (kept at 0 between uses)
+  int64_t part_slots_ = 0, gv_counters_ = 0;  // per group of kMaxRhsChunk right-hand sides
+  bool g_ready_ = false;
 };
 
 // Throws std::runtime_error with the HIP error string on failure.

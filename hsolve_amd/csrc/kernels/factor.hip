@@ -435,10 +435,44 @@ __global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<
   }
 }
 
+// Maximum over the 64 lanes of a wave of a non-negative key (bit pattern of a non-negative
+// double: integer order = value order), returned to every lane. DPP reduction: row_shr 1, 2, 4,
+// 8 leave each 16-lane row's maximum in its lane 15, row_bcast 15 / 31 carry them to lane 63.
+// Lanes a DPP step does not write read 0, the neutral element for non-negative keys.
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long v) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+#define HSG_DPP_MAX_STEP(ctrl, rmask)                                                          \
+  {                                                                                           \
+    const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rmask, 0xf, true); \
+    const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rmask, 0xf, true); \
+    const unsigned long long o = ((unsigned long long)ohi << 32) | olo;                      \
+    const unsigned long long c = ((unsigned long long)hi << 32) | lo;                        \
+    if (o > c) {                                                                              \
+      lo = olo;                                                                               \
+      hi = ohi;                                                                               \
+    }                                                                                         \
+  }
+  HSG_DPP_MAX_STEP(0x111, 0xf)  // row_shr:1
+  HSG_DPP_MAX_STEP(0x112, 0xf)  // row_shr:2
+  HSG_DPP_MAX_STEP(0x114, 0xf)  // row_shr:4
+  HSG_DPP_MAX_STEP(0x118, 0xf)  // row_shr:8
+  HSG_DPP_MAX_STEP(0x142, 0xa)  // row_bcast:15 into rows 1, 3
+  HSG_DPP_MAX_STEP(0x143, 0xc)  // row_bcast:31 into rows 2, 3
+#undef HSG_DPP_MAX_STEP
+  lo = (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
+  hi = (unsigned)__builtin_amdgcn_readlane((int)hi, 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double wave_max_abs(double v) {
+  return __longlong_as_double((long long)wave_max_key((unsigned long long)__double_as_longlong(fabs(v))));
+}
+
 // Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, one 64-lane wave per
-// front: lane i owns row i of the block (LDS, padded leading dimension), pivot searches are wave
-// reductions, the multipliers of a rank-1 / rank-2 update are broadcast by shuffles. Also forms
-// the inverse of the block's unit lower L11 (per-front scratch, read by the panel solve).
+// front: lane i owns row i of the block (LDS, padded leading dimension), pivot searches are DPP
+// wave reductions (argmax: lowest lane holding the maximum, by ballot), the multipliers of a
+// rank-1 / rank-2 update are read from arrays of their own. Also forms the inverse of the block's
+// unit lower L11 (per-front scratch, read by the panel solve).
 constexpr int kBkLd = 65;
 
 template <class T>
@@ -447,8 +481,9 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
                                                 T* __restrict__ Linv) {
   __shared__ T F[64 * kBkLd];
   __shared__ int perm[64];
-  __shared__ unsigned long long s_key[3];  // maxima as bit patterns of non-negative doubles
-  __shared__ int s_idx;
+  // multipliers of the current 1x1 / 2x2 step in arrays of their own: the update loop's loads of
+  // them can then be scheduled ahead of its stores to F
+  __shared__ T s_l1[64], s_l2[64];
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -457,7 +492,15 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
   const int w = min(nb, ncol - c0);
   T* P = N.L + S.panel_off[s];
   const int lane = threadIdx.x;
-  for (int j = 0; j < w; ++j) F[lane + j * kBkLd] = (lane >= j && lane < w) ? P[(int64_t)(c0 + j) * m + c0 + lane] : (T)0;
+  {
+    // branch-free loads (row index clamped into the block) so that they are issued together
+    const T* src = P + (int64_t)c0 * m + c0 + min(lane, w - 1);
+#pragma unroll 16
+    for (int j = 0; j < w; ++j) {
+      const T v = src[(int64_t)j * m];
+      F[lane + j * kBkLd] = (lane >= j && lane < w) ? v : (T)0;
+    }
+  }
   perm[lane] = lane;
   __syncthreads();
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
@@ -465,35 +508,20 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
   int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // lane 0's tallies
   int8_t my_pt = kPiv1;
   T my_dv = (T)0, my_dof = (T)0;
-  // Wave maxima through LDS atomics on the bit patterns of non-negative doubles (integer order =
-  // value order); the pivot loop uses no cross-lane shuffles.
-  auto key_of = [](double v) { return (unsigned long long)__double_as_longlong(v); };
-  auto val_of = [](unsigned long long k2) { return __longlong_as_double((long long)k2); };
   int k = 0;
   while (k < w) {
     // ---- pivot search (column k, candidates k..w-1)
-    if (lane == 0) {
-      s_key[0] = 0ull;
-      s_key[1] = 0ull;
-      s_idx = 64;
-    }
-    __syncthreads();
     const double absakk = fabs((double)F[k + k * kBkLd]);
     const double v = (lane > k && lane < w) ? fabs((double)F[lane + k * kBkLd]) : 0.0;
-    if (lane > k && lane < w) atomicMax(&s_key[0], key_of(v));
-    __syncthreads();
-    const double colmax = val_of(s_key[0]);
-    if (lane > k && lane < w && v == colmax && colmax > 0.0) atomicMin(&s_idx, lane);
-    __syncthreads();
-    const int imax = colmax > 0.0 ? s_idx : -1;
+    const double colmax = wave_max_abs(v);
+    const unsigned long long at_max = __ballot(lane > k && lane < w && v == colmax);
+    const int imax = (colmax > 0.0 && at_max) ? __ffsll((long long)at_max) - 1 : -1;
     int kind = 1, kp = k;
     if (!(absakk >= kBkAlpha * colmax) && imax > k) {
-      if (lane >= k && lane < w && lane != imax) {
-        const double r = lane < imax ? fabs((double)F[imax + lane * kBkLd]) : fabs((double)F[lane + imax * kBkLd]);
-        atomicMax(&s_key[1], key_of(r));
-      }
-      __syncthreads();
-      const double rowmax = val_of(s_key[1]);
+      double r = 0.0;
+      if (lane >= k && lane < w && lane != imax)
+        r = lane < imax ? fabs((double)F[imax + lane * kBkLd]) : fabs((double)F[lane + imax * kBkLd]);
+      const double rowmax = wave_max_abs(r);
       if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
         kind = 1;
         kp = k;
@@ -535,15 +563,15 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
     // ---- remaining column maxima for the null-pivot test (rows of the block below the pivot,
     //      then lazily the panel rows below the block)
     const int s0 = kind == 1 ? k + 1 : k + 2;
-    if (lane == 0) s_key[2] = 0ull;
-    __syncthreads();
-    if (lane >= s0 && lane < w) {
-      double mv = fabs((double)F[lane + k * kBkLd]);
-      if (kind == 2) mv = fmax(mv, fabs((double)F[lane + (k + 1) * kBkLd]));
-      atomicMax(&s_key[2], key_of(mv));
+    double m01 = colmax;  // kind 1 without a swap: the same column below the pivot
+    if (kind == 2 || kp != kk) {
+      double mv = 0.0;
+      if (lane >= s0 && lane < w) {
+        mv = fabs((double)F[lane + k * kBkLd]);
+        if (kind == 2) mv = fmax(mv, fabs((double)F[lane + (k + 1) * kBkLd]));
+      }
+      m01 = wave_max_abs(mv);
     }
-    __syncthreads();
-    const double m01 = val_of(s_key[2]);
     if (kind == 1) {
       const double d = (double)F[k + k * kBkLd];
       bool null_piv = d == 0.0;
@@ -566,15 +594,17 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
         my_dv = (T)d;
         my_dof = (T)0;
       }
-      const T wi = (lane > k && lane < w) ? F[lane + k * kBkLd] : (T)0;
-      // the multipliers replace column k first; the update then reads them by broadcast
-      if (lane > k && lane < w) F[lane + k * kBkLd] = wi * inv;
+      const bool below = lane > k && lane < w;
+      const T wi = below ? F[lane + k * kBkLd] : (T)0;
+      const T li = wi * inv;
+      if (below) F[lane + k * kBkLd] = li;
+      s_l1[lane] = below ? li : (T)0;
       __syncthreads();
-      if (inv != (T)0) {
-        for (int j = k + 1; j < w; ++j) {
-          const T lj = F[j + k * kBkLd];
-          if (lane >= j && lane < w) F[lane + j * kBkLd] -= wi * lj;
-        }
+      // rank-1 update of the lower triangle: row `lane`, columns k+1..lane
+      if (inv != (T)0 && below) {
+        T* row = F + lane;
+#pragma unroll 8
+        for (int j = k + 1; j <= lane; ++j) row[j * kBkLd] -= wi * s_l1[j];
       }
       __syncthreads();
       k += 1;
@@ -613,16 +643,19 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
       const T w1 = (lane > k + 1 && lane < w) ? F[lane + k * kBkLd] : (T)0;
       const T w2 = (lane > k + 1 && lane < w) ? F[lane + (k + 1) * kBkLd] : (T)0;
       const T l1 = w1 * i11 + w2 * i12, l2 = w1 * i12 + w2 * i22;
-      if (lane > k + 1 && lane < w) {
+      const bool below = lane > k + 1 && lane < w;
+      if (below) {
         F[lane + k * kBkLd] = l1;
         F[lane + (k + 1) * kBkLd] = l2;
       }
+      s_l1[lane] = below ? l1 : (T)0;
+      s_l2[lane] = below ? l2 : (T)0;
       __syncthreads();
-      if (!dead) {
-        for (int j = k + 2; j < w; ++j) {
-          const T l1j = F[j + k * kBkLd], l2j = F[j + (k + 1) * kBkLd];
-          if (lane >= j && lane < w) F[lane + j * kBkLd] -= w1 * l1j + w2 * l2j;
-        }
+      // rank-2 update of the lower triangle: row `lane`, columns k+2..lane
+      if (!dead && below) {
+        T* row = F + lane;
+#pragma unroll 8
+        for (int j = k + 2; j <= lane; ++j) row[j * kBkLd] -= w1 * s_l1[j] + w2 * s_l2[j];
       }
       __syncthreads();
       k += 2;
@@ -639,25 +672,26 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
     N.ipiv[first + c0 + perm[lane]] = c0 + lane;
     bperm[(int64_t)f * nb + lane] = perm[lane];
   }
-  // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L): lane j
-  //      forms column j in LDS (rows of F already final), then the block is stored
+  // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L), outer-product
+  //      form: X = I, then for t = 0.. every row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t); lane i
+  //      owns row i, row t is staged in an array of its own
   __shared__ T Xi[64 * kBkLd];
   __shared__ int8_t ptl[64];
   if (lane < w) ptl[lane] = my_pt;
+  for (int j = 0; j < w; ++j) Xi[lane + j * kBkLd] = lane == j ? (T)1 : (T)0;
   __syncthreads();
-  if (lane < w) {
-    const int j = lane;
-    for (int i = 0; i < w; ++i) Xi[i + j * kBkLd] = i == j ? (T)1 : (T)0;
-    for (int i = j + 1; i < w; ++i) {
-      T v2 = (T)0;
-      for (int t = j; t < i; ++t) {
-        const bool skip = (i == t + 1) && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
-        if (!skip) v2 -= F[i + t * kBkLd] * Xi[t + j * kBkLd];
-      }
-      Xi[i + j * kBkLd] = v2;
+  for (int t = 0; t + 1 < w; ++t) {
+    s_l1[lane] = lane <= t ? Xi[t + lane * kBkLd] : (T)0;
+    __syncthreads();
+    const bool pair = lane == t + 1 && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
+    const T lit = (lane > t && lane < w && !pair) ? F[lane + t * kBkLd] : (T)0;
+    if (lit != (T)0) {
+      T* row = Xi + lane;
+#pragma unroll 8
+      for (int j = 0; j <= t; ++j) row[j * kBkLd] -= lit * s_l1[j];
     }
+    __syncthreads();
   }
-  __syncthreads();
   T* out = Linv + (int64_t)f * nb * nb;
   for (int j = 0; j < w; ++j)
     if (lane < w) out[(int64_t)j * w + lane] = Xi[lane + j * kBkLd];
@@ -933,27 +967,168 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
     }
     for (int s2 : L.h_all) {
       L.max_m = std::max(L.max_m, sym_.sn_m[s2]);
-      L.max_cb = std::max(L.max_cb, sym_.sn_m[s2] - sym_.sn_ncol[s2]);
-      L.nblk = std::max(L.nblk, (sym_.sn_ncol[s2] + kSolveBlock - 1) / kSolveBlock);
+      L.max_ncol = std::max(L.max_ncol, sym_.sn_ncol[s2]);
     }
     L.n_all = (int)L.h_all.size();
     L.n_small = (int)L.h_small.size();
     L.n_large = (int)L.h_large.size();
+    L.fwd_tiles = (L.max_m + kGvTile - 1) / kGvTile;
+    L.fwd_chunks = (L.max_ncol + kGvSplit - 1) / kGvSplit;
+    L.bwd_tiles = (L.max_ncol + kGvTile - 1) / kGvTile;
+    L.bwd_chunks = (L.max_m + kGvSplit - 1) / kGvSplit;
+    // scratch of the split products (slots and counters per group of right-hand sides)
+    if (L.fwd_chunks > 1) {
+      part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks);
+      gv_counters_ = std::max<int64_t>(gv_counters_, (int64_t)L.n_all * L.fwd_tiles);
+    }
+    if (L.bwd_chunks > 1) {
+      part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.bwd_tiles * L.bwd_chunks);
+      gv_counters_ = std::max<int64_t>(gv_counters_, (int64_t)L.n_all * L.bwd_tiles);
+    }
     n_small_ += L.n_small;
     n_large_ += L.n_large;
     max_large_per_level_ = std::max(max_large_per_level_, L.n_large);
   }
-  h_linv_off_.assign(nsn, 0);
-  for (int s2 = 0; s2 < nsn; ++s2) {
-    h_linv_off_[s2] = linv_total_;
-    const int nb2 = (sym_.sn_ncol[s2] + kSolveBlock - 1) / kSolveBlock;
+  plan_solve();
+}
+
+void Engine::plan_solve() {
+  const int nsn = sym_.nsn;
+  // diagonal 64-blocks of every L11
+  for (int s = 0; s < nsn; ++s) {
+    const int nb2 = (sym_.sn_ncol[s] + kSolveBlock - 1) / kSolveBlock;
     for (int b = 0; b < nb2; ++b) {
-      h_pairs_.push_back(s2);
+      h_pairs_.push_back(s);
       h_pairs_.push_back(b);
     }
-    linv_total_ += (int64_t)nb2 * kSolveBlock * kSolveBlock;
   }
   n_pairs_ = (int)(h_pairs_.size() / 2);
+  auto task = [](int64_t a, int32_t lda, int8_t abuf, int64_t b, int32_t ldb, int8_t bbuf, int64_t c, int32_t ldc,
+                 int8_t cbuf, int rows, int cols, int k0, int k1, bool neg) {
+    GemmTask t;
+    t.a = a;
+    t.b = b;
+    t.c = c;
+    t.lda = lda;
+    t.ldb = ldb;
+    t.ldc = ldc;
+    t.rows = rows;
+    t.cols = cols;
+    t.k0 = k0;
+    t.k1 = k1;
+    t.abuf = abuf;
+    t.bbuf = bbuf;
+    t.cbuf = cbuf;
+    t.neg = neg ? 1 : 0;
+    return t;
+  };
+  constexpr int8_t kBufL = 0, kBufG = 1, kBufT = 2;
+  // recursive doubling: at half size h, every pair A = [A0, A0 + h), B = [A0 + h, A0 + h + nB):
+  //   T = L(B, A) G(A, A)      (scratch, nB x h, ld nB)
+  //   G(B, A) = -G(B, B) T
+  for (int64_t h = kSolveBlock; h < sym_.max_ncol; h *= 2) {
+    std::vector<GemmTask> b1, b2;
+    int64_t toff = 0;
+    for (int s = 0; s < nsn; ++s) {
+      const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
+      const int64_t po = sym_.panel_off[s];
+      for (int64_t A0 = 0; A0 + h < ncol; A0 += 2 * h) {
+        const int64_t B0 = A0 + h;
+        const int nB = (int)std::min<int64_t>(h, ncol - B0);
+        for (int ri = 0; ri * 64 < nB; ++ri)
+          for (int ci = 0; ci * 64 < h; ++ci) {
+            const int rows = std::min(64, nB - ri * 64);
+            b1.push_back(task(po + A0 * m + B0 + ri * 64, m, kBufL, po + (A0 + ci * 64) * m + A0, m, kBufG,
+                              toff + (int64_t)ci * 64 * nB + ri * 64, nB, kBufT, rows, 64, ci * 64, (int)h, false));
+            b2.push_back(task(po + B0 * m + B0 + ri * 64, m, kBufG, toff + (int64_t)ci * 64 * nB, nB, kBufT,
+                              po + (A0 + ci * 64) * m + B0 + ri * 64, m, kBufG, rows, 64, 0,
+                              std::min(nB, ri * 64 + 64), true));
+          }
+        toff += (int64_t)nB * h;
+      }
+    }
+    t_total_ = std::max(t_total_, toff);
+    for (auto* v : {&b1, &b2}) {
+      gstages_.push_back({(int64_t)h_tasks_.size(), (int64_t)v->size()});
+      h_tasks_.insert(h_tasks_.end(), v->begin(), v->end());
+    }
+  }
+  // G21 = -L21 G11
+  {
+    const int64_t begin = (int64_t)h_tasks_.size();
+    for (int s = 0; s < nsn; ++s) {
+      const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s], cb = m - ncol;
+      const int64_t po = sym_.panel_off[s];
+      for (int ri = 0; ri * 64 < cb; ++ri)
+        for (int ci = 0; ci * 64 < ncol; ++ci)
+          h_tasks_.push_back(task(po + ncol + ri * 64, m, kBufL, po + (int64_t)ci * 64 * m, m, kBufG,
+                                  po + (int64_t)ci * 64 * m + ncol + ri * 64, m, kBufG, std::min(64, cb - ri * 64),
+                                  std::min(64, ncol - ci * 64), ci * 64, ncol, true));
+    }
+    gstages_.push_back({begin, (int64_t)h_tasks_.size() - begin});
+  }
+  // row-major copy: tiles on and below the diagonal tile
+  for (int s = 0; s < nsn; ++s) {
+    const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
+    for (int ti = 0; ti * 64 < m; ++ti)
+      for (int tj = 0; tj * 64 < ncol && tj <= ti; ++tj) {
+        h_ttiles_.push_back(s);
+        h_ttiles_.push_back(ti);
+        h_ttiles_.push_back(tj);
+      }
+  }
+  n_ttiles_ = (int)(h_ttiles_.size() / 3);
+  // pull map of the forward gather: for front row r (index rowoff + r), the children's U rows
+  // mapped onto it, children in child-list order
+  h_cptr_.assign(rows_total_ + 1, 0);
+  for (int s = 0; s < nsn; ++s)
+    for (int q = sym_.child_ptr[s]; q < sym_.child_ptr[s + 1]; ++q) {
+      const int ch = sym_.child_list[q];
+      const int64_t coff = sym_.sn_rowoff[ch] + sym_.sn_ncol[ch];
+      const int cbc = sym_.sn_m[ch] - sym_.sn_ncol[ch];
+      for (int r = 0; r < cbc; ++r) ++h_cptr_[sym_.sn_rowoff[s] + sym_.relmap[coff + r] + 1];
+    }
+  for (int64_t r = 0; r < rows_total_; ++r) h_cptr_[r + 1] += h_cptr_[r];
+  h_cidx_.assign(std::max<int64_t>(1, h_cptr_[rows_total_]), 0);
+  {
+    std::vector<int64_t> cur(h_cptr_.begin(), h_cptr_.end() - 1);
+    for (int s = 0; s < nsn; ++s)
+      for (int q = sym_.child_ptr[s]; q < sym_.child_ptr[s + 1]; ++q) {
+        const int ch = sym_.child_list[q];
+        const int64_t coff = sym_.sn_rowoff[ch] + sym_.sn_ncol[ch];
+        const int cbc = sym_.sn_m[ch] - sym_.sn_ncol[ch];
+        for (int r = 0; r < cbc; ++r) h_cidx_[cur[sym_.sn_rowoff[s] + sym_.relmap[coff + r]]++] = (int32_t)(coff + r);
+      }
+  }
+  if (rows_total_ >= INT32_MAX) throw std::runtime_error("front row lists too long for 32-bit U indices");
+  validate_solve_plan();
+}
+
+// Every element a planned launch of the solve phase may touch lies inside its buffer: tile
+// products against the panel and scratch sizes, split products against their scratch. Throws
+// (no launch happens) on a violation.
+void Engine::validate_solve_plan() const {
+  auto bad = [](const char* what) { throw std::runtime_error(std::string("solve plan: ") + what); };
+  const int64_t sizes[3] = {panel_total_, panel_total_, t_total_};
+  for (const GemmTask& t : h_tasks_) {
+    if (t.abuf < 0 || t.abuf > 2 || t.bbuf < 0 || t.bbuf > 2 || t.cbuf < 1 || t.cbuf > 2) bad("buffer id");
+    if (t.rows < 1 || t.rows > kGmTile || t.cols < 1 || t.cols > kGmTile || t.k0 < 0 || t.k1 <= t.k0) bad("tile shape");
+    const int64_t a_lo = t.a + (int64_t)t.k0 * t.lda, a_hi = t.a + (int64_t)(t.k1 - 1) * t.lda + t.rows - 1;
+    const int64_t b_lo = t.b + t.k0, b_hi = t.b + (int64_t)(t.cols - 1) * t.ldb + t.k1 - 1;
+    const int64_t c_lo = t.c, c_hi = t.c + (int64_t)(t.cols - 1) * t.ldc + t.rows - 1;
+    if (a_lo < 0 || a_hi >= sizes[t.abuf]) bad("A operand out of range");
+    if (b_lo < 0 || b_hi >= sizes[t.bbuf]) bad("B operand out of range");
+    if (c_lo < 0 || c_hi >= sizes[t.cbuf]) bad("C operand out of range");
+  }
+  for (const TaskRange& r : gstages_)
+    if (r.begin < 0 || r.count < 0 || r.begin + r.count > (int64_t)h_tasks_.size()) bad("stage range");
+  for (const Level& L : levels_) {
+    if (L.n_all != (int)L.h_all.size()) bad("level size");
+    if (L.fwd_chunks > 1 && (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks > part_slots_) bad("forward split scratch");
+    if (L.bwd_chunks > 1 && (int64_t)L.n_all * L.bwd_tiles * L.bwd_chunks > part_slots_) bad("backward split scratch");
+    if (L.fwd_chunks > 1 && (int64_t)L.n_all * L.fwd_tiles > gv_counters_) bad("forward split counters");
+    if (L.bwd_chunks > 1 && (int64_t)L.n_all * L.bwd_tiles > gv_counters_) bad("backward split counters");
+  }
 }
 
 void Engine::ensure_device() {
@@ -974,8 +1149,11 @@ void Engine::ensure_device() {
   d_asm_ = upload(h_asm_);
   d_ia_ = upload(h_ia_);
   d_ib_ = upload(h_ib_);
-  d_linv_off_ = upload(h_linv_off_);
   d_pairs_ = upload(h_pairs_);
+  d_ttiles_ = upload(h_ttiles_);
+  d_tasks_ = upload(h_tasks_);
+  d_cptr_ = upload(h_cptr_);
+  d_cidx_ = upload(h_cidx_);
   for (Level& L : levels_) {
     L.d_all = upload(L.h_all);
     if (L.n_small) L.d_small = upload(L.h_small);
@@ -988,7 +1166,8 @@ Engine::~Engine() {
   free_numeric();
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
-                  (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_linv_off_, (void*)d_pairs_})
+                  (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
+                  (void*)d_tasks_, (void*)d_cptr_, (void*)d_cidx_})
     dfree(p);
   for (auto& L : levels_) {
     dfree(L.d_all);
@@ -999,9 +1178,12 @@ Engine::~Engine() {
 
 void Engine::free_numeric() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
-                  (void*)d_bperm_, (void*)d_cnt_, d_Z_, d_Y_, d_U_, d_Linv_, d_pinv_})
+                  (void*)d_bperm_, (void*)d_cnt_, d_Z_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
+                  (void*)d_gv_count_})
     dfree(p);
-  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Z_ = d_Y_ = d_U_ = d_Linv_ = d_pinv_ = nullptr;
+  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Z_ = d_Y_ = d_U_ = d_pinv_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
+  d_gv_count_ = nullptr;
+  g_ready_ = false;
   d_ptype_ = nullptr;
   d_piv_ = d_ipiv_ = d_bperm_ = nullptr;
   d_cnt_ = nullptr;
@@ -1028,7 +1210,6 @@ void Engine::alloc_numeric(int precision) {
   hip_check(hipMalloc(&d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t)),
             "hipMalloc (pivots)");
   hip_check(hipMalloc(&d_cnt_, kCntSlots * sizeof(unsigned long long)), "hipMalloc (counters)");
-  hip_check(hipMalloc(&d_Linv_, std::max<int64_t>(1, linv_total_) * ts), "hipMalloc (diagonal block inverses)");
   hip_check(hipMalloc(&d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts),
             "hipMalloc (panel inverses)");
   precision_ = precision;
@@ -1084,8 +1265,6 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
       }
     }
   }
-  if (n_pairs_ > 0)
-    hipLaunchKernelGGL(k_block_inverse<T>, dim3(n_pairs_), dim3(256), 0, st, S, N, d_pairs_, d_linv_off_, (T*)d_Linv_);
   hip_check(hipGetLastError(), "factorization kernels");
   unsigned long long h[kCntSlots];
   hip_check(hipMemcpyAsync(h, d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
@@ -1104,6 +1283,7 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
 InertiaResult Engine::factorize(double alpha, double beta, const double* av, const double* bv, int precision,
                                 double null_tol, hipStream_t stream) {
   valid_ = false;
+  g_ready_ = false;
   ensure_device();
   alloc_numeric(precision);
   InertiaResult r = precision == 0 ? factorize_t<double>(alpha, beta, av, bv, null_tol, stream)
@@ -1113,52 +1293,72 @@ InertiaResult Engine::factorize(double alpha, double beta, const double* av, con
 }
 
 template <class T>
+void Engine::build_inverse_panels(hipStream_t st) {
+  const size_t ts = sizeof(T);
+  if (!d_G_) hip_check(hipMalloc(&d_G_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
+  if (!d_Gt_) hip_check(hipMalloc(&d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
+  if (!d_T_) hip_check(hipMalloc(&d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
+  DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
+           d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
+  DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
+  T* G = (T*)d_G_;
+  if (n_pairs_ > 0) hipLaunchKernelGGL(k_diag_block_inverse<T>, dim3(n_pairs_), dim3(64), 0, st, S, N, d_pairs_, G);
+  for (const TaskRange& r : gstages_)
+    if (r.count > 0)
+      hipLaunchKernelGGL(k_tile_gemm<T>, dim3((unsigned)r.count), dim3(256), 0, st, d_tasks_ + r.begin, N.L, G,
+                         (T*)d_T_);
+  if (n_ttiles_ > 0)
+    hipLaunchKernelGGL(k_transpose_g<T>, dim3(n_ttiles_), dim3(256), 0, st, S, d_ttiles_, (const T*)G, (T*)d_Gt_);
+  hip_check(hipGetLastError(), "inverse panel kernels");
+  g_ready_ = true;
+}
+
+template <class T>
 void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
+  if (!g_ready_) build_inverse_panels<T>(st);
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
   if (ws_cols_ < nrhs) {
-    dfree(d_Z_);
-    dfree(d_Y_);
-    dfree(d_U_);
-    d_Z_ = d_Y_ = d_U_ = nullptr;
+    for (void* q : {d_Z_, d_Y_, d_U_, d_part_, (void*)d_gv_count_}) dfree(q);
+    d_Z_ = d_Y_ = d_U_ = d_part_ = nullptr;
+    d_gv_count_ = nullptr;
+    const int64_t nrc = (nrhs + kMaxRhsChunk - 1) / kMaxRhsChunk;
     hip_check(hipMalloc(&d_Z_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
     hip_check(hipMalloc(&d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
     hip_check(hipMalloc(&d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
               "hipMalloc (solve workspace)");
+    hip_check(hipMalloc(&d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kGvTile * kMaxRhsChunk * sizeof(T)),
+              "hipMalloc (solve workspace)");
+    const size_t cbytes = (size_t)std::max<int64_t>(1, gv_counters_ * nrc) * sizeof(int);
+    hip_check(hipMalloc(&d_gv_count_, cbytes), "hipMalloc (solve workspace)");
+    hip_check(hipMemset(d_gv_count_, 0, cbytes), "hipMemset (solve workspace)");
     ws_cols_ = nrhs;
   }
+  T* part = (T*)d_part_;
+  int* gvc = d_gv_count_;
   T* Z = (T*)d_Z_;
   T* Y = (T*)d_Y_;
   T* U = (T*)d_U_;
-  const T* Linv = (const T*)d_Linv_;
+  const T* G = (const T*)d_G_;
+  const T* Gt = (const T*)d_Gt_;
   const int p = (int)nrhs;
-  // forward: Z (accumulated) -> Y (final L^-1 rows, pivot order)
+  const int64_t ldu = rows_total_;
+  // forward: gather z, then [y1; u] = G z1 + [0; u] per front
   for (const Level& L : levels_) {
-    hipLaunchKernelGGL((k_fwd_gather<T, double>), dim3(L.n_all), dim3(256), 0, st, S, N, L.d_all, (const double*)x,
-                       ldx, p, Z, U, rows_total_);
-    for (int b = 0; b < L.nblk; ++b) {
-      const int below = std::max(1, L.max_m - b * kSolveBlock);
-      hipLaunchKernelGGL(k_fwd_step<T>, dim3((below + kSolveRows - 1) / kSolveRows, L.n_all), dim3(256), 0, st, S, N,
-                         L.d_all, b, d_linv_off_, Linv, p, Z, Y, U, rows_total_);
-    }
+    hipLaunchKernelGGL((k_fwd_gather<T, double>), dim3((L.max_m + kGatherRows - 1) / kGatherRows, L.n_all),
+                       dim3(kGatherRows), 0, st, S, N, L.d_all, (const double*)x, ldx, p, Z, U, ldu,
+                       (const int64_t*)d_cptr_, (const int32_t*)d_cidx_);
+    hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
+                       (const T*)Z, Y, U, ldu, part, gvc);
   }
   hipLaunchKernelGGL(k_solve_diag<T>, dim3(blocks_for((int64_t)sym_.n * p)), dim3(256), 0, st, sym_.n, N.dval,
                      N.doff, N.ptype, Y, p);
-  // backward: Y (accumulated) -> Z (final rows, pivot order) -> X
+  // backward: x1 = G^T [y1; x_cb] per front, top-down
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
-    if (L.max_cb > 0) {
-      hipLaunchKernelGGL((k_bwd_gather<T, double>), dim3(L.n_all), dim3(256), 0, st, S, L.d_all, (const double*)x,
-                         ldx, p, U, rows_total_);
-      hipLaunchKernelGGL(k_bwd_cb<T>, dim3(L.nblk, L.n_all), dim3(256), 0, st, S, N, L.d_all, p, Y, (const T*)U,
-                         rows_total_);
-    }
-    for (int b = L.nblk - 1; b >= 0; --b)
-      hipLaunchKernelGGL(k_bwd_step<T>, dim3(1 + b, L.n_all), dim3(256), 0, st, S, N, L.d_all, b, d_linv_off_, Linv, p,
-                         Y, Z);
-    hipLaunchKernelGGL((k_bwd_scatter<T, double>), dim3(L.n_all), dim3(256), 0, st, S, N, L.d_all, x, ldx, p,
-                       (const T*)Z);
+    hipLaunchKernelGGL((k_bwd_gemv<T, double>), dim3(L.bwd_tiles, L.n_all, L.bwd_chunks), dim3(256), 0, st, S, N, Gt,
+                       L.d_all, p, (const T*)Y, x, ldx, part, gvc);
   }
   hip_check(hipGetLastError(), "solve kernels");
 }

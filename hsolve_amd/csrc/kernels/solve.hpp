@@ -1,21 +1,24 @@
-// Multifrontal forward / backward substitution with p right-hand sides, level-batched and blocked
-// (the reference's MUMPS JOB = 3, src/factor/mumps_factorization.cpp:458-527).
+// Multifrontal forward / backward substitution with p right-hand sides in partitioned-inverse
+// form (the solve phase the reference hands to MUMPS, JOB = 3,
+// src/factor/mumps_factorization.cpp:458-527).
 //
-// Each front's pivot block L11 (unit lower, ncol x ncol) is split into column blocks of kSolveBlock;
-// the inverses of their diagonal blocks are formed once after the factorization
-// (k_block_inverse), so every step of a triangular solve is a small dense product:
-//   forward, per front:  gather own rows (pivot order) and children's update vectors;
-//                        for each block b: y_b = inv(L_bb) z_b (one workgroup per front),
-//                        then every row below the block - remaining pivot rows and contribution
-//                        rows alike - is updated, z_i -= L(i, b) y_b (many workgroups per front);
-//                        the contribution rows end as the update vector U passed to the parent;
-//   diagonal:            z = D^-1 z (1x1 and 2x2 blocks);
-//   backward, per front: z_j -= sum_i L21(i, j) x_cb(i) with the ancestors' rows (final already),
-//                        then for each block b from the last: x_b = inv(L_bb)^T z_b, and
-//                        z_j -= sum_{i in b} L(i, j) x_i for the pivot columns j before the block;
-//                        scatter into X.
-// Every launch covers all fronts of one level (grid y = front). R is the right-hand-side type
-// (double), T the factor type.
+// With P K P^T = L D L^T and, per front s (ncol pivots, m rows), L = [L11; L21], the solve phase
+// reads, per front,
+//     G  = [ G11 ; G21 ] = [ L11^-1 ; -L21 L11^-1 ]   (m x ncol, column-major, the panel layout)
+//     Gt = G stored row-major                         (same offsets, element (i, j) at i*ncol + j)
+// because L^-1 restricted to one front is [L11^-1 0; -L21 L11^-1 I]. Then
+//   forward, bottom-up per level:  z = own right-hand-side rows + children's update vectors
+//                                  (pull gather, fixed order), then one GEMV per front:
+//                                  [ y1 ; u ] = G z1 + [ 0 ; z_cb ]
+//   diagonal:                      y = D^-1 y (1x1 and 2x2 blocks)
+//   backward, top-down per level:  x1 = G^T [ y1 ; x_cb ] with x_cb read from X (rows of
+//                                  ancestors, final), one GEMV per front, scattered into X.
+// Two launches per level forward, one backward, and no sequential chain inside a front.
+//
+// G is built from L once per factorization by batched 64 x 64 tile products: inverses of the
+// 64 x 64 diagonal blocks of L11, then recursive doubling over block sizes h = 64, 128, ...
+//     inv([A 0; C B]) = [inv(A) 0; -inv(B) C inv(A)  inv(B)],
+// then G21 = -L21 G11, then the row-major copy.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,315 +28,285 @@
 
 namespace hsg {
 
-constexpr int kSolveBlock = 64;
-constexpr int kSolveRows = 64;   // rows per update workgroup
-constexpr int kMaxRhsChunk = 8;  // right-hand sides per pass
+constexpr int kSolveBlock = 64;   // diagonal block of the first inversion step
+constexpr int kGvTile = 64;       // rows (forward) / columns (backward) per GEMV workgroup
+constexpr int kGvChunk = 128;     // vector entries staged in LDS per pass
+constexpr int kMaxRhsChunk = 8;   // right-hand sides per pass
+constexpr int kGatherRows = 256;  // rows per gather workgroup
 
-// L(i, j) of a pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair.
+// L(i, j) of a pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair (that slot of
+// the panel holds D's off-diagonal).
 template <class T>
 __device__ __forceinline__ T lpiv_at(const T* P, int m, int i, int j, const int8_t* pt) {
   if (i == j + 1 && (pt[j] == kPiv2First || pt[j] == kPivNull)) return (T)0;
   return P[(int64_t)j * m + i];
 }
 
-// Inverse of the unit lower diagonal block b of front s (bs x bs, column-major, into Linv).
-// One workgroup per (front, block) pair of `pairs` (2 ints each).
+// ------------------------------------------------------------------ building G from L
+
+// Inverse of the unit lower diagonal block b of front s into G (bs x bs, zeros above the
+// diagonal), and the (j+1, j) slots of the block's pivot pairs in L set to 0 so that the tile
+// products below read L as it is. One 64-thread workgroup per (front, block) pair of `pairs`.
+// Outer-product form: X = I, then for t = 0.. each row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t).
 template <class T>
-__global__ void __launch_bounds__(256) k_block_inverse(DevSym S, DevNum<T> N, const int32_t* __restrict__ pairs,
-                                                        const int64_t* __restrict__ linv_off, T* __restrict__ Linv) {
-  __shared__ T Lb[kSolveBlock][kSolveBlock + 1];
+__global__ void __launch_bounds__(64) k_diag_block_inverse(DevSym S, DevNum<T> N, const int32_t* __restrict__ pairs,
+                                                            T* __restrict__ G) {
+  __shared__ T Xi[kSolveBlock * (kSolveBlock + 1)];
+  __shared__ T rowt[kSolveBlock];
+  constexpr int ld = kSolveBlock + 1;
   const int s = pairs[2 * blockIdx.x], b = pairs[2 * blockIdx.x + 1];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
   const int r0 = b * kSolveBlock;
   const int bs = min(kSolveBlock, ncol - r0);
-  const T* P = N.L + S.panel_off[s];
+  T* P = N.L + S.panel_off[s];
   const int8_t* pt = N.ptype + first;
-  for (int idx = threadIdx.x; idx < bs * bs; idx += blockDim.x) {
-    const int i = idx % bs, j = idx / bs;
-    Lb[i][j] = i > j ? lpiv_at(P, m, r0 + i, r0 + j, pt) : (T)0;
-  }
+  const int lane = threadIdx.x;
+  for (int j = 0; j < bs; ++j) Xi[lane + j * ld] = lane == j ? (T)1 : (T)0;
   __syncthreads();
-  __shared__ T Xi[kSolveBlock][kSolveBlock + 1];
-  T* out = Linv + linv_off[s] + (int64_t)b * kSolveBlock * kSolveBlock;
-  const int j = threadIdx.x;
-  if (j < bs) {
-    // column j of the inverse: x_i = delta_ij - sum_{j <= t < i} L(i, t) x_t
-    for (int i = 0; i < j; ++i) Xi[i][j] = (T)0;
-    for (int i = j; i < bs; ++i) {
-      T v = i == j ? (T)1 : (T)0;
-      for (int t = j; t < i; ++t) v -= Lb[i][t] * Xi[t][j];
-      Xi[i][j] = v;
+  for (int t = 0; t + 1 < bs; ++t) {
+    rowt[lane] = lane <= t ? Xi[t + lane * ld] : (T)0;
+    __syncthreads();
+    const T lit = (lane > t && lane < bs) ? lpiv_at(P, m, r0 + lane, r0 + t, pt) : (T)0;
+    if (lit != (T)0) {
+      T* row = Xi + lane;
+#pragma unroll 8
+      for (int j = 0; j <= t; ++j) row[j * ld] -= lit * rowt[j];
     }
+    __syncthreads();
+  }
+  T* Gs = G + S.panel_off[s];
+  if (lane < bs)
+    for (int j = 0; j < bs; ++j) Gs[(int64_t)(r0 + j) * m + r0 + lane] = Xi[lane + j * ld];
+  // pivot pairs whose second row lies inside this front
+  if (lane < bs) {
+    const int j = r0 + lane;
+    if (j + 1 < ncol && (pt[j] == kPiv2First || pt[j] == kPivNull)) P[(int64_t)j * m + j + 1] = (T)0;
+  }
+}
+
+// C = (neg ? -1 : 1) * A(:, k0:k1) B(k0:k1, :) for one tile of at most 64 x 64, column-major
+// operands given as element offsets into one of three buffers (0: L, 1: G, 2: scratch).
+struct GemmTask {
+  int64_t a, b, c;
+  int32_t lda, ldb, ldc;
+  int32_t rows, cols, k0, k1;
+  int8_t abuf, bbuf, cbuf, neg;
+};
+
+constexpr int kGmTile = 64;
+constexpr int kGmK = 16;
+
+template <class T>
+__device__ __forceinline__ T* pick_buffer(int which, T* b0, T* b1, T* b2) {
+  return which == 0 ? b0 : (which == 1 ? b1 : b2);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_tile_gemm(const GemmTask* __restrict__ tasks, T* b0, T* b1, T* b2) {
+  const GemmTask tk = tasks[blockIdx.x];
+  const T* A = pick_buffer<T>(tk.abuf, b0, b1, b2) + tk.a;
+  const T* B = pick_buffer<T>(tk.bbuf, b0, b1, b2) + tk.b;
+  T* C = pick_buffer<T>(tk.cbuf, b0, b1, b2) + tk.c;
+  __shared__ T As[kGmK][kGmTile + 1];  // As[k][i]
+  __shared__ T Bs[kGmK][kGmTile + 1];  // Bs[k][j]
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
+  for (int kk = tk.k0; kk < tk.k1; kk += kGmK) {
+    for (int idx = tid; idx < kGmK * kGmTile; idx += 256) {
+      const int r = idx % kGmTile, t = idx / kGmTile;  // A: consecutive threads along rows
+      const int k = kk + t;
+      As[t][r] = (r < tk.rows && k < tk.k1) ? A[(int64_t)k * tk.lda + r] : (T)0;
+      const int t2 = idx % kGmK, c2 = idx / kGmK;      // B: consecutive threads along k
+      const int k2 = kk + t2;
+      Bs[t2][c2] = (c2 < tk.cols && k2 < tk.k1) ? B[(int64_t)c2 * tk.ldb + k2] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kGmK; ++t) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[t][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[t][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = tx + 16 * a, j = ty + 16 * b;
+      if (i < tk.rows && j < tk.cols) C[(int64_t)j * tk.ldc + i] = tk.neg ? -acc[a][b] : acc[a][b];
+    }
+}
+
+// Row-major copy of G: Gt(i, j) = G(i, j) on and below the diagonal of G11 and on all of G21,
+// 0 above the diagonal. One workgroup per 64 x 64 tile (front, tile row, tile column) of `tiles`.
+template <class T>
+__global__ void __launch_bounds__(256) k_transpose_g(DevSym S, const int32_t* __restrict__ tiles, const T* __restrict__ G,
+                                                      T* __restrict__ Gt) {
+  __shared__ T tl[64][65];
+  const int s = tiles[3 * blockIdx.x], ti = tiles[3 * blockIdx.x + 1], tj = tiles[3 * blockIdx.x + 2];
+  const int m = S.sn_m[s], ncol = S.sn_ncol[s];
+  const int64_t off = S.panel_off[s];
+  const int i0 = ti * 64, j0 = tj * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int i = idx % 64, j = idx / 64;  // read along i (column-major G)
+    const int gi = i0 + i, gj = j0 + j;
+    T v = (T)0;
+    if (gi < m && gj < ncol && gi >= gj) v = G[off + (int64_t)gj * m + gi];
+    tl[i][j] = v;
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < bs * bs; idx += blockDim.x) out[idx] = Xi[idx % bs][idx / bs];
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int j = idx % 64, i = idx / 64;  // write along j (row-major Gt)
+    const int gi = i0 + i, gj = j0 + j;
+    if (gi < m && gj < ncol) Gt[off + (int64_t)gi * ncol + gj] = tl[i][j];
+  }
 }
 
 // ---------------------------------------------------------------------------------- forward
 
+// z of each front of the level: own rows take the right-hand side (converted to the factor type),
+// every row adds the update vectors of the children's contribution rows mapped onto it (cptr /
+// cidx: per front row, the U rows of the children, in child order). Pivot rows go to Z in pivot
+// order, contribution rows to the front's U rows. grid (row chunks, fronts).
 template <class T, class R>
-__global__ void __launch_bounds__(256) k_fwd_gather(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                     const R* __restrict__ X, int64_t ldx, int p, T* Z, T* U,
-                                                     int64_t ldu) {
-  const int s = list[blockIdx.x];
+__global__ void __launch_bounds__(kGatherRows) k_fwd_gather(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
+                                                             const R* __restrict__ X, int64_t ldx, int p, T* Z, T* U,
+                                                             int64_t ldu, const int64_t* __restrict__ cptr,
+                                                             const int32_t* __restrict__ cidx) {
+  const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int cb = m - ncol;
-  const int n = S.n;
+  const int loc = blockIdx.x * kGatherRows + threadIdx.x;
+  if (loc >= m) return;
   const int64_t roff = S.sn_rowoff[s];
-  const int32_t* rows = S.rows_orig + roff;
-  T* zs = Z + first;
-  T* us = U + roff + ncol;
-  for (int idx = threadIdx.x; idx < ncol * p; idx += blockDim.x) {
-    const int i = idx % ncol, c = idx / ncol;
-    zs[i + (int64_t)c * n] = (T)X[rows[N.piv[first + i]] + (int64_t)c * ldx];
+  const int64_t q0 = cptr[roff + loc], q1 = cptr[roff + loc + 1];
+  const bool own = loc < ncol;
+  const int64_t xrow = own ? S.rows_orig[roff + loc] : 0;
+  const int64_t dst = own ? (int64_t)first + N.ipiv[first + loc] : roff + loc;
+  T* out = own ? Z : U;
+  const int64_t ldo = own ? (int64_t)S.n : ldu;
+  for (int c = 0; c < p; ++c) {
+    T v = own ? (T)X[xrow + c * ldx] : (T)0;
+    for (int64_t q = q0; q < q1; ++q) v += U[cidx[q] + c * ldu];
+    out[dst + c * ldo] = v;
   }
-  for (int idx = threadIdx.x; idx < cb * p; idx += blockDim.x) {
-    const int i = idx % cb, c = idx / cb;
-    us[i + (int64_t)c * ldu] = (T)0;
-  }
+}
+
+// Split products: a tile's long dimension (columns of G forward, rows of Gt backward) is cut into
+// chunks of kGvSplit, one workgroup each. With more than one chunk, each workgroup stores its
+// partial sums (64 x p) in its slot of `part`, and the last workgroup of the tile to finish
+// (device-scope fence, then a counter) adds the partials in chunk order and writes the result,
+// so the sum does not depend on the order the workgroups run in. The counter is reset by that
+// workgroup for the next use. Counters: ((front index in level) * tiles + tile) * rhs_chunks +
+// rhs_chunk (rhs_chunk = which group of kMaxRhsChunk right-hand sides); slots: counter * chunks +
+// chunk.
+constexpr int kGvSplit = 256;
+
+// Returns true in the workgroup that holds the complete sums of the tile in red[.][lane][c]
+// (red = this workgroup's own sums when the tile has one chunk).
+template <class T>
+__device__ __forceinline__ bool split_combine(T (*red)[kGvTile][kMaxRhsChunk + 1], int nchunk, int chunk,
+                                              int64_t slot0, int pc, T* __restrict__ part, int* __restrict__ counter,
+                                              int cidx) {
+  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
+  // red[0..3] hold the 4 group sums; fold them into red[0]
+  for (int c = g; c < pc; c += 4) red[0][lane][c] = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
   __syncthreads();
-  for (int q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
-    const int ch = S.child_list[q];
-    const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
-    const int64_t coff = S.sn_rowoff[ch] + S.sn_ncol[ch];
-    const int32_t* rel = S.relmap + coff;
-    const T* uc = U + coff;
-    for (int idx = threadIdx.x; idx < cbc * p; idx += blockDim.x) {
-      const int i = idx % cbc, c = idx / cbc;
-      const int loc = rel[i];
-      const T v = uc[i + (int64_t)c * ldu];
-      if (loc < ncol) zs[N.ipiv[first + loc] + (int64_t)c * n] += v;
-      else us[loc - ncol + (int64_t)c * ldu] += v;
-    }
-    __syncthreads();
+  if (nchunk == 1) return true;
+  T* mine = part + (slot0 + chunk) * (int64_t)(kGvTile * kMaxRhsChunk);
+  for (int c = g; c < pc; c += 4) mine[c * kGvTile + lane] = red[0][lane][c];
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_last;
+  if (threadIdx.x == 0) s_last = atomicAdd(&counter[cidx], 1) == nchunk - 1;
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  for (int c = g; c < pc; c += 4) {
+    T v = (T)0;
+    for (int q = 0; q < nchunk; ++q) v += part[(slot0 + q) * (int64_t)(kGvTile * kMaxRhsChunk) + c * kGvTile + lane];
+    red[0][lane][c] = v;
   }
+  if (threadIdx.x == 0) counter[cidx] = 0;
+  __syncthreads();
+  return true;
 }
 
-// y_b = inv(L_bb) z_b in place (forward) or x_b = inv(L_bb)^T z_b (backward); one workgroup per front.
-template <class T, bool kTranspose>
-__global__ void __launch_bounds__(256) k_diag_block(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int b,
-                                                     const int64_t* __restrict__ linv_off, const T* __restrict__ Linv,
-                                                     int p, T* Z) {
-  __shared__ T Li[kSolveBlock][kSolveBlock + 1];
-  __shared__ T zb[kSolveBlock][kMaxRhsChunk];
-  const int s = list[blockIdx.x];
-  const int ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int r0 = b * kSolveBlock;
-  if (r0 >= ncol) return;
-  const int bs = min(kSolveBlock, ncol - r0);
-  const int n = S.n;
-  const T* li = Linv + linv_off[s] + (int64_t)b * kSolveBlock * kSolveBlock;
-  for (int idx = threadIdx.x; idx < bs * bs; idx += blockDim.x) Li[idx % bs][idx / bs] = li[idx];
-  T* zs = Z + first + r0;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
-    const int pc = min(kMaxRhsChunk, p - c0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int i = idx % bs, c = idx / bs;
-      zb[i][c] = zs[i + (int64_t)(c0 + c) * n];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int i = idx % bs, c = idx / bs;
-      T acc = (T)0;
-      if (!kTranspose) {
-        for (int t = 0; t <= i; ++t) acc += Li[i][t] * zb[t][c];
-      } else {
-        for (int t = i; t < bs; ++t) acc += Li[t][i] * zb[t][c];
-      }
-      zs[i + (int64_t)(c0 + c) * n] = acc;
-    }
-  }
-}
-
-// Rows below block b: z_i -= L(i, b) y_b for pivot rows, u_i -= ... for contribution rows.
-// grid.x: chunks of kSolveRows rows over [r0 + bs, m); 256 threads = kSolveRows rows x 4 groups of
-// the block's columns, partial sums combined in LDS.
+// [y1; u] = G z1 + [0; u] for a tile of 64 front rows and one chunk of G's columns: rows < ncol
+// (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution rows
+// add G21 z1 to the front's U rows. 256 threads = 64 rows x 4 column groups; z1 is staged through
+// LDS, G read along columns. grid (row tiles, fronts, column chunks).
 template <class T>
-__global__ void __launch_bounds__(256) k_fwd_update(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int b,
-                                                     int p, T* Z, T* U, int64_t ldu) {
-  __shared__ T yb[kSolveBlock][kMaxRhsChunk];
-  __shared__ T part[4][kSolveRows][kMaxRhsChunk + 1];
-  const int s = list[blockIdx.y];
+__global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict__ G, const int32_t* __restrict__ list,
+                                                   int p, const T* __restrict__ Z, T* __restrict__ Y, T* __restrict__ U,
+                                                   int64_t ldu, T* __restrict__ part, int* __restrict__ counter) {
+  __shared__ T zs[kGvChunk][kMaxRhsChunk];
+  __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
+  const int f = blockIdx.y;
+  const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int r0 = b * kSolveBlock;
-  if (r0 >= ncol) return;
-  const int bs = min(kSolveBlock, ncol - r0);
-  const int rstart = r0 + bs + blockIdx.x * kSolveRows;
-  if (rstart >= m) return;
-  const int rn = min(kSolveRows, m - rstart);
+  const int tile = blockIdx.x, chunk = blockIdx.z;
+  const int r0 = tile * kGvTile;
+  if (r0 >= m) return;
+  const int kend = min(ncol, r0 + kGvTile);
+  const int nchunk = (kend + kGvSplit - 1) / kGvSplit;
+  if (chunk >= nchunk) return;
+  const int kbeg = chunk * kGvSplit, kstop = min(kend, kbeg + kGvSplit);
   const int n = S.n;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* zs = Z + first;
-  T* us = U + S.sn_rowoff[s] + ncol;
-  const int lr = threadIdx.x % kSolveRows, g = threadIdx.x / kSolveRows;
+  const T* Gs = G + S.panel_off[s];
+  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
+  const int i = r0 + lane;
+  const bool rin = i < m;
+  const T* zf = Z + first;
+  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
     const int pc = min(kMaxRhsChunk, p - c0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int t = idx % bs, c = idx / bs;
-      yb[t][c] = zs[r0 + t + (int64_t)(c0 + c) * n];
-    }
-    __syncthreads();
+    const int cidx = (f * (int)gridDim.x + tile) * nrc + c0 / kMaxRhsChunk;
+    const int64_t slot0 = (int64_t)cidx * gridDim.z;
     T acc[kMaxRhsChunk];
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
-    const int i = rstart + lr;
-    if (lr < rn) {
-      for (int t = g; t < bs; t += 4) {
-        const T l = lpiv_at(P, m, i, r0 + t, pt);
+    for (int k0 = kbeg; k0 < kstop; k0 += kGvChunk) {
+      const int kn = min(kGvChunk, kstop - k0);
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < kGvChunk * kMaxRhsChunk; idx += 256) {
+        const int t = idx % kGvChunk, c = idx / kGvChunk;
+        zs[t][c] = (t < kn && c < pc) ? zf[k0 + t + (int64_t)(c0 + c) * n] : (T)0;
+      }
+      __syncthreads();
+      if (rin) {
+        const T* gcol = Gs + (int64_t)k0 * m + i;
+#pragma unroll 4
+        for (int t = g; t < kn; t += 4) {
+          const T gv = gcol[(int64_t)t * m];
 #pragma unroll
-        for (int c = 0; c < kMaxRhsChunk; ++c)
-          if (c < pc) acc[c] += l * yb[t][c];
+          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv * zs[t][c];
+        }
       }
     }
+    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c) part[g][lr][c] = acc[c];
+    for (int c = 0; c < kMaxRhsChunk; ++c) red[g][lane][c] = acc[c];
     __syncthreads();
-    if (g == 0 && lr < rn) {
-      for (int c = 0; c < pc; ++c) {
-        const T v = part[0][lr][c] + part[1][lr][c] + part[2][lr][c] + part[3][lr][c];
-        if (i < ncol) zs[i + (int64_t)(c0 + c) * n] -= v;
-        else us[i - ncol + (int64_t)(c0 + c) * ldu] -= v;
+    if (split_combine<T>(red, nchunk, chunk, slot0, pc, part, counter, cidx) && rin) {
+      for (int c = g; c < pc; c += 4) {
+        const T v = red[0][lane][c];
+        if (i < ncol) Y[first + i + (int64_t)(c0 + c) * n] = v;
+        else U[S.sn_rowoff[s] + i + (int64_t)(c0 + c) * ldu] += v;
       }
     }
-  }
-}
-
-// Fused forward block step b: every workgroup forms y_b = inv(L_bb) z_b for itself (small, from
-// LDS), workgroup 0 stores it into Y (the forward result, a separate buffer, so that no
-// workgroup of this launch reads a value another one rewrites), and each workgroup updates its
-// chunk of the rows below the block: z_i -= L(i, b) y_b (pivot rows in Z, contribution rows in U).
-template <class T>
-__global__ void __launch_bounds__(256) k_fwd_step(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int b,
-                                                   const int64_t* __restrict__ linv_off, const T* __restrict__ Linv,
-                                                   int p, T* Z, T* Y, T* U, int64_t ldu) {
-  __shared__ T Li[kSolveBlock][kSolveBlock + 1];
-  __shared__ T zb[kSolveBlock][kMaxRhsChunk];
-  __shared__ T yb[kSolveBlock][kMaxRhsChunk];
-  __shared__ T part[4][kSolveRows][kMaxRhsChunk + 1];
-  const int s = list[blockIdx.y];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int r0 = b * kSolveBlock;
-  if (r0 >= ncol) return;
-  const int bs = min(kSolveBlock, ncol - r0);
-  const int rstart = r0 + bs + blockIdx.x * kSolveRows;
-  if (blockIdx.x > 0 && rstart >= m) return;
-  const int rn = max(0, min(kSolveRows, m - rstart));
-  const int n = S.n;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* zs = Z + first;
-  T* ys = Y + first;
-  T* us = U + S.sn_rowoff[s] + ncol;
-  const T* li = Linv + linv_off[s] + (int64_t)b * kSolveBlock * kSolveBlock;
-  for (int idx = threadIdx.x; idx < bs * bs; idx += blockDim.x) Li[idx % bs][idx / bs] = li[idx];
-  const int lr = threadIdx.x % kSolveRows, g = threadIdx.x / kSolveRows;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
-    const int pc = min(kMaxRhsChunk, p - c0);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int t = idx % bs, c = idx / bs;
-      zb[t][c] = zs[r0 + t + (int64_t)(c0 + c) * n];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int i = idx % bs, c = idx / bs;
-      T acc = (T)0;
-      for (int t = 0; t <= i; ++t) acc += Li[i][t] * zb[t][c];
-      yb[i][c] = acc;
-      if (blockIdx.x == 0) ys[r0 + i + (int64_t)(c0 + c) * n] = acc;
-    }
-    __syncthreads();
-    T acc[kMaxRhsChunk];
-#pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
-    const int i = rstart + lr;
-    if (lr < rn) {
-      for (int t = g; t < bs; t += 4) {
-        const T l = lpiv_at(P, m, i, r0 + t, pt);
-#pragma unroll
-        for (int c = 0; c < kMaxRhsChunk; ++c)
-          if (c < pc) acc[c] += l * yb[t][c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c) part[g][lr][c] = acc[c];
-    __syncthreads();
-    if (g == 0 && lr < rn) {
-      for (int c = 0; c < pc; ++c) {
-        const T v = part[0][lr][c] + part[1][lr][c] + part[2][lr][c] + part[3][lr][c];
-        if (i < ncol) zs[i + (int64_t)(c0 + c) * n] -= v;
-        else us[i - ncol + (int64_t)(c0 + c) * ldu] -= v;
-      }
-    }
-  }
-}
-
-// Fused backward block step b: every workgroup forms x_b = inv(L_bb)^T y_b, workgroup 0 stores it
-// into X (the backward result in pivot order, a separate buffer), and the workgroups of the
-// column chunks before the block update y_j -= sum_{i in b} L(i, j) x_i for their columns.
-// grid.x = 1 + number of 64-column chunks in [0, r0).
-template <class T>
-__global__ void __launch_bounds__(256) k_bwd_step(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int b,
-                                                   const int64_t* __restrict__ linv_off, const T* __restrict__ Linv,
-                                                   int p, T* Y, T* Xo) {
-  __shared__ T Li[kSolveBlock][kSolveBlock + 1];
-  __shared__ T Lt[kSolveBlock][kSolveBlock + 1];
-  __shared__ T yb[kSolveBlock][kMaxRhsChunk];
-  __shared__ T xb[kSolveBlock][kMaxRhsChunk];
-  const int s = list[blockIdx.y];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int r0 = b * kSolveBlock;
-  if (r0 >= ncol) return;
-  const int bs = min(kSolveBlock, ncol - r0);
-  const int chunk = (int)blockIdx.x - 1;  // -1: the storing workgroup
-  const int j0 = chunk * kSolveBlock;
-  if (chunk >= 0 && j0 >= r0) return;
-  const int jn = chunk >= 0 ? min(kSolveBlock, r0 - j0) : 0;
-  const int n = S.n;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* ys = Y + first;
-  T* xs = Xo + first;
-  const T* li = Linv + linv_off[s] + (int64_t)b * kSolveBlock * kSolveBlock;
-  for (int idx = threadIdx.x; idx < bs * bs; idx += blockDim.x) Li[idx % bs][idx / bs] = li[idx];
-  if (chunk >= 0)
-    for (int idx = threadIdx.x; idx < kSolveBlock * kSolveBlock; idx += blockDim.x) {
-      const int t = idx % kSolveBlock, j = idx / kSolveBlock;
-      Lt[t][j] = (t < bs && j < jn) ? lpiv_at(P, m, r0 + t, j0 + j, pt) : (T)0;
-    }
-  const int jj = threadIdx.x % kSolveBlock, cq = threadIdx.x / kSolveBlock;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
-    const int pc = min(kMaxRhsChunk, p - c0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < bs * pc; idx += blockDim.x) {
-      const int t = idx % bs, c = idx / bs;
-      yb[t][c] = ys[r0 + t + (int64_t)(c0 + c) * n];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < kSolveBlock * kMaxRhsChunk; idx += blockDim.x) {
-      const int i = idx % kSolveBlock, c = idx / kSolveBlock;
-      T acc = (T)0;
-      if (i < bs && c < pc)
-        for (int t = i; t < bs; ++t) acc += Li[t][i] * yb[t][c];
-      xb[i][c] = acc;
-      if (chunk < 0 && i < bs && c < pc) xs[r0 + i + (int64_t)(c0 + c) * n] = acc;
-    }
-    __syncthreads();
-    if (chunk >= 0) {
-      T acc0 = (T)0, acc1 = (T)0;
-      for (int t = 0; t < bs; ++t) {
-        const T l = Lt[t][jj];
-        acc0 += l * xb[t][cq];
-        acc1 += l * xb[t][cq + 4];
-      }
-      if (jj < jn) {
-        if (cq < pc) ys[j0 + jj + (int64_t)(c0 + cq) * n] -= acc0;
-        if (cq + 4 < pc) ys[j0 + jj + (int64_t)(c0 + cq + 4) * n] -= acc1;
-      }
-    }
   }
 }
 
@@ -365,123 +338,73 @@ __global__ void k_solve_diag(int n, const T* __restrict__ dval, const T* __restr
 
 // ---------------------------------------------------------------------------------- backward
 
-// x_cb = X at the front's contribution rows (ancestors' rows, final), into the U buffer.
+// x1 = G^T [y1; x_cb] for a tile of 64 pivot columns of a front and one chunk of its rows
+// i >= the tile's first column (G11 lower triangular): y1 from Y (pivot order), x_cb from X at the
+// front's contribution rows (rows of ancestors, final already). The result is scattered into X at
+// the pivots' rows. 256 threads = 64 columns x 4 row groups; Gt read along rows.
+// grid (column tiles, fronts, row chunks).
 template <class T, class R>
-__global__ void __launch_bounds__(256) k_bwd_gather(DevSym S, const int32_t* __restrict__ list, const R* X,
-                                                     int64_t ldx, int p, T* U, int64_t ldu) {
-  const int s = list[blockIdx.x];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s];
-  const int cb = m - ncol;
-  const int64_t off = S.sn_rowoff[s] + ncol;
-  const int32_t* rows = S.rows_orig + off;
-  for (int idx = threadIdx.x; idx < cb * p; idx += blockDim.x) {
-    const int i = idx % cb, c = idx / cb;
-    U[off + i + (int64_t)c * ldu] = (T)X[rows[i] + (int64_t)c * ldx];
-  }
-}
-
-// z_j -= sum_i L21(i, j) x_cb(i) for the pivot columns j of chunk blockIdx.x (64 columns):
-// tiles of 64 contribution rows x 64 columns staged through LDS with coalesced column reads,
-// one thread per (column, right-hand-side group) accumulating over the rows.
-template <class T>
-__global__ void __launch_bounds__(256) k_bwd_cb(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int p, T* Z,
-                                                 const T* U, int64_t ldu) {
-  __shared__ T Lt[kSolveBlock][kSolveBlock + 1];
-  __shared__ T xt[kSolveBlock][kMaxRhsChunk];
-  const int s = list[blockIdx.y];
+__global__ void __launch_bounds__(256) k_bwd_gemv(DevSym S, DevNum<T> N, const T* __restrict__ Gt,
+                                                   const int32_t* __restrict__ list, int p, const T* __restrict__ Y,
+                                                   R* X, int64_t ldx, T* __restrict__ part, int* __restrict__ counter) {
+  __shared__ T ws[kGvChunk][kMaxRhsChunk];
+  __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
+  const int f = blockIdx.y;
+  const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int cb = m - ncol;
-  const int j0 = blockIdx.x * kSolveBlock;
-  if (cb == 0 || j0 >= ncol) return;
+  const int tile = blockIdx.x, chunk = blockIdx.z;
+  const int j0 = tile * kGvTile;
+  if (j0 >= ncol) return;
+  const int nchunk = (m - j0 + kGvSplit - 1) / kGvSplit;
+  if (chunk >= nchunk) return;
+  const int ibeg = j0 + chunk * kGvSplit, istop = min(m, ibeg + kGvSplit);
   const int n = S.n;
-  const T* P = N.L + S.panel_off[s];
-  const T* xc = U + S.sn_rowoff[s] + ncol;
-  T* zs = Z + first;
-  const int jj = threadIdx.x % kSolveBlock, cq = threadIdx.x / kSolveBlock;  // cq in [0, 4)
-  const int jn = min(kSolveBlock, ncol - j0);
+  const int64_t roff = S.sn_rowoff[s];
+  const int32_t* rows = S.rows_orig + roff;
+  const T* Gs = Gt + S.panel_off[s];
+  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
+  const int j = j0 + lane;
+  const bool jin = j < ncol;
+  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
     const int pc = min(kMaxRhsChunk, p - c0);
-    T acc0 = (T)0, acc1 = (T)0;
-    for (int r0 = 0; r0 < cb; r0 += kSolveBlock) {
-      const int rn = min(kSolveBlock, cb - r0);
+    const int cidx = (f * (int)gridDim.x + tile) * nrc + c0 / kMaxRhsChunk;
+    const int64_t slot0 = (int64_t)cidx * gridDim.z;
+    T acc[kMaxRhsChunk];
+#pragma unroll
+    for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
+    for (int i0 = ibeg; i0 < istop; i0 += kGvChunk) {
+      const int kn = min(kGvChunk, istop - i0);
       __syncthreads();
-      for (int idx = threadIdx.x; idx < kSolveBlock * kSolveBlock; idx += blockDim.x) {
-        const int r = idx % kSolveBlock, j = idx / kSolveBlock;
-        Lt[r][j] = (r < rn && j < jn) ? P[(int64_t)(j0 + j) * m + ncol + r0 + r] : (T)0;
-      }
-      for (int idx = threadIdx.x; idx < kSolveBlock * kMaxRhsChunk; idx += blockDim.x) {
-        const int r = idx % kSolveBlock, c = idx / kSolveBlock;
-        xt[r][c] = (r < rn && c < pc) ? xc[r0 + r + (int64_t)(c0 + c) * ldu] : (T)0;
+      for (int idx = threadIdx.x; idx < kGvChunk * kMaxRhsChunk; idx += 256) {
+        const int t = idx % kGvChunk, c = idx / kGvChunk;
+        T v = (T)0;
+        if (t < kn && c < pc) {
+          const int i = i0 + t;
+          v = i < ncol ? Y[first + i + (int64_t)(c0 + c) * n] : (T)X[rows[i] + (int64_t)(c0 + c) * ldx];
+        }
+        ws[t][c] = v;
       }
       __syncthreads();
-      for (int r = 0; r < rn; ++r) {
-        const T l = Lt[r][jj];
-        acc0 += l * xt[r][cq];
-        acc1 += l * xt[r][cq + 4];
+      if (jin) {
+        const T* grow = Gs + (int64_t)i0 * ncol + j;
+#pragma unroll 4
+        for (int t = g; t < kn; t += 4) {
+          const T gv = grow[(int64_t)t * ncol];
+#pragma unroll
+          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv * ws[t][c];
+        }
       }
-    }
-    if (jj < jn) {
-      if (cq < pc) zs[j0 + jj + (int64_t)(c0 + cq) * n] -= acc0;
-      if (cq + 4 < pc) zs[j0 + jj + (int64_t)(c0 + cq + 4) * n] -= acc1;
-    }
-  }
-}
-
-// z_j -= sum_{i in block b} L(i, j) x_i for pivot columns j < r0 (chunks of 64 columns); the
-// block's rows of those columns are staged through LDS, one thread per (column, rhs group).
-template <class T>
-__global__ void __launch_bounds__(256) k_bwd_update(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int b,
-                                                     int p, T* Z) {
-  __shared__ T Lt[kSolveBlock][kSolveBlock + 1];
-  __shared__ T xb[kSolveBlock][kMaxRhsChunk];
-  const int s = list[blockIdx.y];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int r0 = b * kSolveBlock;
-  if (r0 >= ncol) return;
-  const int bs = min(kSolveBlock, ncol - r0);
-  const int j0 = blockIdx.x * kSolveBlock;
-  if (j0 >= r0) return;
-  const int jn = min(kSolveBlock, r0 - j0);
-  const int n = S.n;
-  const T* P = N.L + S.panel_off[s];
-  const int8_t* pt = N.ptype + first;
-  T* zs = Z + first;
-  const int jj = threadIdx.x % kSolveBlock, cq = threadIdx.x / kSolveBlock;
-  for (int idx = threadIdx.x; idx < kSolveBlock * kSolveBlock; idx += blockDim.x) {
-    const int t = idx % kSolveBlock, j = idx / kSolveBlock;
-    Lt[t][j] = (t < bs && j < jn) ? lpiv_at(P, m, r0 + t, j0 + j, pt) : (T)0;
-  }
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
-    const int pc = min(kMaxRhsChunk, p - c0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < kSolveBlock * kMaxRhsChunk; idx += blockDim.x) {
-      const int t = idx % kSolveBlock, c = idx / kSolveBlock;
-      xb[t][c] = (t < bs && c < pc) ? zs[r0 + t + (int64_t)(c0 + c) * n] : (T)0;
     }
     __syncthreads();
-    T acc0 = (T)0, acc1 = (T)0;
-    for (int t = 0; t < bs; ++t) {
-      const T l = Lt[t][jj];
-      acc0 += l * xb[t][cq];
-      acc1 += l * xb[t][cq + 4];
+#pragma unroll
+    for (int c = 0; c < kMaxRhsChunk; ++c) red[g][lane][c] = acc[c];
+    __syncthreads();
+    if (split_combine<T>(red, nchunk, chunk, slot0, pc, part, counter, cidx) && jin) {
+      const int64_t xrow = rows[N.piv[first + j]];
+      for (int c = g; c < pc; c += 4) X[xrow + (int64_t)(c0 + c) * ldx] = (R)red[0][lane][c];
     }
-    if (jj < jn) {
-      if (cq < pc) zs[j0 + jj + (int64_t)(c0 + cq) * n] -= acc0;
-      if (cq + 4 < pc) zs[j0 + jj + (int64_t)(c0 + cq + 4) * n] -= acc1;
-    }
-  }
-}
-
-template <class T, class R>
-__global__ void __launch_bounds__(256) k_bwd_scatter(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, R* X,
-                                                      int64_t ldx, int p, const T* Z) {
-  const int s = list[blockIdx.x];
-  const int ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int n = S.n;
-  const int32_t* rows = S.rows_orig + S.sn_rowoff[s];
-  for (int idx = threadIdx.x; idx < ncol * p; idx += blockDim.x) {
-    const int i = idx % ncol, c = idx / ncol;
-    X[rows[N.piv[first + i]] + (int64_t)c * ldx] = (R)Z[first + i + (int64_t)c * n];
+    __syncthreads();
   }
 }
 
