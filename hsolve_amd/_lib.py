@@ -55,6 +55,8 @@ SIGNATURES = {
     "hsg_column_dots": (C.c_int, [_I64, _P, _I64, _P, _I64, _I64, _P, _P, _P]),
     "hsg_axpby": (C.c_int, [_I64, _D, _P, _I64, _D, _P, _I64, _I64, _P]),
     "hsg_pair_sums": (C.c_int, [_I64, _P, _P, _P, _I64, _I64, _P, _P, _P]),
+    "hsg_timing_enable": (C.c_int, [C.c_int]),
+    "hsg_timing_read": (C.c_int, [C.c_int, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -45,7 +45,7 @@ def build(verbose: bool = False) -> Path:
     jobs = []
     for src in sorted((CSRC / "host").glob("*.cpp")):
         jobs.append([cxx, *CXX_FLAGS, "-c", str(src), "-o", str(BUILD / (src.stem + ".o"))])
-    for src in sorted((CSRC / "kernels").glob("*.hip")):
+    for src in sorted((CSRC / "kernels").glob("*.hip")) + sorted((CSRC / "kernels").glob("*.cpp")):
         jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(BUILD / (src.stem + ".o"))])
     jobs.append([hipcc, *HIP_FLAGS, "-c", str(CSRC / "capi.cpp"), "-o", str(BUILD / "capi.o")])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:

@@ -259,4 +259,19 @@ int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, doubl
   });
 }
 
+int hsg_timing_enable(int on) {
+  return guarded([&] { hsg::timing_enable(on != 0); });
+}
+
+int hsg_timing_read(int phase, double* ms, int64_t* launches, double* work, int64_t* calls) {
+  return guarded([&] {
+    if (phase < 0 || phase >= hsg::kPhaseCount) throw InvalidInput("timing phase out of range");
+    const hsg::PhaseTotals t = hsg::timing_read(phase);
+    if (ms) *ms = t.ms;
+    if (launches) *launches = t.launches;
+    if (work) *work = t.work;
+    if (calls) *calls = t.calls;
+  });
+}
+
 }  // extern "C"

@@ -11,6 +11,7 @@
 
 #include "../host/analysis.hpp"
 #include "solve.hpp"
+#include "timing.hpp"
 
 namespace hsg {
 
@@ -88,6 +89,7 @@ class Engine {
   std::vector<int64_t> h_cptr_;
   std::vector<int32_t> h_cidx_;
   int64_t t_total_ = 0;
+  double g_flops_ = 0.0;  // algorithmic flops of the inverse panels: sum of cb ncol^2 + ncol^3 / 3
   int n_pairs_ = 0, n_ttiles_ = 0;
   int32_t *d_pairs_ = nullptr, *d_ttiles_ = nullptr, *d_cidx_ = nullptr;
   int64_t* d_cptr_ = nullptr;

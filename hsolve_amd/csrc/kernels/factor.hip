@@ -1003,6 +1003,10 @@ void Engine::plan_solve() {
     }
   }
   n_pairs_ = (int)(h_pairs_.size() / 2);
+  for (int s = 0; s < nsn; ++s) {
+    const double nc = sym_.sn_ncol[s], cb = sym_.sn_m[s] - sym_.sn_ncol[s];
+    g_flops_ += cb * nc * nc + nc * nc * nc / 3.0;
+  }
   auto task = [](int64_t a, int32_t lda, int8_t abuf, int64_t b, int32_t ldb, int8_t bbuf, int64_t c, int32_t ldc,
                  int8_t cbuf, int rows, int cols, int k0, int k1, bool neg) {
     GemmTask t;
@@ -1215,6 +1219,13 @@ void Engine::alloc_numeric(int precision) {
   precision_ = precision;
 }
 
+// a kernel launch counted in `nl` (launches of a timed phase)
+#define HSG_LAUNCH_COUNTED(...)     \
+  do {                              \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+    ++nl;                           \
+  } while (0)
+
 template <class T>
 InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
                                   hipStream_t st) {
@@ -1223,22 +1234,24 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
   const int nb = opt_.panel_width;
   const int detect = null_tol > 0 ? 1 : 0;
-  hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for(panel_total_)), dim3(256), 0, st, N.L, panel_total_, (T)0);
+  const int64_t tok = timing_begin(kPhaseFactor, st);
+  int64_t nl = 0;
+  HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(panel_total_)), dim3(256), 0, st, N.L, panel_total_, (T)0);
   if (cb_total_ > 0)
-    hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for(cb_total_)), dim3(256), 0, st, N.CB, cb_total_, (T)0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, d_cnt_, (int)kCntSlots);
+    HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(cb_total_)), dim3(256), 0, st, N.CB, cb_total_, (T)0);
+  HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, d_cnt_, (int)kCntSlots);
   if (nnz_ > 0)
-    hipLaunchKernelGGL(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
+    HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
                        d_ib_, av, bv, alpha, beta, N.L, d_cnt_);
   for (const Level& L : levels_) {
     if (L.n_small) {
       const int m = L.small_maxm;
       const size_t shm = (size_t)m * m * sizeof(T) + 4 * (size_t)m * sizeof(T) + (size_t)m * sizeof(int) + m + 16;
-      hipLaunchKernelGGL(k_factor_small<T>, dim3(L.n_small), dim3(kThreads), shm, st, S, N, L.d_small, null_tol,
+      HSG_LAUNCH_COUNTED(k_factor_small<T>, dim3(L.n_small), dim3(kThreads), shm, st, S, N, L.d_small, null_tol,
                          detect);
     }
     if (L.n_large) {
-      hipLaunchKernelGGL(k_extend_add_large<T>, dim3((L.large_maxm + kEaCols - 1) / kEaCols, L.n_large),
+      HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_maxm + kEaCols - 1) / kEaCols, L.n_large),
                          dim3(kThreads), 0, st, S, N, L.d_large);
       const int npan = (L.large_max_ncol + nb - 1) / nb;
       const size_t shm_bk = (size_t)nb * nb * sizeof(T) + 4 * (size_t)nb * sizeof(T) + nb * sizeof(int) + nb + 16;
@@ -1247,25 +1260,26 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
-        hipLaunchKernelGGL(k_bk_diag<T>, dim3(L.n_large), dim3(64), 0, st, S, N, L.d_large, p, nb, d_bperm_, null_tol,
+        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large), dim3(64), 0, st, S, N, L.d_large, p, nb, d_bperm_, null_tol,
                            detect, (T*)d_pinv_);
-        hipLaunchKernelGGL(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
+        HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
                            nb, (const int32_t*)d_bperm_, (const T*)d_pinv_);
         const int rem = L.large_maxm - (p + 1) * nb;
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
           const int tc = (L.large_max_ncol - (p + 1) * nb + kTile - 1) / kTile;
-          hipLaunchKernelGGL(k_gemm_update<T>, dim3(t * tc, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 0, p,
+          HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * tc, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 0, p,
                              nb);
         }
       }
       if (L.large_max_cb > 0) {
         const int t = (L.large_max_cb + kTile - 1) / kTile;
-        hipLaunchKernelGGL(k_gemm_update<T>, dim3(t * t, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0, nb);
+        HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * t, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0, nb);
       }
     }
   }
   hip_check(hipGetLastError(), "factorization kernels");
+  timing_end(tok, st, nl, sym_.flops);
   unsigned long long h[kCntSlots];
   hip_check(hipMemcpyAsync(h, d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
   hip_check(hipStreamSynchronize(st), "factorization");
@@ -1302,14 +1316,24 @@ void Engine::build_inverse_panels(hipStream_t st) {
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
   T* G = (T*)d_G_;
-  if (n_pairs_ > 0) hipLaunchKernelGGL(k_diag_block_inverse<T>, dim3(n_pairs_), dim3(64), 0, st, S, N, d_pairs_, G);
+  const int64_t tok = timing_begin(kPhaseInverse, st);
+  int64_t nl = 0;
+  if (n_pairs_ > 0) {
+    hipLaunchKernelGGL(k_diag_block_inverse<T>, dim3(n_pairs_), dim3(64), 0, st, S, N, d_pairs_, G);
+    ++nl;
+  }
   for (const TaskRange& r : gstages_)
-    if (r.count > 0)
+    if (r.count > 0) {
       hipLaunchKernelGGL(k_tile_gemm<T>, dim3((unsigned)r.count), dim3(256), 0, st, d_tasks_ + r.begin, N.L, G,
                          (T*)d_T_);
-  if (n_ttiles_ > 0)
+      ++nl;
+    }
+  if (n_ttiles_ > 0) {
     hipLaunchKernelGGL(k_transpose_g<T>, dim3(n_ttiles_), dim3(256), 0, st, S, d_ttiles_, (const T*)G, (T*)d_Gt_);
+    ++nl;
+  }
   hip_check(hipGetLastError(), "inverse panel kernels");
+  timing_end(tok, st, nl, g_flops_);
   g_ready_ = true;
 }
 
@@ -1344,6 +1368,10 @@ void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
   const T* Gt = (const T*)d_Gt_;
   const int p = (int)nrhs;
   const int64_t ldu = rows_total_;
+  // algorithmic bytes per sweep: every stored factor entry once, the right-hand sides read and
+  // written once (FP64)
+  const double sweep_bytes = sym_.factor_entries * (double)sizeof(T) + 2.0 * sym_.n * (double)p * sizeof(double);
+  int64_t tok = timing_begin(kPhaseForward, st);
   // forward: gather z, then [y1; u] = G z1 + [0; u] per front
   for (const Level& L : levels_) {
     hipLaunchKernelGGL((k_fwd_gather<T, double>), dim3((L.max_m + kGatherRows - 1) / kGatherRows, L.n_all),
@@ -1352,14 +1380,17 @@ void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
     hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
                        (const T*)Z, Y, U, ldu, part, gvc);
   }
+  timing_end(tok, st, 2 * (int64_t)levels_.size(), sweep_bytes);
   hipLaunchKernelGGL(k_solve_diag<T>, dim3(blocks_for((int64_t)sym_.n * p)), dim3(256), 0, st, sym_.n, N.dval,
                      N.doff, N.ptype, Y, p);
+  tok = timing_begin(kPhaseBackward, st);
   // backward: x1 = G^T [y1; x_cb] per front, top-down
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
     hipLaunchKernelGGL((k_bwd_gemv<T, double>), dim3(L.bwd_tiles, L.n_all, L.bwd_chunks), dim3(256), 0, st, S, N, Gt,
                        L.d_all, p, (const T*)Y, x, ldx, part, gvc);
   }
+  timing_end(tok, st, (int64_t)levels_.size(), sweep_bytes);
   hip_check(hipGetLastError(), "solve kernels");
 }
 

@@ -21,6 +21,7 @@ from __future__ import annotations
 import math
 import resource
 import time
+from dataclasses import dataclass
 
 import numpy as np
 import scipy.sparse as sp
@@ -73,23 +74,55 @@ def _b_kind(B):
     return "diagonal" if off.nnz == 0 else "general"
 
 
-def solve(A, B=None, config: dict | None = None):
-    """Returns (eigenvalues ndarray, eigenvectors ndarray (n, m) Fortran order, report dict, config dict)."""
+@dataclass
+class Prepared:
+    """Checked input of one pencil with its device copies (see prepare())."""
+    A: sp.csr_matrix
+    B: sp.csr_matrix | None
+    b_kind: str
+    defect_a: float
+    defect_b: float
+    pencil: Pencil
+
+
+def prepare(A, B=None, config: dict | None = None) -> Prepared:
+    """Input stage of a solve (solver.cpp:70-114 and the pencil set-up of pencil.cpp:67-170): the
+    exact Hermitian parts of A and B with their defects, the B kind, the union pattern and the
+    device copies of the values. A Prepared pencil can be solved many times (solve(prepared=...))."""
     cfg = cfgmod.parse_config(config or {})
-    rep = cfgmod.new_report()
-    clock = PhaseClock()
-    clock.push("setup")
-    t_start = time.perf_counter()
     A, def_a = hermitian_input(A, "A", cfg["hermitian_tol"])
-    rep["hermitian_defect_a"] = def_a
     bk = _b_kind(B)
+    def_b = 0.0
     if B is not None:
         B, def_b = hermitian_input(B, "B", cfg["hermitian_tol"])
         if B.shape != A.shape:
             raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {B.shape}")
-        rep["hermitian_defect_b"] = def_b
         if bk == "diagonal":
             B = sp.diags(B.diagonal(), format="csr")
+    if B is not None and bk == "diagonal" and np.any(B.diagonal() <= 0):
+        raise UnsupportedError("B is diagonal but not positive definite (GHIEP): not supported by this engine")
+    pencil = Pencil(A, B)
+    pencil.to_device()
+    return Prepared(A, B, bk, def_a, def_b, pencil)
+
+
+def solve(A=None, B=None, config: dict | None = None, *, prepared: Prepared | None = None, vectors: str = "host"):
+    """Returns (eigenvalues ndarray, eigenvectors, report dict, config dict). The eigenvectors are an
+    (n, m) Fortran-order ndarray (vectors="host") or an (m, n) float64 tensor on the device whose
+    row j is eigenvector j (vectors="device"). `prepared` (from prepare() on the same matrices)
+    skips the input stage and the uploads."""
+    if vectors not in ("host", "device"):
+        raise InvalidInputError(f"vectors = {vectors!r} is not 'host' or 'device'")
+    cfg = cfgmod.parse_config(config or {})
+    rep = cfgmod.new_report()
+    clock = PhaseClock()
+    clock.push("setup")
+    if prepared is None:
+        prepared = prepare(A, B, cfg)
+    A, B, bk, pencil = prepared.A, prepared.B, prepared.b_kind, prepared.pencil
+    rep["hermitian_defect_a"] = prepared.defect_a
+    if B is not None:
+        rep["hermitian_defect_b"] = prepared.defect_b
     n = A.shape[0]
     rep["n"] = n
     rep["nnz_a"] = int(A.nnz)
@@ -97,9 +130,6 @@ def solve(A, B=None, config: dict | None = None):
     rep["sigma_requested"] = cfg["target"]["sigma"]
     if cfg["problem"] == "generalized_indefinite":
         raise UnsupportedError("indefinite B (GHIEP) is not supported by this engine")
-    if B is not None and bk == "diagonal" and np.any(B.diagonal() <= 0):
-        raise UnsupportedError("B is diagonal but not positive definite (GHIEP): not supported by this engine")
-    pencil = Pencil(A, B)
     ops = DeviceOps(pencil)
     n2a = estimate_norm2(ops, ops.apply_a, 30, cfg["seed"])
     n2b = 1.0 if B is None else estimate_norm2(ops, ops.apply_b, 30, cfg["seed"])
@@ -156,7 +186,10 @@ def solve(A, B=None, config: dict | None = None):
     # report
     with clock.phase("verify"):
         lam = np.asarray(sel.lam, dtype=np.float64)
-        vecs = sel.x.detach().cpu().numpy().T.copy(order="F") if sel.size else np.zeros((n, 0), order="F")
+        if vectors == "device":
+            vecs = sel.x.detach() if sel.size else torch.zeros((0, n), dtype=torch.float64, device=ops.dev)
+        else:
+            vecs = sel.x.detach().cpu().numpy().T.copy(order="F") if sel.size else np.zeros((n, 0), order="F")
     all_ok = ok
     for j, m in enumerate(metrics):
         acc = m.backward_error <= cfg["tol"]

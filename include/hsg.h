@@ -191,6 +191,17 @@ int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x
 int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
               void* stream);
 
+/* ---------------------------------------------------------------- measurement
+ * No counterpart in the reference (its timings are host wall-clock phases, SolveReport time_*_s,
+ * include/hsolve/solver.hpp). Phase timing of factorizations and solves with HIP events recorded
+ * on each call's stream: phase 0 numeric factorization (work = flops of the symbolic analysis),
+ * 1 inverse panels built on the first solve after a factorization (flops), 2 forward sweep and
+ * 3 backward sweep of a solve (bytes: every stored factor entry once plus the right-hand sides
+ * read and written once). hsg_timing_enable clears the records (on = 0 also stops recording);
+ * hsg_timing_read synchronizes the recorded events and returns the sums over all calls. */
+int hsg_timing_enable(int on);
+int hsg_timing_read(int phase, double* ms, int64_t* launches, double* work, int64_t* calls);
+
 #ifdef __cplusplus
 }
 #endif

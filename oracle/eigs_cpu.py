@@ -81,3 +81,26 @@ def solve_slice(A, B, lower: float, upper: float, tol: float = 1e-11, seed: int 
     res.update({"count_inertia": want, "count_found": found, "passed": found == want,
                 "time_certificate_s": res["time_certificate_s"] + t_count, "time_total_s": time.perf_counter() - t0})
     return res
+
+
+def factor_solve_sample(A, B, sigma: float, perm: np.ndarray, p: int = 8, seed: int = 2026) -> dict:
+    """The CPU path's two hot operations on one shifted pencil, timed (the bench's bounded CPU
+    sample): the elimination of K = A - sigma B in the fill-reducing order `perm` (new -> old; the
+    order the reference hands MUMPS as PERM_IN, include/hsolve/factor.hpp:598-633) with SuperLU
+    in symmetric mode and diagonal pivots (the LDL^T-equivalent elimination, one thread), then one
+    solve with p right-hand sides (the reference's multi-RHS solve,
+    src/factor/mumps_factorization.cpp:458-527). Returns the two times and the LU size."""
+    n = A.shape[0]
+    Bm = sp.identity(n, format="csc") if B is None else sp.csc_matrix(B)
+    perm = np.asarray(perm, dtype=np.int64)
+    K = (sp.csc_matrix(A) - sigma * Bm).tocsc()[perm][:, perm].tocsc()
+    t0 = time.perf_counter()
+    lu = sla.splu(K, permc_spec="NATURAL", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
+    t_factor = time.perf_counter() - t0
+    rhs = np.random.default_rng(seed).standard_normal((n, p))
+    t0 = time.perf_counter()
+    x = lu.solve(rhs)
+    t_solve = time.perf_counter() - t0
+    resid = float(np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs))
+    return {"factor_s": t_factor, "solve_s": t_solve, "p": p, "lu_entries": int(lu.L.nnz + lu.U.nnz),
+            "relative_residual": resid, "cores": 1}
