@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # no packed FP32 instructions (v_pk_*_f32) in any kernel: some of their forms have
              # given silently wrong results on this hardware; scalar FP32 code is used instead.
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread"]
 
 
 def _hipcc() -> str:
@@ -52,7 +52,7 @@ def build(verbose: bool = False) -> Path:
         list(ex.map(_run, jobs))
     objs = [j[-1] for j in jobs]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])
+    _run([hipcc, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

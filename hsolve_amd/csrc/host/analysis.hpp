@@ -17,6 +17,7 @@ namespace hsg {
 struct OrderingOptions {
   int leaf_size = 64;        // subgraphs at most this large are not dissected further
   int min_degree_leaves = 1; // 1: minimum-degree order inside leaves, 0: natural order
+  int threads = 0;           // worker threads of the dissection (0: min(16, hardware threads))
 };
 
 struct AmalgamationOptions {

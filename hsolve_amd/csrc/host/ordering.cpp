@@ -9,10 +9,15 @@
 // last; leaves get a minimum-degree order computed on bit-set elimination graphs. The result is
 // deterministic (a fixed-seed generator drives the matching order).
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -379,18 +384,32 @@ void min_degree_leaf(const Graph& g, const std::vector<int32_t>& verts, const st
 }  // namespace
 
 std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt) {
+  // Subgraphs are independent once their separator is numbered, so they are dissected by a pool
+  // of threads. Every task owns the number range [lo, hi) of its vertices and a generator seeded
+  // from its position in the dissection tree, so the result does not depend on which thread
+  // runs which task or when.
   const int32_t n = g.n;
   std::vector<int32_t> perm(n, -1);
-  std::vector<int32_t> label(n, 0);   // subgraph id of each unnumbered vertex
-  std::vector<int32_t> local(n, -1);  // local index scratch
-  Rng rng(0x5EED2026ull);
+  // subgraph id of each unnumbered vertex; read for neighbours that other tasks may relabel at the
+  // same time (a neighbour outside the task never carries the task's id, before or after)
+  std::vector<std::atomic<int32_t>> label(n);
+  for (auto& l : label) l.store(0, std::memory_order_relaxed);
   struct Task {
     std::vector<int32_t> verts;
-    int32_t lo, hi;  // numbers [lo, hi) are assigned to these vertices
-    int32_t id;
+    int32_t lo = 0, hi = 0;  // numbers [lo, hi) are assigned to these vertices
+    int32_t id = 0;
+    uint64_t seed = 0;
   };
-  int32_t next_id = 1;
+  auto child_seed = [](uint64_t parent, uint64_t k) {
+    Rng r(parent ^ (0x9E3779B97F4A7C15ull * (k + 1)));
+    return r.next();
+  };
+  std::atomic<int32_t> next_id{1};
+  std::mutex mu;
+  std::condition_variable cv;
   std::vector<Task> stack;
+  int busy = 0;
+  std::exception_ptr failure;
   {
     Task t;
     t.verts.resize(n);
@@ -398,14 +417,15 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
     t.lo = 0;
     t.hi = n;
     t.id = 0;
+    t.seed = 0x5EED2026ull;
     stack.push_back(std::move(t));
   }
-  while (!stack.empty()) {
-    Task t = std::move(stack.back());
-    stack.pop_back();
+  // one task: a leaf gets its minimum-degree order, a component split or a bisection pushes
+  // the parts as new tasks; local is the calling thread's scratch map (all -1 between tasks)
+  auto process = [&](Task& t, std::vector<int32_t>& local, std::vector<Task>& out) {
     const int32_t nv = static_cast<int32_t>(t.verts.size());
-    if (nv == 0) continue;
-    for (int32_t v : t.verts) label[v] = t.id;
+    if (nv == 0) return;
+    for (int32_t v : t.verts) label[v].store(t.id, std::memory_order_relaxed);
     if (nv <= opt.leaf_size) {
       std::vector<int32_t> ord;
       ord.reserve(nv);
@@ -417,7 +437,7 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
         ord = t.verts;
       }
       for (int32_t a = 0; a < nv; ++a) perm[t.lo + a] = ord[a];
-      continue;
+      return;
     }
     // local weighted graph of the subgraph
     for (int32_t a = 0; a < nv; ++a) local[t.verts[a]] = a;
@@ -428,22 +448,23 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
       const int32_t v = t.verts[a];
       for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
         const int32_t u = g.adj[p];
-        if (label[u] == t.id && local[u] >= 0) wg.adj.push_back(local[u]);
+        if (label[u].load(std::memory_order_relaxed) == t.id) wg.adj.push_back(local[u]);
       }
       wg.ptr[a + 1] = static_cast<int64_t>(wg.adj.size());
     }
     wg.ew.assign(wg.adj.size(), 1);
     wg.vw.assign(nv, 1);
     wg.total_vw = nv;
+    for (int32_t v : t.verts) local[v] = -1;
     // connected components: dissect each separately (no separator needed between them)
     std::vector<int32_t> comp(nv, -1);
     int32_t ncomp = 0;
     {
       std::vector<int32_t> q;
-      for (int32_t s = 0; s < nv; ++s) {
-        if (comp[s] >= 0) continue;
-        q.assign(1, s);
-        comp[s] = ncomp;
+      for (int32_t s0 = 0; s0 < nv; ++s0) {
+        if (comp[s0] >= 0) continue;
+        q.assign(1, s0);
+        comp[s0] = ncomp;
         for (size_t h = 0; h < q.size(); ++h)
           for (int64_t p = wg.ptr[q[h]]; p < wg.ptr[q[h] + 1]; ++p)
             if (comp[wg.adj[p]] < 0) {
@@ -456,19 +477,20 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
     if (ncomp > 1) {
       std::vector<std::vector<int32_t>> parts(ncomp);
       for (int32_t a = 0; a < nv; ++a) parts[comp[a]].push_back(t.verts[a]);
-      for (int32_t v : t.verts) local[v] = -1;
       int32_t lo = t.lo;
-      for (auto& pv : parts) {
+      for (int32_t k = 0; k < ncomp; ++k) {
         Task c;
         c.lo = lo;
-        c.hi = lo + static_cast<int32_t>(pv.size());
+        c.hi = lo + static_cast<int32_t>(parts[k].size());
         lo = c.hi;
-        c.verts = std::move(pv);
-        c.id = next_id++;
-        stack.push_back(std::move(c));
+        c.verts = std::move(parts[k]);
+        c.id = next_id.fetch_add(1);
+        c.seed = child_seed(t.seed, static_cast<uint64_t>(k));
+        out.push_back(std::move(c));
       }
-      continue;
+      return;
     }
+    Rng rng(t.seed);
     std::vector<int8_t> part;
     multilevel_bisect(wg, part, rng);
     // vertex separator: boundary vertices of the side with fewer of them
@@ -504,14 +526,13 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
       const int32_t v = t.verts[a];
       (where[a] == 0 ? p0 : where[a] == 1 ? p1 : sep).push_back(v);
     }
-    for (int32_t v : t.verts) local[v] = -1;
     if (p0.empty() || p1.empty()) {  // degenerate: order as a leaf
       for (int32_t a = 0; a < nv; ++a) perm[t.lo + a] = t.verts[a];
-      continue;
+      return;
     }
     const int32_t ns = static_cast<int32_t>(sep.size());
     for (int32_t a = 0; a < ns; ++a) perm[t.hi - ns + a] = sep[a];
-    for (int32_t v : sep) label[v] = -1;
+    for (int32_t v : sep) label[v].store(-1, std::memory_order_relaxed);
     Task c0, c1;
     c0.lo = t.lo;
     c0.hi = t.lo + static_cast<int32_t>(p0.size());
@@ -519,11 +540,48 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
     c1.hi = t.hi - ns;
     c0.verts = std::move(p0);
     c1.verts = std::move(p1);
-    c0.id = next_id++;
-    c1.id = next_id++;
-    stack.push_back(std::move(c0));
-    stack.push_back(std::move(c1));
-  }
+    c0.id = next_id.fetch_add(1);
+    c1.id = next_id.fetch_add(1);
+    c0.seed = child_seed(t.seed, 0);
+    c1.seed = child_seed(t.seed, 1);
+    out.push_back(std::move(c0));
+    out.push_back(std::move(c1));
+  };
+  auto worker = [&]() {
+    std::vector<int32_t> local(n, -1);
+    for (;;) {
+      Task t;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !stack.empty() || busy == 0; });
+        if (stack.empty()) return;
+        t = std::move(stack.back());
+        stack.pop_back();
+        ++busy;
+      }
+      std::vector<Task> out;
+      try {
+        if (!failure) process(t, local, out);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!failure) failure = std::current_exception();
+        out.clear();
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Task& c : out) stack.push_back(std::move(c));
+        --busy;
+      }
+      cv.notify_all();
+    }
+  };
+  int nthreads = opt.threads > 0 ? opt.threads : std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 4096) nthreads = 1;
+  std::vector<std::thread> pool;
+  for (int i = 1; i < nthreads; ++i) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  if (failure) std::rethrow_exception(failure);
   for (int32_t i = 0; i < n; ++i)
     if (perm[i] < 0) throw std::runtime_error("nested_dissection: incomplete ordering");
   return perm;
