@@ -60,7 +60,9 @@ class Engine {
     int large_max_ncol = 0;
     int large_max_cb = 0;
     int max_m = 0, max_ncol = 0;  // over all fronts of the level (solve launches)
-    int fwd_tiles = 0, fwd_chunks = 0, bwd_tiles = 0, bwd_chunks = 0;  // split GEMV grids
+    int fwd_tiles = 0, fwd_chunks = 0;   // forward split-GEMV grid
+    int64_t bt_begin = 0, bt_count = 0;  // backward tasks of the level
+    int64_t bh_begin = 0, bh_count = 0;  // its split column tiles (their chunk-0 tasks)
   };
   struct TaskRange {
     int64_t begin = 0, count = 0;
@@ -85,6 +87,8 @@ class Engine {
   // and the pull map of the forward gather (per front row, the children's U rows mapped onto it)
   std::vector<int32_t> h_pairs_, h_ttiles_;
   std::vector<GemmTask> h_tasks_;
+  std::vector<BwdTask> h_btasks_, h_bheads_;
+  BwdTask *d_btasks_ = nullptr, *d_bheads_ = nullptr;
   std::vector<TaskRange> gstages_;
   std::vector<int64_t> h_cptr_;
   std::vector<int32_t> h_cidx_;
@@ -113,12 +117,12 @@ class Engine {
   int8_t* d_ptype_ = nullptr;
   int32_t *d_piv_ = nullptr, *d_ipiv_ = nullptr, *d_bperm_ = nullptr;
   unsigned long long* d_cnt_ = nullptr;
-  void *d_Z_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;
+  void *d_Xp_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;  // solve work arrays (pivot numbering, interleaved)
   int64_t ws_cols_ = 0;
   void *d_G_ = nullptr, *d_Gt_ = nullptr, *d_T_ = nullptr;  // inverse panels (solve phase)
-  void* d_part_ = nullptr;      // partial sums of split GEMV tiles
-  int* d_gv_count_ = nullptr;   // arrival counters of split GEMV tiles (kept at 0 between uses)
-  int64_t part_slots_ = 0, gv_counters_ = 0;  // per group of kMaxRhsChunk right-hand sides
+  int32_t *d_pos_ = nullptr, *d_rpos_ = nullptr;  // pivot position of each original row / front row
+  void* d_part_ = nullptr;       // partial sums of split product tiles
+  int64_t part_slots_ = 0;       // slots per group of kMaxRhsChunk right-hand sides
   bool g_ready_ = false;
 };
 

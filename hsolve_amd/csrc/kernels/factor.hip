@@ -974,17 +974,8 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
     L.n_large = (int)L.h_large.size();
     L.fwd_tiles = (L.max_m + kGvTile - 1) / kGvTile;
     L.fwd_chunks = (L.max_ncol + kGvSplit - 1) / kGvSplit;
-    L.bwd_tiles = (L.max_ncol + kGvTile - 1) / kGvTile;
-    L.bwd_chunks = (L.max_m + kGvSplit - 1) / kGvSplit;
     // scratch of the split products (slots and counters per group of right-hand sides)
-    if (L.fwd_chunks > 1) {
-      part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks);
-      gv_counters_ = std::max<int64_t>(gv_counters_, (int64_t)L.n_all * L.fwd_tiles);
-    }
-    if (L.bwd_chunks > 1) {
-      part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.bwd_tiles * L.bwd_chunks);
-      gv_counters_ = std::max<int64_t>(gv_counters_, (int64_t)L.n_all * L.bwd_tiles);
-    }
+    if (L.fwd_chunks > 1) part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks);
     n_small_ += L.n_small;
     n_large_ += L.n_large;
     max_large_per_level_ = std::max(max_large_per_level_, L.n_large);
@@ -1082,6 +1073,42 @@ void Engine::plan_solve() {
       }
   }
   n_ttiles_ = (int)(h_ttiles_.size() / 3);
+  // backward tasks per level: column tiles of at most 64 (width rounded up to a power of two >= 8:
+  // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group
+  for (Level& L : levels_) {
+    L.bt_begin = (int64_t)h_btasks_.size();
+    L.bh_begin = (int64_t)h_bheads_.size();
+    int64_t slots = 0;
+    for (int s : L.h_all) {
+      const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
+      for (int j0 = 0; j0 < ncol; j0 += kGvTile) {
+        const int jn = std::min(kGvTile, ncol - j0);
+        int jw = 8;
+        while (jw < jn) jw *= 2;
+        const int rc = 32 * (256 / jw);
+        const int rows = m - j0;
+        const int nchunk = (rows + rc - 1) / rc;
+        BwdTask t{};
+        t.s = s;
+        t.j0 = j0;
+        t.jn = jn;
+        t.jw = jw;
+        t.nchunk = nchunk;
+        t.slot0 = nchunk > 1 ? slots : -1;
+        for (int c = 0; c < nchunk; ++c) {
+          t.chunk = c;
+          t.i0 = j0 + c * rc;
+          t.in = std::min(rc, m - t.i0);
+          h_btasks_.push_back(t);
+          if (c == 0 && nchunk > 1) h_bheads_.push_back(t);
+        }
+        if (nchunk > 1) slots += nchunk;
+      }
+    }
+    L.bt_count = (int64_t)h_btasks_.size() - L.bt_begin;
+    L.bh_count = (int64_t)h_bheads_.size() - L.bh_begin;
+    part_slots_ = std::max(part_slots_, slots);
+  }
   // pull map of the forward gather: for front row r (index rowoff + r), the children's U rows
   // mapped onto it, children in child-list order
   h_cptr_.assign(rows_total_ + 1, 0);
@@ -1129,10 +1156,18 @@ void Engine::validate_solve_plan() const {
   for (const Level& L : levels_) {
     if (L.n_all != (int)L.h_all.size()) bad("level size");
     if (L.fwd_chunks > 1 && (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks > part_slots_) bad("forward split scratch");
-    if (L.bwd_chunks > 1 && (int64_t)L.n_all * L.bwd_tiles * L.bwd_chunks > part_slots_) bad("backward split scratch");
-    if (L.fwd_chunks > 1 && (int64_t)L.n_all * L.fwd_tiles > gv_counters_) bad("forward split counters");
-    if (L.bwd_chunks > 1 && (int64_t)L.n_all * L.bwd_tiles > gv_counters_) bad("backward split counters");
+    if (L.bt_begin < 0 || L.bt_begin + L.bt_count > (int64_t)h_btasks_.size()) bad("backward task range");
+    if (L.bh_begin < 0 || L.bh_begin + L.bh_count > (int64_t)h_bheads_.size()) bad("backward split range");
   }
+  for (const BwdTask& t : h_btasks_) {
+    if (t.s < 0 || t.s >= sym_.nsn) bad("backward task front");
+    const int ncol = sym_.sn_ncol[t.s], m = sym_.sn_m[t.s];
+    if (t.jw < 8 || t.jw > kGvTile || (t.jw & (t.jw - 1)) || t.jn < 1 || t.jn > t.jw) bad("backward task width");
+    if (t.j0 < 0 || t.j0 + t.jn > ncol || t.i0 < t.j0 || t.in < 1 || t.i0 + t.in > m) bad("backward task range");
+    if (t.nchunk > 1 && (t.slot0 < 0 || t.slot0 + t.nchunk > part_slots_)) bad("backward task scratch");
+  }
+  for (const BwdTask& t : h_bheads_)
+    if (t.chunk != 0 || t.nchunk < 2 || t.slot0 < 0 || t.slot0 + t.nchunk > part_slots_) bad("backward split head");
 }
 
 void Engine::ensure_device() {
@@ -1156,6 +1191,8 @@ void Engine::ensure_device() {
   d_pairs_ = upload(h_pairs_);
   d_ttiles_ = upload(h_ttiles_);
   d_tasks_ = upload(h_tasks_);
+  d_btasks_ = upload(h_btasks_);
+  d_bheads_ = upload(h_bheads_);
   d_cptr_ = upload(h_cptr_);
   d_cidx_ = upload(h_cidx_);
   for (Level& L : levels_) {
@@ -1171,7 +1208,8 @@ Engine::~Engine() {
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
                   (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
-                  (void*)d_tasks_, (void*)d_cptr_, (void*)d_cidx_})
+                  (void*)d_tasks_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_cptr_,
+                  (void*)d_cidx_})
     dfree(p);
   for (auto& L : levels_) {
     dfree(L.d_all);
@@ -1182,11 +1220,11 @@ Engine::~Engine() {
 
 void Engine::free_numeric() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
-                  (void*)d_bperm_, (void*)d_cnt_, d_Z_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
-                  (void*)d_gv_count_})
+                  (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
+                  (void*)d_pos_, (void*)d_rpos_})
     dfree(p);
-  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Z_ = d_Y_ = d_U_ = d_pinv_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
-  d_gv_count_ = nullptr;
+  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Xp_ = d_Y_ = d_U_ = d_pinv_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
+  d_pos_ = d_rpos_ = nullptr;
   g_ready_ = false;
   d_ptype_ = nullptr;
   d_piv_ = d_ipiv_ = d_bperm_ = nullptr;
@@ -1312,6 +1350,9 @@ void Engine::build_inverse_panels(hipStream_t st) {
   if (!d_G_) hip_check(hipMalloc(&d_G_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
   if (!d_Gt_) hip_check(hipMalloc(&d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
   if (!d_T_) hip_check(hipMalloc(&d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
+  if (!d_pos_) hip_check(hipMalloc(&d_pos_, (size_t)sym_.n * sizeof(int32_t)), "hipMalloc (pivot positions)");
+  if (!d_rpos_)
+    hip_check(hipMalloc(&d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t)), "hipMalloc (pivot positions)");
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
@@ -1332,6 +1373,11 @@ void Engine::build_inverse_panels(hipStream_t st) {
     hipLaunchKernelGGL(k_transpose_g<T>, dim3(n_ttiles_), dim3(256), 0, st, S, d_ttiles_, (const T*)G, (T*)d_Gt_);
     ++nl;
   }
+  // pivot positions of the rows after this factorization's local pivoting
+  hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, d_pos_);
+  hipLaunchKernelGGL(k_row_positions, dim3(blocks_for(rows_total_)), dim3(256), 0, st, rows_total_,
+                     (const int32_t*)d_rows_orig_, (const int32_t*)d_pos_, d_rpos_);
+  nl += 2;
   hip_check(hipGetLastError(), "inverse panel kernels");
   timing_end(tok, st, nl, g_flops_);
   g_ready_ = true;
@@ -1344,53 +1390,68 @@ void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
   if (ws_cols_ < nrhs) {
-    for (void* q : {d_Z_, d_Y_, d_U_, d_part_, (void*)d_gv_count_}) dfree(q);
-    d_Z_ = d_Y_ = d_U_ = d_part_ = nullptr;
-    d_gv_count_ = nullptr;
+    for (void* q : {d_Xp_, d_Y_, d_U_, d_part_}) dfree(q);
+    d_Xp_ = d_Y_ = d_U_ = d_part_ = nullptr;
     const int64_t nrc = (nrhs + kMaxRhsChunk - 1) / kMaxRhsChunk;
-    hip_check(hipMalloc(&d_Z_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
+    hip_check(hipMalloc(&d_Xp_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
     hip_check(hipMalloc(&d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
     hip_check(hipMalloc(&d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
               "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kGvTile * kMaxRhsChunk * sizeof(T)),
+    hip_check(hipMalloc(&d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T)),
               "hipMalloc (solve workspace)");
-    const size_t cbytes = (size_t)std::max<int64_t>(1, gv_counters_ * nrc) * sizeof(int);
-    hip_check(hipMalloc(&d_gv_count_, cbytes), "hipMalloc (solve workspace)");
-    hip_check(hipMemset(d_gv_count_, 0, cbytes), "hipMemset (solve workspace)");
     ws_cols_ = nrhs;
   }
   T* part = (T*)d_part_;
-  int* gvc = d_gv_count_;
-  T* Z = (T*)d_Z_;
+  T* Xp = (T*)d_Xp_;
   T* Y = (T*)d_Y_;
   T* U = (T*)d_U_;
   const T* G = (const T*)d_G_;
   const T* Gt = (const T*)d_Gt_;
+  const int32_t* pos = d_pos_;
+  const int32_t* rpos = d_rpos_;
   const int p = (int)nrhs;
-  const int64_t ldu = rows_total_;
   // algorithmic bytes per sweep: every stored factor entry once, the right-hand sides read and
   // written once (FP64)
   const double sweep_bytes = sym_.factor_entries * (double)sizeof(T) + 2.0 * sym_.n * (double)p * sizeof(double);
+  hipLaunchKernelGGL((k_permute_in<double, T>), dim3(blocks_for(sym_.n)), dim3(256), 0, st, sym_.n, p,
+                     (const double*)x, ldx, pos, Xp);
   int64_t tok = timing_begin(kPhaseForward, st);
-  // forward: gather z, then [y1; u] = G z1 + [0; u] per front
+  int64_t nl = 0;
+  // forward: gather z (in place in Xp), then [y1; u] = G z1 + [0; u] per front
   for (const Level& L : levels_) {
-    hipLaunchKernelGGL((k_fwd_gather<T, double>), dim3((L.max_m + kGatherRows - 1) / kGatherRows, L.n_all),
-                       dim3(kGatherRows), 0, st, S, N, L.d_all, (const double*)x, ldx, p, Z, U, ldu,
-                       (const int64_t*)d_cptr_, (const int32_t*)d_cidx_);
+    hipLaunchKernelGGL(k_fwd_gather<T>, dim3((L.max_m + kGatherRows - 1) / kGatherRows, L.n_all), dim3(kGatherRows), 0,
+                       st, S, L.d_all, p, Xp, U, rpos, (const int64_t*)d_cptr_, (const int32_t*)d_cidx_);
     hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
-                       (const T*)Z, Y, U, ldu, part, gvc);
+                       (const T*)Xp, Y, U, part);
+    nl += 2;
+    if (L.fwd_chunks > 1) {
+      hipLaunchKernelGGL(k_fwd_reduce<T>, dim3(L.fwd_tiles, L.n_all), dim3(256), 0, st, S, L.d_all, p, L.fwd_chunks, Y,
+                         U, (const T*)part);
+      ++nl;
+    }
   }
-  timing_end(tok, st, 2 * (int64_t)levels_.size(), sweep_bytes);
+  timing_end(tok, st, nl, sweep_bytes);
   hipLaunchKernelGGL(k_solve_diag<T>, dim3(blocks_for((int64_t)sym_.n * p)), dim3(256), 0, st, sym_.n, N.dval,
                      N.doff, N.ptype, Y, p);
   tok = timing_begin(kPhaseBackward, st);
-  // backward: x1 = G^T [y1; x_cb] per front, top-down
+  nl = 0;
+  // backward: x1 = G^T [y1; x_cb] per front, top-down, into Xp
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
-    hipLaunchKernelGGL((k_bwd_gemv<T, double>), dim3(L.bwd_tiles, L.n_all, L.bwd_chunks), dim3(256), 0, st, S, N, Gt,
-                       L.d_all, p, (const T*)Y, x, ldx, part, gvc);
+    if (L.bt_count > 0) {
+      hipLaunchKernelGGL(k_bwd_task<T>, dim3((unsigned)L.bt_count), dim3(256), 0, st, S, Gt, d_btasks_ + L.bt_begin, p,
+                         (const T*)Y, Xp, rpos, part);
+      ++nl;
+    }
+    if (L.bh_count > 0) {
+      hipLaunchKernelGGL(k_bwd_reduce<T>, dim3((unsigned)L.bh_count), dim3(256), 0, st, S, d_bheads_ + L.bh_begin, p, Xp,
+                         (const T*)part);
+      ++nl;
+    }
   }
-  timing_end(tok, st, (int64_t)levels_.size(), sweep_bytes);
+  timing_end(tok, st, nl, sweep_bytes);
+  hipLaunchKernelGGL((k_permute_out<T, double>), dim3(blocks_for(sym_.n)), dim3(256), 0, st, sym_.n, p, (const T*)Xp,
+                     pos, x, ldx);
   hip_check(hipGetLastError(), "solve kernels");
 }
 

@@ -11,9 +11,13 @@
 //                                  (pull gather, fixed order), then one GEMV per front:
 //                                  [ y1 ; u ] = G z1 + [ 0 ; z_cb ]
 //   diagonal:                      y = D^-1 y (1x1 and 2x2 blocks)
-//   backward, top-down per level:  x1 = G^T [ y1 ; x_cb ] with x_cb read from X (rows of
-//                                  ancestors, final), one GEMV per front, scattered into X.
+//   backward, top-down per level:  x1 = G^T [ y1 ; x_cb ] with x_cb the solution at the
+//                                  front's contribution rows (rows of ancestors, final).
 // Two launches per level forward, one backward, and no sequential chain inside a front.
+// Inside a solve the right-hand sides live in pivot numbering with the p columns interleaved
+// (Xp[pos * p + c], the same for Y and the update rows U[(front row) * p + c]): a front's rows are
+// contiguous and a gathered row is one contiguous run of p values. The caller's column-major X is
+// permuted in at the start and out at the end (pos: pivot position of each original row).
 //
 // G is built from L once per factorization by batched 64 x 64 tile products: inverses of the
 // 64 x 64 diagonal blocks of L11, then recursive doubling over block sizes h = 64, 128, ...
@@ -33,6 +37,22 @@ constexpr int kGvTile = 64;       // rows (forward) / columns (backward) per GEM
 constexpr int kGvChunk = 128;     // vector entries staged in LDS per pass
 constexpr int kMaxRhsChunk = 8;   // right-hand sides per pass
 constexpr int kGatherRows = 256;  // rows per gather workgroup
+
+// Stages kn rows of right-hand-side values [c0, c0 + pc) into ws (zeros elsewhere): two threads
+// per row, four values each; row_ptr(t) gives the address of row t's value c0. All loads of a
+// thread are issued before any is used (no branch around them: invalid rows read row 0).
+static_assert(kGvChunk * 2 == 256 && kMaxRhsChunk == 8, "staging layout: 2 threads x 4 values per row");
+template <class T, class RowPtr>
+__device__ __forceinline__ void stage_rows(T (*ws)[kMaxRhsChunk], int kn, int pc, RowPtr row_ptr) {
+  const int t = threadIdx.x >> 1, h = (threadIdx.x & 1) * 4;
+  const bool valid = t < kn;
+  const T* src = row_ptr(valid ? t : 0);
+  T v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = src[min(h + c, pc - 1)];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) ws[t][h + c] = (valid && h + c < pc) ? v[c] : (T)0;
+}
 
 // L(i, j) of a pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair (that slot of
 // the panel holds D's off-diagonal).
@@ -176,72 +196,85 @@ __global__ void __launch_bounds__(256) k_transpose_g(DevSym S, const int32_t* __
 
 // ---------------------------------------------------------------------------------- forward
 
-// z of each front of the level: own rows take the right-hand side (converted to the factor type),
-// every row adds the update vectors of the children's contribution rows mapped onto it (cptr /
-// cidx: per front row, the U rows of the children, in child order). Pivot rows go to Z in pivot
-// order, contribution rows to the front's U rows. grid (row chunks, fronts).
+// Pivot position of every original row: pos[rows_orig[rowoff + r]] = first + ipiv[first + r] for
+// the pivot rows r < ncol of each front (one workgroup per front, all fronts).
+template <class T>
+__global__ void __launch_bounds__(256) k_pivot_positions(DevSym S, DevNum<T> N, int32_t* __restrict__ pos) {
+  const int s = blockIdx.x;
+  const int ncol = S.sn_ncol[s], first = S.sn_first[s];
+  const int64_t roff = S.sn_rowoff[s];
+  for (int r = threadIdx.x; r < ncol; r += blockDim.x) pos[S.rows_orig[roff + r]] = first + N.ipiv[first + r];
+}
+
+// Pivot position of every front row (index rowoff + i): rpos = pos[rows_orig].
+static __global__ void k_row_positions(int64_t rows_total, const int32_t* __restrict__ rows_orig,
+                                const int32_t* __restrict__ pos, int32_t* __restrict__ rpos) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < rows_total; k += (int64_t)gridDim.x * blockDim.x)
+    rpos[k] = pos[rows_orig[k]];
+}
+
+// Xp[pos[i] * p + c] = X[i + c * ldx] (into pivot numbering, columns interleaved).
+template <class R, class T>
+__global__ void k_permute_in(int n, int p, const R* __restrict__ X, int64_t ldx, const int32_t* __restrict__ pos,
+                             T* __restrict__ Xp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T* dst = Xp + (int64_t)pos[i] * p;
+    for (int c = 0; c < p; ++c) dst[c] = (T)X[i + c * ldx];
+  }
+}
+
+// X[i + c * ldx] = Xp[pos[i] * p + c] (back to the caller's numbering and layout).
 template <class T, class R>
-__global__ void __launch_bounds__(kGatherRows) k_fwd_gather(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                             const R* __restrict__ X, int64_t ldx, int p, T* Z, T* U,
-                                                             int64_t ldu, const int64_t* __restrict__ cptr,
+__global__ void k_permute_out(int n, int p, const T* __restrict__ Xp, const int32_t* __restrict__ pos, R* __restrict__ X,
+                              int64_t ldx) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T* src = Xp + (int64_t)pos[i] * p;
+    for (int c = 0; c < p; ++c) X[i + c * ldx] = (R)src[c];
+  }
+}
+
+// z of each front of the level: a pivot row adds, to its right-hand side in Xp (in place), the
+// update rows of the children mapped onto it; a contribution row's U row becomes the sum of
+// those (cptr / cidx: per front row, the children's U rows, in child order). grid (row chunks,
+// fronts).
+template <class T>
+__global__ void __launch_bounds__(kGatherRows) k_fwd_gather(DevSym S, const int32_t* __restrict__ list, int p, T* Xp,
+                                                             T* U, const int32_t* __restrict__ rpos,
+                                                             const int64_t* __restrict__ cptr,
                                                              const int32_t* __restrict__ cidx) {
   const int s = list[blockIdx.y];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
+  const int m = S.sn_m[s], ncol = S.sn_ncol[s];
   const int loc = blockIdx.x * kGatherRows + threadIdx.x;
   if (loc >= m) return;
   const int64_t roff = S.sn_rowoff[s];
   const int64_t q0 = cptr[roff + loc], q1 = cptr[roff + loc + 1];
   const bool own = loc < ncol;
-  const int64_t xrow = own ? S.rows_orig[roff + loc] : 0;
-  const int64_t dst = own ? (int64_t)first + N.ipiv[first + loc] : roff + loc;
-  T* out = own ? Z : U;
-  const int64_t ldo = own ? (int64_t)S.n : ldu;
-  for (int c = 0; c < p; ++c) {
-    T v = own ? (T)X[xrow + c * ldx] : (T)0;
-    for (int64_t q = q0; q < q1; ++q) v += U[cidx[q] + c * ldu];
-    out[dst + c * ldo] = v;
+  T* out = own ? Xp + (int64_t)rpos[roff + loc] * p : U + (roff + loc) * p;
+  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
+    const int pc = min(kMaxRhsChunk, p - c0);
+    T v[kMaxRhsChunk];
+#pragma unroll
+    for (int c = 0; c < kMaxRhsChunk; ++c) v[c] = (own && c < pc) ? out[c0 + c] : (T)0;
+    for (int64_t q = q0; q < q1; ++q) {
+      const T* u = U + (int64_t)cidx[q] * p + c0;
+#pragma unroll
+      for (int c = 0; c < kMaxRhsChunk; ++c)
+        if (c < pc) v[c] += u[c];
+    }
+#pragma unroll
+    for (int c = 0; c < kMaxRhsChunk; ++c)
+      if (c < pc) out[c0 + c] = v[c];
   }
 }
 
 // Split products: a tile's long dimension (columns of G forward, rows of Gt backward) is cut into
-// chunks of kGvSplit, one workgroup each. With more than one chunk, each workgroup stores its
-// partial sums (64 x p) in its slot of `part`, and the last workgroup of the tile to finish
-// (device-scope fence, then a counter) adds the partials in chunk order and writes the result,
-// so the sum does not depend on the order the workgroups run in. The counter is reset by that
-// workgroup for the next use. Counters: ((front index in level) * tiles + tile) * rhs_chunks +
-// rhs_chunk (rhs_chunk = which group of kMaxRhsChunk right-hand sides); slots: counter * chunks +
-// chunk.
+// chunks, one workgroup each. With more than one chunk, each workgroup stores its partial sums
+// (64 x 8 per group of right-hand sides) in its slot of `part`, and a second launch adds the
+// chunks' partials in chunk order and writes the result: the sum does not depend on the order
+// the workgroups ran in, and stream order makes the partials visible (no cross-workgroup
+// synchronization inside a launch). Slot index: (slot of chunk) * rhs_groups + rhs_group.
 constexpr int kGvSplit = 256;
-
-// Returns true in the workgroup that holds the complete sums of the tile in red[.][lane][c]
-// (red = this workgroup's own sums when the tile has one chunk).
-template <class T>
-__device__ __forceinline__ bool split_combine(T (*red)[kGvTile][kMaxRhsChunk + 1], int nchunk, int chunk,
-                                              int64_t slot0, int pc, T* __restrict__ part, int* __restrict__ counter,
-                                              int cidx) {
-  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
-  // red[0..3] hold the 4 group sums; fold them into red[0]
-  for (int c = g; c < pc; c += 4) red[0][lane][c] = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
-  __syncthreads();
-  if (nchunk == 1) return true;
-  T* mine = part + (slot0 + chunk) * (int64_t)(kGvTile * kMaxRhsChunk);
-  for (int c = g; c < pc; c += 4) mine[c * kGvTile + lane] = red[0][lane][c];
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0) s_last = atomicAdd(&counter[cidx], 1) == nchunk - 1;
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  for (int c = g; c < pc; c += 4) {
-    T v = (T)0;
-    for (int q = 0; q < nchunk; ++q) v += part[(slot0 + q) * (int64_t)(kGvTile * kMaxRhsChunk) + c * kGvTile + lane];
-    red[0][lane][c] = v;
-  }
-  if (threadIdx.x == 0) counter[cidx] = 0;
-  __syncthreads();
-  return true;
-}
+constexpr int kPartSlot = kGvTile * kMaxRhsChunk;  // values per partial slot
 
 // [y1; u] = G z1 + [0; u] for a tile of 64 front rows and one chunk of G's columns: rows < ncol
 // (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution rows
@@ -249,8 +282,8 @@ __device__ __forceinline__ bool split_combine(T (*red)[kGvTile][kMaxRhsChunk + 1
 // LDS, G read along columns. grid (row tiles, fronts, column chunks).
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict__ G, const int32_t* __restrict__ list,
-                                                   int p, const T* __restrict__ Z, T* __restrict__ Y, T* __restrict__ U,
-                                                   int64_t ldu, T* __restrict__ part, int* __restrict__ counter) {
+                                                   int p, const T* __restrict__ Xp, T* __restrict__ Y, T* __restrict__ U,
+                                                   T* __restrict__ part) {
   __shared__ T zs[kGvChunk][kMaxRhsChunk];
   __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
   const int f = blockIdx.y;
@@ -263,27 +296,23 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
   const int nchunk = (kend + kGvSplit - 1) / kGvSplit;
   if (chunk >= nchunk) return;
   const int kbeg = chunk * kGvSplit, kstop = min(kend, kbeg + kGvSplit);
-  const int n = S.n;
   const T* Gs = G + S.panel_off[s];
   const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
   const int i = r0 + lane;
   const bool rin = i < m;
-  const T* zf = Z + first;
+  const T* zf = Xp + (int64_t)first * p;
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
     const int pc = min(kMaxRhsChunk, p - c0);
-    const int cidx = (f * (int)gridDim.x + tile) * nrc + c0 / kMaxRhsChunk;
-    const int64_t slot0 = (int64_t)cidx * gridDim.z;
+    // slot of this (front, tile, chunk)
+    const int64_t slot = (((int64_t)f * gridDim.x + tile) * gridDim.z + chunk) * nrc + c0 / kMaxRhsChunk;
     T acc[kMaxRhsChunk];
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
     for (int k0 = kbeg; k0 < kstop; k0 += kGvChunk) {
       const int kn = min(kGvChunk, kstop - k0);
       __syncthreads();
-      for (int idx = threadIdx.x; idx < kGvChunk * kMaxRhsChunk; idx += 256) {
-        const int t = idx % kGvChunk, c = idx / kGvChunk;
-        zs[t][c] = (t < kn && c < pc) ? zf[k0 + t + (int64_t)(c0 + c) * n] : (T)0;
-      }
+      stage_rows<T>(zs, kn, pc, [&](int t) { return zf + (int64_t)(k0 + t) * p + c0; });
       __syncthreads();
       if (rin) {
         const T* gcol = Gs + (int64_t)k0 * m + i;
@@ -299,112 +328,162 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) red[g][lane][c] = acc[c];
     __syncthreads();
-    if (split_combine<T>(red, nchunk, chunk, slot0, pc, part, counter, cidx) && rin) {
-      for (int c = g; c < pc; c += 4) {
-        const T v = red[0][lane][c];
-        if (i < ncol) Y[first + i + (int64_t)(c0 + c) * n] = v;
-        else U[S.sn_rowoff[s] + i + (int64_t)(c0 + c) * ldu] += v;
+    for (int c = g; c < pc; c += 4) {
+      const T v = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
+      if (nchunk > 1) {
+        part[slot * kPartSlot + c * kGvTile + lane] = v;
+      } else if (rin) {
+        if (i < ncol) Y[(int64_t)(first + i) * p + c0 + c] = v;
+        else U[(S.sn_rowoff[s] + i) * p + c0 + c] += v;
       }
     }
     __syncthreads();
   }
 }
 
+// Second launch of a split forward level: for every (front, row tile) with more than one column
+// chunk, the chunks' partials added in chunk order, written like k_fwd_gemv's result.
+// grid (row tiles, fronts), chunks = the forward launch's grid.z.
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __restrict__ list, int p, int chunks,
+                                                     T* __restrict__ Y, T* __restrict__ U, const T* __restrict__ part) {
+  const int f = blockIdx.y;
+  const int s = list[f];
+  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
+  const int tile = blockIdx.x;
+  const int r0 = tile * kGvTile;
+  if (r0 >= m) return;
+  const int kend = min(ncol, r0 + kGvTile);
+  const int nchunk = (kend + kGvSplit - 1) / kGvSplit;
+  if (nchunk <= 1) return;
+  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
+  for (int idx = threadIdx.x; idx < kGvTile * p; idx += blockDim.x) {
+    const int lane = idx % kGvTile, c = idx / kGvTile;
+    const int i = r0 + lane;
+    if (i >= m) continue;
+    const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
+    T v = (T)0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const int64_t slot = (((int64_t)f * gridDim.x + tile) * chunks + ch) * nrc + q;
+      v += part[slot * kPartSlot + cc * kGvTile + lane];
+    }
+    if (i < ncol) Y[(int64_t)(first + i) * p + c] = v;
+    else U[(S.sn_rowoff[s] + i) * p + c] += v;
+  }
+}
+
 // ---------------------------------------------------------------------------------- diagonal
 
+// Y (pivot numbering, p columns interleaved) = D^-1 Y.
 template <class T>
 __global__ void k_solve_diag(int n, const T* __restrict__ dval, const T* __restrict__ doff,
-                             const int8_t* __restrict__ ptype, T* Z, int p) {
+                             const int8_t* __restrict__ ptype, T* Y, int p) {
   const int64_t total = (int64_t)n * p;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % n);
-    const int64_t c = idx / n;
+    const int i = (int)(idx / p);
     const int8_t t = ptype[i];
-    T* z = Z + c * n;
+    T* z = Y + idx;
     if (t == kPiv1) {
-      z[i] = z[i] / dval[i];
+      z[0] = z[0] / dval[i];
     } else if (t == kPiv2First) {
       const T a = dval[i], bb = doff[i], d = dval[i + 1];
       const T det = a * d - bb * bb;
-      const T z1 = z[i], z2 = z[i + 1];
-      z[i] = (d * z1 - bb * z2) / det;
-      z[i + 1] = (a * z2 - bb * z1) / det;
+      const T z1 = z[0], z2 = z[p];
+      z[0] = (d * z1 - bb * z2) / det;
+      z[p] = (a * z2 - bb * z1) / det;
     } else if (t == kPivNull) {
-      z[i] = (T)0;
+      z[0] = (T)0;
     }
   }
 }
 
 // ---------------------------------------------------------------------------------- backward
 
-// x1 = G^T [y1; x_cb] for a tile of 64 pivot columns of a front and one chunk of its rows
-// i >= the tile's first column (G11 lower triangular): y1 from Y (pivot order), x_cb from X at the
-// front's contribution rows (rows of ancestors, final already). The result is scattered into X at
-// the pivots' rows. 256 threads = 64 columns x 4 row groups; Gt read along rows.
-// grid (column tiles, fronts, row chunks).
-template <class T, class R>
-__global__ void __launch_bounds__(256) k_bwd_gemv(DevSym S, DevNum<T> N, const T* __restrict__ Gt,
-                                                   const int32_t* __restrict__ list, int p, const T* __restrict__ Y,
-                                                   R* X, int64_t ldx, T* __restrict__ part, int* __restrict__ counter) {
+// One task of the backward sweep: columns [j0, j0 + jn) of front s (jw = the power of two >= jn
+// that the 256 threads are laid out by: jw columns x 256 / jw row groups) and rows [i0, i0 + in)
+// of its Gt, chunk `chunk` of the nchunk row chunks of that column tile (slot0: the slot of its
+// chunk 0 when nchunk > 1).
+struct BwdTask {
+  int32_t s, j0, jn, jw, i0, in, chunk, nchunk;
+  int64_t slot0;
+};
+
+// x1 = G^T [y1; x_cb] over one task: y1 from Y (pivot rows), x_cb from Xp at the pivot positions
+// of the front's contribution rows (rows of ancestors, final already); the result goes to Xp at
+// the front's own pivot positions, or, for a split tile, to the task's partial slot. Gt is read
+// along rows (jw consecutive columns per row group).
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_task(DevSym S, const T* __restrict__ Gt, const BwdTask* __restrict__ tasks,
+                                                   int p, const T* __restrict__ Y, T* Xp, const int32_t* __restrict__ rpos,
+                                                   T* __restrict__ part) {
   __shared__ T ws[kGvChunk][kMaxRhsChunk];
-  __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
-  const int f = blockIdx.y;
-  const int s = list[f];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int tile = blockIdx.x, chunk = blockIdx.z;
-  const int j0 = tile * kGvTile;
-  if (j0 >= ncol) return;
-  const int nchunk = (m - j0 + kGvSplit - 1) / kGvSplit;
-  if (chunk >= nchunk) return;
-  const int ibeg = j0 + chunk * kGvSplit, istop = min(m, ibeg + kGvSplit);
-  const int n = S.n;
-  const int64_t roff = S.sn_rowoff[s];
-  const int32_t* rows = S.rows_orig + roff;
+  __shared__ T red[256][kMaxRhsChunk + 1];
+  const BwdTask tk = tasks[blockIdx.x];
+  const int s = tk.s;
+  const int ncol = S.sn_ncol[s], first = S.sn_first[s];
+  const int JW = tk.jw, ngrp = 256 / JW;
+  const int tid = threadIdx.x, jj = tid % JW, grp = tid / JW;
+  const int j = tk.j0 + jj;
+  const bool jin = jj < tk.jn;
+  const int32_t* rp = rpos + S.sn_rowoff[s];
   const T* Gs = Gt + S.panel_off[s];
-  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
-  const int j = j0 + lane;
-  const bool jin = j < ncol;
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
+  const int iend = tk.i0 + tk.in;
   for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
     const int pc = min(kMaxRhsChunk, p - c0);
-    const int cidx = (f * (int)gridDim.x + tile) * nrc + c0 / kMaxRhsChunk;
-    const int64_t slot0 = (int64_t)cidx * gridDim.z;
     T acc[kMaxRhsChunk];
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
-    for (int i0 = ibeg; i0 < istop; i0 += kGvChunk) {
-      const int kn = min(kGvChunk, istop - i0);
+    for (int i0 = tk.i0; i0 < iend; i0 += kGvChunk) {
+      const int kn = min(kGvChunk, iend - i0);
       __syncthreads();
-      for (int idx = threadIdx.x; idx < kGvChunk * kMaxRhsChunk; idx += 256) {
-        const int t = idx % kGvChunk, c = idx / kGvChunk;
-        T v = (T)0;
-        if (t < kn && c < pc) {
-          const int i = i0 + t;
-          v = i < ncol ? Y[first + i + (int64_t)(c0 + c) * n] : (T)X[rows[i] + (int64_t)(c0 + c) * ldx];
-        }
-        ws[t][c] = v;
-      }
+      stage_rows<T>(ws, kn, pc, [&](int t) {
+        const int i = i0 + t;
+        const int64_t r = rp[i];  // every front row has a pivot position; used for contribution rows
+        return i < ncol ? Y + (int64_t)(first + i) * p + c0 : Xp + r * p + c0;
+      });
       __syncthreads();
       if (jin) {
         const T* grow = Gs + (int64_t)i0 * ncol + j;
-#pragma unroll 4
-        for (int t = g; t < kn; t += 4) {
+#pragma unroll 8
+        for (int t = grp; t < kn; t += ngrp) {
           const T gv = grow[(int64_t)t * ncol];
 #pragma unroll
           for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv * ws[t][c];
         }
       }
     }
-    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c) red[g][lane][c] = acc[c];
+    for (int c = 0; c < kMaxRhsChunk; ++c) red[tid][c] = acc[c];
     __syncthreads();
-    if (split_combine<T>(red, nchunk, chunk, slot0, pc, part, counter, cidx) && jin) {
-      const int64_t xrow = rows[N.piv[first + j]];
-      for (int c = g; c < pc; c += 4) X[xrow + (int64_t)(c0 + c) * ldx] = (R)red[0][lane][c];
+    T* slot = tk.nchunk > 1 ? part + ((tk.slot0 + tk.chunk) * nrc + c0 / kMaxRhsChunk) * (int64_t)kPartSlot : nullptr;
+    for (int idx = tid; idx < JW * pc; idx += 256) {
+      const int j2 = idx % JW, c = idx / JW;
+      if (j2 >= tk.jn) continue;
+      T v = (T)0;
+      for (int g2 = 0; g2 < ngrp; ++g2) v += red[g2 * JW + j2][c];
+      if (slot) slot[c * kGvTile + j2] = v;
+      else Xp[(int64_t)(first + tk.j0 + j2) * p + c0 + c] = v;
     }
     __syncthreads();
+  }
+}
+
+// Second launch of a split backward level: one workgroup per split column tile (the task of its
+// chunk 0, in `heads`), the chunks' partials added in chunk order and written to Xp.
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_reduce(DevSym S, const BwdTask* __restrict__ heads, int p, T* Xp,
+                                                     const T* __restrict__ part) {
+  const BwdTask tk = heads[blockIdx.x];
+  const int first = S.sn_first[tk.s];
+  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
+  for (int idx = threadIdx.x; idx < tk.jn * p; idx += blockDim.x) {
+    const int j2 = idx % tk.jn, c = idx / tk.jn;
+    const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
+    T v = (T)0;
+    for (int ch = 0; ch < tk.nchunk; ++ch) v += part[((tk.slot0 + ch) * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + j2];
+    Xp[(int64_t)(first + tk.j0 + j2) * p + c] = v;
   }
 }
 
