@@ -46,6 +46,7 @@ SIGNATURES = {
     "hsg_solve": (C.c_int, [_P, _P, _I64, _I64, _P]),
     "hsg_release_factors": (C.c_int, [_P]),
     "hsg_factor_destroy": (None, [_P]),
+    "hsg_factor_clone": (C.c_int, [_P, C.POINTER(_P)]),
     "hsg_csr_spmm": (C.c_int, [_I64, _P, _P, _P, _P, _I64, _P, _I64, _I64, _D, _D, _P]),
     "hsg_diag_spmm": (C.c_int, [_I64, _P, _P, _I64, _P, _I64, _I64, _D, _D, _P]),
     "hsg_shifted_residual": (C.c_int, [_I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _I64, _P, _I64, _P, _I64, _I64,

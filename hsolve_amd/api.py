@@ -181,18 +181,6 @@ def _matrix(m, name: str, storage: str):
     return _mirror(M, storage)
 
 
-def _small_lapack_threads():
-    """The host-side dense work of a solve is small (projected problems of order <= a few hundred,
-    p x p Gram matrices): multithreaded BLAS only adds synchronisation there, so a solve runs it on
-    one thread (restored afterwards)."""
-    try:
-        from threadpoolctl import threadpool_limits
-        return threadpool_limits(limits=1, user_api="blas")
-    except Exception:  # threadpoolctl missing: leave the BLAS as configured
-        import contextlib
-        return contextlib.nullcontext()
-
-
 def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = _Default(6),
           mode: str = _Default("nearest"), lower: float | None = None, upper: float | None = None,
           tol: float = _Default(1e-10), config=None, threads: int = _Default(0), verbose: int = 0,
@@ -211,7 +199,7 @@ def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = _Defaul
     Bm = _matrix(B, "B", storage)
     complex_dtype = np.iscomplexobj(Am.data) or (Bm is not None and np.iscomplexobj(Bm.data))
     from .solver import solve as engine_solve
-    with _SOLVE_LOCK, _small_lapack_threads():
+    with _SOLVE_LOCK:
         w, v, report, cfg_out = engine_solve(Am, Bm, cfg)
     report.pop("_events", None)
     if complex_dtype:

@@ -4,6 +4,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -16,7 +17,8 @@
 #include "kernels/krylov.hpp"
 
 struct hsg_factor {
-  std::unique_ptr<hsg::Engine> engine;
+  std::shared_ptr<hsg::Engine> engine;  // the analysis, shared by clones
+  hsg::Numeric num;                     // this handle's factorization
   hsg_analysis_info info{};
 };
 
@@ -53,12 +55,20 @@ int guarded(F&& f) {
   }
 }
 
-// One scratch arena for the reduction partials of the Krylov kernels, per process. Calls that
-// use it are serialized by its mutex; the kernels themselves are stream-ordered.
+// Scratch arenas for the reduction partials of the Krylov kernels, one per stream: kernels on
+// different streams may run at the same time, kernels on one stream are ordered. The map is
+// guarded by g_scratch_mu; a call holds its arena's mutex while it launches.
+struct StreamScratch {
+  std::mutex mu;
+  hsg::Scratch arena;
+};
 std::mutex g_scratch_mu;
-hsg::Scratch& scratch() {
-  static hsg::Scratch s;
-  return s;
+std::map<hipStream_t, std::unique_ptr<StreamScratch>> g_scratch;
+StreamScratch& scratch_for(hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  std::unique_ptr<StreamScratch>& p = g_scratch[st];
+  if (!p) p = std::make_unique<StreamScratch>();
+  return *p;
 }
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -113,7 +123,7 @@ int hsg_analyze(int64_t n, const int64_t* rowptr, const int32_t* colind, const i
     in.cb_bytes_double = 8.0 * static_cast<double>(sym.cb_off[sym.nsn]);
     in.analyze_s = std::chrono::duration<double>(t1 - t0).count();
     in.ordering_s = 0.0;
-    f->engine = std::make_unique<hsg::Engine>(std::move(sym), std::move(dst), std::move(ia), std::move(ib), eo);
+    f->engine = std::make_shared<hsg::Engine>(std::move(sym), std::move(dst), std::move(ia), std::move(ib), eo);
     in.front_rows = static_cast<int64_t>(f->engine->symbolic().rows.size());
     in.small_fronts = f->engine->small_fronts();
     in.large_fronts = f->engine->large_fronts();
@@ -158,7 +168,7 @@ int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_valu
     require(null_pivot_tol >= 0.0 && null_pivot_tol < 1.0, "hsg_factorize: null_pivot_tol must be in [0, 1)");
     require(a_values != nullptr, "hsg_factorize: a_values is NULL");
     const hsg::InertiaResult r =
-        f->engine->factorize(alpha, beta, a_values, b_values, precision, null_pivot_tol, as_stream(stream));
+        f->engine->factorize(f->num, alpha, beta, a_values, b_values, precision, null_pivot_tol, as_stream(stream));
     inertia->neg = r.neg;
     inertia->zero = r.zero;
     inertia->pos = r.pos;
@@ -173,19 +183,30 @@ int hsg_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream)
     require(f && x, "hsg_solve: NULL argument");
     require(nrhs >= 0, "hsg_solve: nrhs < 0");
     require(ldx >= f->info.n, "hsg_solve: ldx < n");
-    if (!f->engine->has_factor()) throw std::runtime_error("hsg_solve: no usable factorization");
-    f->engine->solve(x, nrhs, ldx, as_stream(stream));
+    if (!hsg::Engine::has_factor(f->num)) throw std::runtime_error("hsg_solve: no usable factorization");
+    f->engine->solve(f->num, x, nrhs, ldx, as_stream(stream));
   });
 }
 
 int hsg_release_factors(hsg_factor* f) {
   return guarded([&] {
     require(f != nullptr, "hsg_release_factors: NULL argument");
-    f->engine->release();
+    f->engine->release(f->num);
   });
 }
 
 void hsg_factor_destroy(hsg_factor* f) { delete f; }
+
+int hsg_factor_clone(const hsg_factor* src, hsg_factor** out) {
+  return guarded([&] {
+    require(src != nullptr && out != nullptr, "hsg_factor_clone: NULL argument");
+    *out = nullptr;
+    auto f = std::make_unique<hsg_factor>();
+    f->engine = src->engine;
+    f->info = src->info;
+    *out = f.release();
+  });
+}
 
 int hsg_csr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values, const double* x,
                  int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta, void* stream) {
@@ -210,8 +231,9 @@ int hsg_shifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_co
                          void* stream) {
   return guarded([&] {
     require(ldx >= n && ldy >= n && ldr >= n && p >= 0, "hsg_shifted_residual: invalid sizes");
-    std::lock_guard<std::mutex> lock(g_scratch_mu);
-    hsg::shifted_residual(scratch(), n, a_rowptr, a_colind, a_values, b_rowptr, b_colind, b_values, b_diag, sigma, x,
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::shifted_residual(sc.arena, n, a_rowptr, a_colind, a_values, b_rowptr, b_colind, b_values, b_diag, sigma, x,
                           ldx, y, ldy, r, ldr, p, rnorm2, omega, as_stream(stream));
   });
 }
@@ -220,8 +242,9 @@ int hsg_gram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w
              int64_t p, double* c, int64_t ldc, void* stream) {
   return guarded([&] {
     require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_gram: invalid sizes");
-    std::lock_guard<std::mutex> lock(g_scratch_mu);
-    hsg::gram(scratch(), n, v, ldv, k, w_diag, w, ldw, p, c, ldc, as_stream(stream));
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::gram(sc.arena, n, v, ldv, k, w_diag, w, ldw, p, c, ldc, as_stream(stream));
   });
 }
 
@@ -237,8 +260,9 @@ int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, in
                     const double* w_diag, double* out, void* stream) {
   return guarded([&] {
     require(ldx >= n && ldy >= n && p >= 0, "hsg_column_dots: invalid sizes");
-    std::lock_guard<std::mutex> lock(g_scratch_mu);
-    hsg::column_dots(scratch(), n, x, ldx, y, ldy, p, w_diag, out, as_stream(stream));
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::column_dots(sc.arena, n, x, ldx, y, ldy, p, w_diag, out, as_stream(stream));
   });
 }
 
@@ -246,8 +270,9 @@ int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x
                   const double* lambda, double* out, void* stream) {
   return guarded([&] {
     require(ld >= n && p >= 0, "hsg_pair_sums: invalid sizes");
-    std::lock_guard<std::mutex> lock(g_scratch_mu);
-    hsg::pair_sums(scratch(), n, ax, bx, x, ld, p, lambda, out, as_stream(stream));
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::pair_sums(sc.arena, n, ax, bx, x, ld, p, lambda, out, as_stream(stream));
   });
 }
 

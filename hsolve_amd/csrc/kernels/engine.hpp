@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,30 @@ struct InertiaResult {
   bool singular = false;
 };
 
+// The numeric state of one factorization over an analysis (one Engine may serve several, e.g. the
+// spectrum slices solved at the same time on different streams): factor panels, contribution
+// blocks, D and pivots, inertia counters, the solve phase's inverse panels and work arrays.
+struct Numeric {
+  Numeric() = default;
+  ~Numeric();
+  Numeric(const Numeric&) = delete;
+  Numeric& operator=(const Numeric&) = delete;
+  void free_all();
+  int precision_ = -1;
+  bool valid_ = false;
+  void *d_L_ = nullptr, *d_CB_ = nullptr, *d_W_ = nullptr, *d_dval_ = nullptr, *d_doff_ = nullptr;
+  int8_t* d_ptype_ = nullptr;
+  int32_t *d_piv_ = nullptr, *d_ipiv_ = nullptr, *d_bperm_ = nullptr;
+  unsigned long long* d_cnt_ = nullptr;
+  void *d_Xp_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;  // solve work arrays (pivot numbering, interleaved)
+  int64_t ws_cols_ = 0;
+  void *d_G_ = nullptr, *d_Gt_ = nullptr, *d_T_ = nullptr;  // inverse panels (solve phase)
+  int32_t *d_pos_ = nullptr, *d_rpos_ = nullptr;  // pivot position of each original row / front row
+  void* d_part_ = nullptr;       // partial sums of split product tiles
+  void* d_pinv_ = nullptr;  // panel inverses of the large fronts of one level (panel solve)
+  bool g_ready_ = false;
+};
+
 class Engine {
  public:
   Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> ia, std::vector<int64_t> ib,
@@ -37,14 +62,13 @@ class Engine {
   int64_t small_fronts() const { return n_small_; }
   int64_t large_fronts() const { return n_large_; }
 
-  // Numeric factorization; returns the inertia counts (synchronizes the stream).
-  InertiaResult factorize(double alpha, double beta, const double* a_values, const double* b_values, int precision,
-                          double null_tol, hipStream_t stream);
-  // In-place solve of nrhs FP64 columns with the last factorization.
-  void solve(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
-  void release();
-  bool has_factor() const { return valid_; }
-  int precision() const { return precision_; }
+  // Numeric factorization into nm; returns the inertia counts (synchronizes the stream).
+  InertiaResult factorize(Numeric& nm, double alpha, double beta, const double* a_values, const double* b_values,
+                          int precision, double null_tol, hipStream_t stream);
+  // In-place solve of nrhs FP64 columns with the last factorization held in nm.
+  void solve(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
+  void release(Numeric& nm);
+  static bool has_factor(const Numeric& nm) { return nm.valid_; }
 
  private:
   struct Level {
@@ -68,17 +92,16 @@ class Engine {
     int64_t begin = 0, count = 0;
   };
   template <class T>
-  InertiaResult factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
-                            hipStream_t stream);
+  InertiaResult factorize_t(Numeric& nm, double alpha, double beta, const double* av, const double* bv,
+                            double null_tol, hipStream_t stream);
   template <class T>
-  void solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
+  void solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
   template <class T>
-  void build_inverse_panels(hipStream_t stream);  // G and Gt from L (first solve after a factorization)
+  void build_inverse_panels(Numeric& nm, hipStream_t stream);  // G, Gt from L (first solve after a factorization)
   void plan_solve();                              // host task lists of the solve phase (symbolic)
   void validate_solve_plan() const;
-  void alloc_numeric(int precision);
+  void alloc_numeric(Numeric& nm, int precision);
   void ensure_device();  // uploads the analysis on first use (the analysis itself is host-only)
-  void free_numeric();
 
   Symbolic sym_;
   EngineOptions opt_;
@@ -98,7 +121,6 @@ class Engine {
   int32_t *d_pairs_ = nullptr, *d_ttiles_ = nullptr, *d_cidx_ = nullptr;
   int64_t* d_cptr_ = nullptr;
   GemmTask* d_tasks_ = nullptr;
-  void* d_pinv_ = nullptr;  // panel inverses of the large fronts of one level (panel solve)
   bool on_device_ = false;
   int64_t nnz_ = 0;
   int64_t n_small_ = 0, n_large_ = 0;
@@ -110,20 +132,8 @@ class Engine {
   int64_t *d_rowoff_ = nullptr, *d_panel_off_ = nullptr, *d_cb_off_ = nullptr, *d_w_off_ = nullptr;
   int64_t *d_asm_ = nullptr, *d_ia_ = nullptr, *d_ib_ = nullptr;
   std::vector<Level> levels_;
-  // numeric
-  int precision_ = -1;
-  bool valid_ = false;
-  void *d_L_ = nullptr, *d_CB_ = nullptr, *d_W_ = nullptr, *d_dval_ = nullptr, *d_doff_ = nullptr;
-  int8_t* d_ptype_ = nullptr;
-  int32_t *d_piv_ = nullptr, *d_ipiv_ = nullptr, *d_bperm_ = nullptr;
-  unsigned long long* d_cnt_ = nullptr;
-  void *d_Xp_ = nullptr, *d_Y_ = nullptr, *d_U_ = nullptr;  // solve work arrays (pivot numbering, interleaved)
-  int64_t ws_cols_ = 0;
-  void *d_G_ = nullptr, *d_Gt_ = nullptr, *d_T_ = nullptr;  // inverse panels (solve phase)
-  int32_t *d_pos_ = nullptr, *d_rpos_ = nullptr;  // pivot position of each original row / front row
-  void* d_part_ = nullptr;       // partial sums of split product tiles
   int64_t part_slots_ = 0;       // slots per group of kMaxRhsChunk right-hand sides
-  bool g_ready_ = false;
+  mutable std::mutex device_mu_;  // ensure_device from factorizations running on several threads
 };
 
 // Throws std::runtime_error with the HIP error string on failure.

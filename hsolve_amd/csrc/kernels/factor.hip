@@ -468,22 +468,28 @@ __device__ __forceinline__ double wave_max_abs(double v) {
   return __longlong_as_double((long long)wave_max_key((unsigned long long)__double_as_longlong(fabs(v))));
 }
 
-// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, one 64-lane wave per
-// front: lane i owns row i of the block (LDS, padded leading dimension), pivot searches are DPP
-// wave reductions (argmax: lowest lane holding the maximum, by ballot), the multipliers of a
-// rank-1 / rank-2 update are read from arrays of their own. Also forms the inverse of the block's
-// unit lower L11 (per-front scratch, read by the panel solve).
+// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, four 64-lane waves per
+// front: lane i of every wave stands for row i of the block (LDS, padded leading dimension).
+// Every wave runs the same pivot search on the same data (DPP wave reductions; argmax = lowest
+// lane holding the maximum, by ballot), so all reach the same decision without a barrier; wave w
+// then updates the columns j = w (mod 4) of the trailing triangle, with the step's multipliers
+// formed from the pivot column (still unscaled) in registers. Wave 0 alone swaps rows and
+// columns, writes the multipliers into the pivot column after the update, keeps the counts and
+// the pivot data. One barrier per step (four when rows are swapped). Also forms the inverse of
+// the block's unit lower L11 (per-front scratch, read by the panel solve), split over the waves
+// the same way.
 constexpr int kBkLd = 65;
+constexpr int kBkWaves = 4;
+constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
 
 template <class T>
-__global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list, int p,
-                                                int nb, int32_t* __restrict__ bperm, double null_tol, int detect,
-                                                T* __restrict__ Linv) {
+__global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
+                                                           int p, int nb, int32_t* __restrict__ bperm, double null_tol,
+                                                           int detect, T* __restrict__ Linv) {
   __shared__ T F[64 * kBkLd];
+  __shared__ T Xi[64 * kBkLd];
   __shared__ int perm[64];
-  // multipliers of the current 1x1 / 2x2 step in arrays of their own: the update loop's loads of
-  // them can then be scheduled ahead of its stores to F
-  __shared__ T s_l1[64], s_l2[64];
+  __shared__ int8_t ptl[64];
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -491,36 +497,39 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
   if (c0 >= ncol) return;
   const int w = min(nb, ncol - c0);
   T* P = N.L + S.panel_off[s];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const bool lead = wv == 0;
   {
     // branch-free loads (row index clamped into the block) so that they are issued together
     const T* src = P + (int64_t)c0 * m + c0 + min(lane, w - 1);
-#pragma unroll 16
-    for (int j = 0; j < w; ++j) {
+#pragma unroll 4
+    for (int j = wv; j < w; j += kBkWaves) {
       const T v = src[(int64_t)j * m];
       F[lane + j * kBkLd] = (lane >= j && lane < w) ? v : (T)0;
     }
   }
-  perm[lane] = lane;
+  if (lead) perm[lane] = lane;
   __syncthreads();
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   const bool det = detect != 0;
-  int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // lane 0's tallies
+  int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // tallies (wave 0, lane 0)
   int8_t my_pt = kPiv1;
   T my_dv = (T)0, my_dof = (T)0;
   int k = 0;
   while (k < w) {
-    // ---- pivot search (column k, candidates k..w-1)
+    // ---- pivot search (column k, candidates k..w-1), the same in every wave
+    // LDS reads below are unconditional (every index lies inside F) so that a step's reads are
+    // issued together; conditions only select values and mask stores
     const double absakk = fabs((double)F[k + k * kBkLd]);
-    const double v = (lane > k && lane < w) ? fabs((double)F[lane + k * kBkLd]) : 0.0;
+    const T fk = F[lane + k * kBkLd];
+    const double v = (lane > k && lane < w) ? fabs((double)fk) : 0.0;
     const double colmax = wave_max_abs(v);
     const unsigned long long at_max = __ballot(lane > k && lane < w && v == colmax);
     const int imax = (colmax > 0.0 && at_max) ? __ffsll((long long)at_max) - 1 : -1;
     int kind = 1, kp = k;
     if (!(absakk >= kBkAlpha * colmax) && imax > k) {
-      double r = 0.0;
-      if (lane >= k && lane < w && lane != imax)
-        r = lane < imax ? fabs((double)F[imax + lane * kBkLd]) : fabs((double)F[lane + imax * kBkLd]);
+      const T ra = F[imax + lane * kBkLd], rb = F[lane + imax * kBkLd];
+      const double r = (lane >= k && lane < w && lane != imax) ? fabs((double)(lane < imax ? ra : rb)) : 0.0;
       const double rowmax = wave_max_abs(r);
       if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
         kind = 1;
@@ -536,21 +545,24 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
     const int kk = kind == 1 ? k : k + 1;
     if (kp != kk) {
       const int pp = kk, q = kp;
-      if (lane < pp) {  // rows p, q of the columns before p
-        const T a = F[pp + lane * kBkLd];
-        F[pp + lane * kBkLd] = F[q + lane * kBkLd];
-        F[q + lane * kBkLd] = a;
-      } else if (lane > pp && lane < q) {  // A(j, p) <-> A(q, j)
-        const T a = F[lane + pp * kBkLd];
-        F[lane + pp * kBkLd] = F[q + lane * kBkLd];
-        F[q + lane * kBkLd] = a;
-      } else if (lane > q && lane < w) {  // A(i, p) <-> A(i, q)
-        const T a = F[lane + pp * kBkLd];
-        F[lane + pp * kBkLd] = F[lane + q * kBkLd];
-        F[lane + q * kBkLd] = a;
+      __syncthreads();  // every wave has finished reading the block for its search
+      if (lead) {
+        if (lane < pp) {  // rows p, q of the columns before p
+          const T a = F[pp + lane * kBkLd];
+          F[pp + lane * kBkLd] = F[q + lane * kBkLd];
+          F[q + lane * kBkLd] = a;
+        } else if (lane > pp && lane < q) {  // A(j, p) <-> A(q, j)
+          const T a = F[lane + pp * kBkLd];
+          F[lane + pp * kBkLd] = F[q + lane * kBkLd];
+          F[q + lane * kBkLd] = a;
+        } else if (lane > q && lane < w) {  // A(i, p) <-> A(i, q)
+          const T a = F[lane + pp * kBkLd];
+          F[lane + pp * kBkLd] = F[lane + q * kBkLd];
+          F[lane + q * kBkLd] = a;
+        }
       }
       __syncthreads();
-      if (lane == 0) {
+      if (lead && lane == 0) {
         const T a = F[pp + pp * kBkLd];
         F[pp + pp * kBkLd] = F[q + q * kBkLd];
         F[q + q * kBkLd] = a;
@@ -579,11 +591,11 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
         double mb = 0.0;
         const int jc = perm[k];
         for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs((double)P[(int64_t)(c0 + jc) * m + r2]));
-        for (int off = 32; off > 0; off >>= 1) mb = fmax(mb, __shfl_xor(mb, off));
+        mb = wave_max_abs(mb);
         if (mb <= tau) null_piv = true;
       }
       const T inv = null_piv ? (T)0 : (T)(1.0 / d);
-      if (lane == 0) {
+      if (lead && lane == 0) {
         if (d == 0.0) ++exz;
         if (null_piv) ++zero;
         else if (d < 0) ++neg;
@@ -595,18 +607,29 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
         my_dof = (T)0;
       }
       const bool below = lane > k && lane < w;
-      const T wi = below ? F[lane + k * kBkLd] : (T)0;
-      const T li = wi * inv;
-      if (below) F[lane + k * kBkLd] = li;
-      s_l1[lane] = below ? li : (T)0;
-      __syncthreads();
-      // rank-1 update of the lower triangle: row `lane`, columns k+1..lane
-      if (inv != (T)0 && below) {
-        T* row = F + lane;
-#pragma unroll 8
-        for (int j = k + 1; j <= lane; ++j) row[j * kBkLd] -= wi * s_l1[j];
+      const T fk2 = F[lane + k * kBkLd];  // after a swap column k has changed
+      const T wi = below ? fk2 : (T)0;
+      // this wave's multipliers l_j = F(j, k) / d, j = wv (mod 4)
+      T lv[kBkCols];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) {
+        const int j = wv + kBkWaves * q;
+        const T fj = F[j + k * kBkLd];
+        lv[q] = (j > k && j < w) ? fj * inv : (T)0;
+      }
+      // rank-1 update of the lower triangle: row `lane`, this wave's columns k+1..lane
+      const bool upd = inv != (T)0 && below;
+      T* row = F + lane;
+      T old[kBkCols];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) {
+        const int j = wv + kBkWaves * q;
+        if (upd && j > k && j <= lane) row[j * kBkLd] = old[q] - wi * lv[q];
       }
       __syncthreads();
+      if (lead && below) F[lane + k * kBkLd] = wi * inv;
       k += 1;
     } else {
       const double a = (double)F[k + k * kBkLd], b = (double)F[k + 1 + k * kBkLd], c = (double)F[k + 1 + (k + 1) * kBkLd];
@@ -623,7 +646,7 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
       }
       const bool dead = nz > 0 || dt == 0.0;
       const T i11 = dead ? (T)0 : (T)(c / dt), i12 = dead ? (T)0 : (T)(-b / dt), i22 = dead ? (T)0 : (T)(a / dt);
-      if (lane == 0) {
+      if (lead && lane == 0) {
         if (dt == 0.0) ++exz;
         zero += nz;
         neg += nn;
@@ -640,62 +663,79 @@ __global__ void __launch_bounds__(64) k_bk_diag(DevSym S, DevNum<T> N, const int
         my_dv = (T)c;
         my_dof = (T)0;
       }
-      const T w1 = (lane > k + 1 && lane < w) ? F[lane + k * kBkLd] : (T)0;
-      const T w2 = (lane > k + 1 && lane < w) ? F[lane + (k + 1) * kBkLd] : (T)0;
-      const T l1 = w1 * i11 + w2 * i12, l2 = w1 * i12 + w2 * i22;
       const bool below = lane > k + 1 && lane < w;
-      if (below) {
-        F[lane + k * kBkLd] = l1;
-        F[lane + (k + 1) * kBkLd] = l2;
+      const T c1 = F[lane + k * kBkLd], c2 = F[lane + (k + 1) * kBkLd];
+      const T w1 = below ? c1 : (T)0;
+      const T w2 = below ? c2 : (T)0;
+      // this wave's multipliers of the 2x2 step, j = wv (mod 4)
+      T l1v[kBkCols], l2v[kBkCols];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) {
+        const int j = wv + kBkWaves * q;
+        const bool in = j > k + 1 && j < w;
+        const T f1 = F[j + k * kBkLd], f2 = F[j + (k + 1) * kBkLd];
+        const T a1 = in ? f1 : (T)0, a2 = in ? f2 : (T)0;
+        l1v[q] = a1 * i11 + a2 * i12;
+        l2v[q] = a1 * i12 + a2 * i22;
       }
-      s_l1[lane] = below ? l1 : (T)0;
-      s_l2[lane] = below ? l2 : (T)0;
-      __syncthreads();
-      // rank-2 update of the lower triangle: row `lane`, columns k+2..lane
-      if (!dead && below) {
-        T* row = F + lane;
-#pragma unroll 8
-        for (int j = k + 2; j <= lane; ++j) row[j * kBkLd] -= w1 * s_l1[j] + w2 * s_l2[j];
+      const bool upd = !dead && below;
+      T* row = F + lane;
+      T old[kBkCols];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
+#pragma unroll
+      for (int q = 0; q < kBkCols; ++q) {
+        const int j = wv + kBkWaves * q;
+        if (upd && j > k + 1 && j <= lane) row[j * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
       }
       __syncthreads();
+      if (lead && below) {
+        F[lane + k * kBkLd] = w1 * i11 + w2 * i12;
+        F[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
+      }
       k += 2;
     }
   }
+  __syncthreads();
   // ---- write back L11 and D, pivot data
-  for (int j = 0; j < w; ++j)
+  for (int j = wv; j < w; j += kBkWaves)
     if (lane >= j && lane < w) P[(int64_t)(c0 + j) * m + c0 + lane] = F[lane + j * kBkLd];
-  if (lane < w) {
+  if (lead && lane < w) {
     N.dval[first + c0 + lane] = my_dv;
     N.doff[first + c0 + lane] = my_dof;
     N.ptype[first + c0 + lane] = my_pt;
     N.piv[first + c0 + lane] = c0 + perm[lane];
     N.ipiv[first + c0 + perm[lane]] = c0 + lane;
     bperm[(int64_t)f * nb + lane] = perm[lane];
+    ptl[lane] = my_pt;
   }
   // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L), outer-product
-  //      form: X = I, then for t = 0.. every row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t); lane i
-  //      owns row i, row t is staged in an array of its own
-  __shared__ T Xi[64 * kBkLd];
-  __shared__ int8_t ptl[64];
-  if (lane < w) ptl[lane] = my_pt;
-  for (int j = 0; j < w; ++j) Xi[lane + j * kBkLd] = lane == j ? (T)1 : (T)0;
+  //      form: X = I, then for t = 0.. every row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t); wave w
+  //      takes the columns j = w (mod 4), row t is read into registers first
+  for (int j = wv; j < w; j += kBkWaves) Xi[lane + j * kBkLd] = lane == j ? (T)1 : (T)0;
   __syncthreads();
   for (int t = 0; t + 1 < w; ++t) {
-    s_l1[lane] = lane <= t ? Xi[t + lane * kBkLd] : (T)0;
-    __syncthreads();
+    T rt[kBkCols], old[kBkCols];
+    T* row = Xi + lane;
+#pragma unroll
+    for (int q = 0; q < kBkCols; ++q) {
+      rt[q] = Xi[t + (wv + kBkWaves * q) * kBkLd];
+      old[q] = row[(wv + kBkWaves * q) * kBkLd];
+    }
     const bool pair = lane == t + 1 && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
-    const T lit = (lane > t && lane < w && !pair) ? F[lane + t * kBkLd] : (T)0;
-    if (lit != (T)0) {
-      T* row = Xi + lane;
-#pragma unroll 8
-      for (int j = 0; j <= t; ++j) row[j * kBkLd] -= lit * s_l1[j];
+    const T ft = F[lane + t * kBkLd];
+    const T lit = (lane > t && lane < w && !pair) ? ft : (T)0;
+#pragma unroll
+    for (int q = 0; q < kBkCols; ++q) {
+      const int j = wv + kBkWaves * q;
+      if (lit != (T)0 && j <= t) row[j * kBkLd] = old[q] - lit * rt[q];
     }
     __syncthreads();
   }
   T* out = Linv + (int64_t)f * nb * nb;
-  for (int j = 0; j < w; ++j)
+  for (int j = wv; j < w; j += kBkWaves)
     if (lane < w) out[(int64_t)j * w + lane] = Xi[lane + j * kBkLd];
-  if (lane == 0) {
+  if (lead && lane == 0) {
     unsigned long long* cnt = N.cnt;
     if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
     if (zero) atomicAdd(&cnt[kCntZero], (unsigned long long)zero);
@@ -1171,6 +1211,7 @@ void Engine::validate_solve_plan() const {
 }
 
 void Engine::ensure_device() {
+  std::lock_guard<std::mutex> lk(device_mu_);
   if (on_device_) return;
   std::vector<int32_t> rows_orig(sym_.rows.size());
   for (size_t k = 0; k < sym_.rows.size(); ++k) rows_orig[k] = sym_.order[sym_.rows[k]];
@@ -1204,7 +1245,6 @@ void Engine::ensure_device() {
 }
 
 Engine::~Engine() {
-  free_numeric();
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
                   (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
@@ -1218,7 +1258,9 @@ Engine::~Engine() {
   }
 }
 
-void Engine::free_numeric() {
+Numeric::~Numeric() { free_all(); }
+
+void Numeric::free_all() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
                   (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
                   (void*)d_pos_, (void*)d_rpos_})
@@ -1234,27 +1276,27 @@ void Engine::free_numeric() {
   valid_ = false;
 }
 
-void Engine::release() { free_numeric(); }
+void Engine::release(Numeric& nm) { nm.free_all(); }
 
-void Engine::alloc_numeric(int precision) {
-  if (precision_ == precision && d_L_) return;
-  free_numeric();
+void Engine::alloc_numeric(Numeric& nm, int precision) {
+  if (nm.precision_ == precision && nm.d_L_) return;
+  nm.free_all();
   const size_t ts = precision == 0 ? sizeof(double) : sizeof(float);
   const int64_t n = sym_.n;
-  hip_check(hipMalloc(&d_L_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (factor panels)");
-  hip_check(hipMalloc(&d_CB_, std::max<int64_t>(1, cb_total_) * ts), "hipMalloc (contribution blocks)");
-  hip_check(hipMalloc(&d_W_, std::max<int64_t>(1, w_total_) * ts), "hipMalloc (panel workspace)");
-  hip_check(hipMalloc(&d_dval_, n * ts), "hipMalloc (D)");
-  hip_check(hipMalloc(&d_doff_, n * ts), "hipMalloc (D)");
-  hip_check(hipMalloc(&d_ptype_, n), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&d_piv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&d_ipiv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t)),
+  hip_check(hipMalloc(&nm.d_L_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (factor panels)");
+  hip_check(hipMalloc(&nm.d_CB_, std::max<int64_t>(1, cb_total_) * ts), "hipMalloc (contribution blocks)");
+  hip_check(hipMalloc(&nm.d_W_, std::max<int64_t>(1, w_total_) * ts), "hipMalloc (panel workspace)");
+  hip_check(hipMalloc(&nm.d_dval_, n * ts), "hipMalloc (D)");
+  hip_check(hipMalloc(&nm.d_doff_, n * ts), "hipMalloc (D)");
+  hip_check(hipMalloc(&nm.d_ptype_, n), "hipMalloc (pivots)");
+  hip_check(hipMalloc(&nm.d_piv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
+  hip_check(hipMalloc(&nm.d_ipiv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
+  hip_check(hipMalloc(&nm.d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t)),
             "hipMalloc (pivots)");
-  hip_check(hipMalloc(&d_cnt_, kCntSlots * sizeof(unsigned long long)), "hipMalloc (counters)");
-  hip_check(hipMalloc(&d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts),
+  hip_check(hipMalloc(&nm.d_cnt_, kCntSlots * sizeof(unsigned long long)), "hipMalloc (counters)");
+  hip_check(hipMalloc(&nm.d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts),
             "hipMalloc (panel inverses)");
-  precision_ = precision;
+  nm.precision_ = precision;
 }
 
 // a kernel launch counted in `nl` (launches of a timed phase)
@@ -1265,11 +1307,11 @@ void Engine::alloc_numeric(int precision) {
   } while (0)
 
 template <class T>
-InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, const double* bv, double null_tol,
+InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const double* av, const double* bv, double null_tol,
                                   hipStream_t st) {
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
-  DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
+  DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
   const int nb = opt_.panel_width;
   const int detect = null_tol > 0 ? 1 : 0;
   const int64_t tok = timing_begin(kPhaseFactor, st);
@@ -1277,10 +1319,10 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
   HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(panel_total_)), dim3(256), 0, st, N.L, panel_total_, (T)0);
   if (cb_total_ > 0)
     HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(cb_total_)), dim3(256), 0, st, N.CB, cb_total_, (T)0);
-  HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, d_cnt_, (int)kCntSlots);
+  HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, nm.d_cnt_, (int)kCntSlots);
   if (nnz_ > 0)
     HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
-                       d_ib_, av, bv, alpha, beta, N.L, d_cnt_);
+                       d_ib_, av, bv, alpha, beta, N.L, nm.d_cnt_);
   for (const Level& L : levels_) {
     if (L.n_small) {
       const int m = L.small_maxm;
@@ -1298,10 +1340,10 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
-        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large), dim3(64), 0, st, S, N, L.d_large, p, nb, d_bperm_, null_tol,
-                           detect, (T*)d_pinv_);
+        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb, nm.d_bperm_,
+                           null_tol, detect, (T*)nm.d_pinv_);
         HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
-                           nb, (const int32_t*)d_bperm_, (const T*)d_pinv_);
+                           nb, (const int32_t*)nm.d_bperm_, (const T*)nm.d_pinv_);
         const int rem = L.large_maxm - (p + 1) * nb;
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
@@ -1319,7 +1361,7 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
   hip_check(hipGetLastError(), "factorization kernels");
   timing_end(tok, st, nl, sym_.flops);
   unsigned long long h[kCntSlots];
-  hip_check(hipMemcpyAsync(h, d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
+  hip_check(hipMemcpyAsync(h, nm.d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
   hip_check(hipStreamSynchronize(st), "factorization");
   InertiaResult r;
   r.neg = (int64_t)h[kCntNeg];
@@ -1332,31 +1374,31 @@ InertiaResult Engine::factorize_t(double alpha, double beta, const double* av, c
   return r;
 }
 
-InertiaResult Engine::factorize(double alpha, double beta, const double* av, const double* bv, int precision,
+InertiaResult Engine::factorize(Numeric& nm, double alpha, double beta, const double* av, const double* bv, int precision,
                                 double null_tol, hipStream_t stream) {
-  valid_ = false;
-  g_ready_ = false;
+  nm.valid_ = false;
+  nm.g_ready_ = false;
   ensure_device();
-  alloc_numeric(precision);
-  InertiaResult r = precision == 0 ? factorize_t<double>(alpha, beta, av, bv, null_tol, stream)
-                                   : factorize_t<float>(alpha, beta, av, bv, null_tol, stream);
-  valid_ = !r.singular;
+  alloc_numeric(nm, precision);
+  InertiaResult r = precision == 0 ? factorize_t<double>(nm, alpha, beta, av, bv, null_tol, stream)
+                                   : factorize_t<float>(nm, alpha, beta, av, bv, null_tol, stream);
+  nm.valid_ = !r.singular;
   return r;
 }
 
 template <class T>
-void Engine::build_inverse_panels(hipStream_t st) {
+void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
   const size_t ts = sizeof(T);
-  if (!d_G_) hip_check(hipMalloc(&d_G_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
-  if (!d_Gt_) hip_check(hipMalloc(&d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
-  if (!d_T_) hip_check(hipMalloc(&d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
-  if (!d_pos_) hip_check(hipMalloc(&d_pos_, (size_t)sym_.n * sizeof(int32_t)), "hipMalloc (pivot positions)");
-  if (!d_rpos_)
-    hip_check(hipMalloc(&d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t)), "hipMalloc (pivot positions)");
+  if (!nm.d_G_) hip_check(hipMalloc(&nm.d_G_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
+  if (!nm.d_Gt_) hip_check(hipMalloc(&nm.d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
+  if (!nm.d_T_) hip_check(hipMalloc(&nm.d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
+  if (!nm.d_pos_) hip_check(hipMalloc(&nm.d_pos_, (size_t)sym_.n * sizeof(int32_t)), "hipMalloc (pivot positions)");
+  if (!nm.d_rpos_)
+    hip_check(hipMalloc(&nm.d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t)), "hipMalloc (pivot positions)");
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
-  DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
-  T* G = (T*)d_G_;
+  DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
+  T* G = (T*)nm.d_G_;
   const int64_t tok = timing_begin(kPhaseInverse, st);
   int64_t nl = 0;
   if (n_pairs_ > 0) {
@@ -1366,49 +1408,49 @@ void Engine::build_inverse_panels(hipStream_t st) {
   for (const TaskRange& r : gstages_)
     if (r.count > 0) {
       hipLaunchKernelGGL(k_tile_gemm<T>, dim3((unsigned)r.count), dim3(256), 0, st, d_tasks_ + r.begin, N.L, G,
-                         (T*)d_T_);
+                         (T*)nm.d_T_);
       ++nl;
     }
   if (n_ttiles_ > 0) {
-    hipLaunchKernelGGL(k_transpose_g<T>, dim3(n_ttiles_), dim3(256), 0, st, S, d_ttiles_, (const T*)G, (T*)d_Gt_);
+    hipLaunchKernelGGL(k_transpose_g<T>, dim3(n_ttiles_), dim3(256), 0, st, S, d_ttiles_, (const T*)G, (T*)nm.d_Gt_);
     ++nl;
   }
   // pivot positions of the rows after this factorization's local pivoting
-  hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, d_pos_);
+  hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, nm.d_pos_);
   hipLaunchKernelGGL(k_row_positions, dim3(blocks_for(rows_total_)), dim3(256), 0, st, rows_total_,
-                     (const int32_t*)d_rows_orig_, (const int32_t*)d_pos_, d_rpos_);
+                     (const int32_t*)d_rows_orig_, (const int32_t*)nm.d_pos_, nm.d_rpos_);
   nl += 2;
   hip_check(hipGetLastError(), "inverse panel kernels");
   timing_end(tok, st, nl, g_flops_);
-  g_ready_ = true;
+  nm.g_ready_ = true;
 }
 
 template <class T>
-void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
-  if (!g_ready_) build_inverse_panels<T>(st);
+void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
+  if (!nm.g_ready_) build_inverse_panels<T>(nm, st);
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
-  DevNum<T> N{(T*)d_L_, (T*)d_CB_, (T*)d_W_, (T*)d_dval_, (T*)d_doff_, d_ptype_, d_piv_, d_ipiv_, d_cnt_};
-  if (ws_cols_ < nrhs) {
-    for (void* q : {d_Xp_, d_Y_, d_U_, d_part_}) dfree(q);
-    d_Xp_ = d_Y_ = d_U_ = d_part_ = nullptr;
+  DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
+  if (nm.ws_cols_ < nrhs) {
+    for (void* q : {nm.d_Xp_, nm.d_Y_, nm.d_U_, nm.d_part_}) dfree(q);
+    nm.d_Xp_ = nm.d_Y_ = nm.d_U_ = nm.d_part_ = nullptr;
     const int64_t nrc = (nrhs + kMaxRhsChunk - 1) / kMaxRhsChunk;
-    hip_check(hipMalloc(&d_Xp_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
+    hip_check(hipMalloc(&nm.d_Xp_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
+    hip_check(hipMalloc(&nm.d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
+    hip_check(hipMalloc(&nm.d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
               "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T)),
+    hip_check(hipMalloc(&nm.d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T)),
               "hipMalloc (solve workspace)");
-    ws_cols_ = nrhs;
+    nm.ws_cols_ = nrhs;
   }
-  T* part = (T*)d_part_;
-  T* Xp = (T*)d_Xp_;
-  T* Y = (T*)d_Y_;
-  T* U = (T*)d_U_;
-  const T* G = (const T*)d_G_;
-  const T* Gt = (const T*)d_Gt_;
-  const int32_t* pos = d_pos_;
-  const int32_t* rpos = d_rpos_;
+  T* part = (T*)nm.d_part_;
+  T* Xp = (T*)nm.d_Xp_;
+  T* Y = (T*)nm.d_Y_;
+  T* U = (T*)nm.d_U_;
+  const T* G = (const T*)nm.d_G_;
+  const T* Gt = (const T*)nm.d_Gt_;
+  const int32_t* pos = nm.d_pos_;
+  const int32_t* rpos = nm.d_rpos_;
   const int p = (int)nrhs;
   // algorithmic bytes per sweep: every stored factor entry once, the right-hand sides read and
   // written once (FP64)
@@ -1455,12 +1497,12 @@ void Engine::solve_t(double* x, int64_t nrhs, int64_t ldx, hipStream_t st) {
   hip_check(hipGetLastError(), "solve kernels");
 }
 
-void Engine::solve(double* x, int64_t nrhs, int64_t ldx, hipStream_t stream) {
-  if (!valid_) throw std::runtime_error("solve without a valid factorization");
+void Engine::solve(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t stream) {
+  if (!nm.valid_) throw std::runtime_error("solve without a valid factorization");
   if (nrhs <= 0) return;
   if (ldx < sym_.n) throw std::runtime_error("solve: ldx < n");
-  if (precision_ == 0) solve_t<double>(x, nrhs, ldx, stream);
-  else solve_t<float>(x, nrhs, ldx, stream);
+  if (nm.precision_ == 0) solve_t<double>(nm, x, nrhs, ldx, stream);
+  else solve_t<float>(nm, x, nrhs, ldx, stream);
 }
 
 

@@ -27,6 +27,7 @@ import torch
 from .errors import InternalError, SingularError
 from .factor import GpuFactorization, Pencil
 from .krylov import KrylovOptions, lanczos_krylov_schur
+from . import hostla
 from .ops import DeviceOps, InexactFactorFailure, ShiftInvertOperator, SolveRefinement
 
 EPS = float(np.finfo(np.float64).eps)
@@ -101,6 +102,7 @@ class Ctx:
     scale: float = 1.0
     held: FactorHeld = field(default_factory=FactorHeld)
     precision: str = "double"      # precision of the working (Krylov) factors
+    streams: int = 1               # concurrent streams for spectrum slices
     events: list = field(default_factory=list)
     certified: list = field(default_factory=list)
 
@@ -203,13 +205,13 @@ def rayleigh_ritz_definite(blk: Block, drop_tol: float = DEPENDENT_GRAM) -> int:
     h = ops.gram(blk.x, blk.bx).cpu().numpy().T
     g = 0.5 * (g + g.T)
     h = 0.5 * (h + h.T)
-    mu, V = np.linalg.eigh(h)
+    mu, V = hostla.eigh(h)
     keep = [q for q in range(m) if mu[q] > drop_tol * mu[-1] and mu[q] > 0.0]
     r = len(keep)
     Y = V[:, keep] / np.sqrt(mu[keep])
     gr = Y.T @ g @ Y
     gr = 0.5 * (gr + gr.T)
-    theta, Z = np.linalg.eigh(gr)
+    theta, Z = hostla.eigh(gr)
     coef = Y @ Z
     blk.x = _replace_by_product(ops, blk.x, coef)
     blk.ax = _replace_by_product(ops, blk.ax, coef)

@@ -257,6 +257,28 @@ class GpuFactorization:
         self.stats.rhs_solved += int(x2.shape[0])
         return x
 
+    def clone(self) -> "GpuFactorization":
+        """A factorization object over the same analysis (not recomputed) with numeric storage of
+        its own: independent factorizations (spectrum slices, counts) can then be held at the same
+        time and run on different streams. Keeps this object alive while the clone exists."""
+        other = GpuFactorization.__new__(GpuFactorization)
+        other.pencil = self.pencil
+        other.precision = self.precision
+        other.stats = FactorStats(factor_entries=self.stats.factor_entries, flops=self.stats.flops,
+                                  supernodes=self.stats.supernodes)
+        other.stats.max_front = getattr(self.stats, "max_front", 0)
+        other.detection = True
+        other._inertia = Inertia()
+        other._opts = self._opts
+        other._given_order = self._given_order
+        other.sigma_held = None
+        other.info = self.info
+        other._parent = self
+        h = C.c_void_p(0)
+        _lib.call("hsg_factor_clone", self._handle, C.byref(h))
+        other._handle = h
+        return other
+
     def release_factors(self) -> bool:
         _lib.call("hsg_release_factors", self._handle)
         self._inertia = Inertia(self._inertia.neg, self._inertia.zero, self._inertia.pos, False)

@@ -23,9 +23,9 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
-import scipy.linalg as sla
 import torch
 
+from . import hostla
 from .ops import DeviceOps
 
 EPS = float(np.finfo(np.float64).eps)
@@ -207,7 +207,7 @@ class BlockLanczos:
         if np.all(np.diag(G) > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
             for _ in range(2):
                 try:
-                    Rk = sla.cholesky(G, lower=False)
+                    Rk = hostla.cholesky_upper(G)
                 except np.linalg.LinAlgError:
                     ok = False
                     break
@@ -215,7 +215,7 @@ class BlockLanczos:
                 if d.min() <= 1e-7 * d.max():  # too ill-conditioned for Cholesky-QR
                     ok = False
                     break
-                Rinv = np.linalg.inv(Rk)
+                Rinv = hostla.inv(Rk)
                 tmp = ops.empty(p)
                 ops.update(w, torch.from_numpy(np.ascontiguousarray(Rinv.T)).to(ops.dev), tmp, 1.0, 0.0)
                 w.copy_(tmp)
@@ -298,7 +298,7 @@ class BlockLanczos:
         m, p = self.m, self.p
         Hm = self.H[:m, :m]
         Hm = 0.5 * (Hm + Hm.T)
-        theta, S = np.linalg.eigh(Hm)
+        theta, S = hostla.eigh(Hm)
         self.coupling = self.rlast @ S[m - p:m, :]
         est = np.linalg.norm(self.coupling, axis=0)
         self.rr = RitzPairs(theta, S, est, self.opt.tol, self.which)

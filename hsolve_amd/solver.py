@@ -106,13 +106,46 @@ def prepare(A, B=None, config: dict | None = None) -> Prepared:
     return Prepared(A, B, bk, def_a, def_b, pencil)
 
 
-def solve(A=None, B=None, config: dict | None = None, *, prepared: Prepared | None = None, vectors: str = "host"):
+def slice_streams(fac: GpuFactorization, requested: int | None) -> int:
+    """Streams for concurrent spectrum slices: `requested`, or by default up to 8, as many as the
+    free device memory holds extra factorization objects for (panels, contribution blocks,
+    panel workspace and the solve phase's two inverse-panel copies each)."""
+    if requested is not None:
+        return max(1, int(requested))
+    per = 4.0 * fac.info.panel_bytes_double + fac.info.cb_bytes_double
+    try:
+        free, _ = torch.cuda.mem_get_info()
+    except Exception:
+        return 1
+    return int(max(1, min(8, 1 + (0.5 * free) // max(per, 1.0))))
+
+
+def solve(A=None, B=None, config: dict | None = None, *, prepared: Prepared | None = None, vectors: str = "host",
+          streams: int | None = None):
     """Returns (eigenvalues ndarray, eigenvectors, report dict, config dict). The eigenvectors are an
     (n, m) Fortran-order ndarray (vectors="host") or an (m, n) float64 tensor on the device whose
     row j is eigenvector j (vectors="device"). `prepared` (from prepare() on the same matrices)
-    skips the input stage and the uploads."""
+    skips the input stage and the uploads. `streams`: concurrent streams for the spectrum slices
+    of a window target (default: automatic, see slice_streams; 1 = one slice after the other)."""
     if vectors not in ("host", "device"):
         raise InvalidInputError(f"vectors = {vectors!r} is not 'host' or 'device'")
+    with _one_blas_thread():
+        return _solve(A, B, config, prepared, vectors, streams)
+
+
+def _one_blas_thread():
+    """The host-side dense work of a solve is small (projected problems of order <= a few hundred,
+    p x p Gram matrices): multithreaded BLAS only adds synchronization there, so a solve runs it
+    on one thread (restored afterwards)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        return threadpool_limits(limits=1, user_api="blas")
+    except Exception:  # threadpoolctl missing: leave the BLAS as configured
+        import contextlib
+        return contextlib.nullcontext()
+
+
+def _solve(A, B, config, prepared, vectors, streams):
     cfg = cfgmod.parse_config(config or {})
     rep = cfgmod.new_report()
     clock = PhaseClock()
@@ -149,6 +182,7 @@ def solve(A=None, B=None, config: dict | None = None, *, prepared: Prepared | No
             if not ok or inn.neg > 0:
                 raise UnsupportedError("B is not positive definite (GHIEP): not supported by this engine")
     ctx = Ctx(cfg=cfg, rep=rep, clock=clock, pencil=pencil, ops=ops, fac=fac, n2a=n2a, n2b=n2b, scale=scale)
+    ctx.streams = slice_streams(fac, streams)
     requested = cfg["factor_precision"]
     ctx.precision = "single" if requested == "single" else "double"
     rep["precision"]["requested"] = requested
@@ -230,4 +264,4 @@ def _solve_definite(ctx: Ctx, mode: str):
         from .nearest import solve_nearest
         return solve_nearest(ctx)
     from .windows import solve_window
-    return solve_window(ctx)
+    return solve_window(ctx, ctx.streams)

@@ -14,18 +14,25 @@ without the zero band (zero_filter_tol < 0, the default) and without helper proc
     shift), at most missing + 4 (<= 32) rounds, stop after 2 stalls (:1362-1520);
   * rank-revealing de-duplication of pairs near the cuts (:1522-1556), assignment of pairs to
     windows with the end checks (assign_to_windows, :66-134), target order of the output.
+The reference hands slices to helper processes (definite.cpp:1586-1837); here independent slices
+run at the same time on separate streams of the one device, each on its own factorization object
+over the shared analysis, and their results are merged in slice order (so they do not depend on
+which slice finished first).
 """
 from __future__ import annotations
 
+import dataclasses
 import math
+import threading
 from dataclasses import dataclass
 
 import numpy as np
 import torch
 
-from .driver_core import (Block, Ctx, DefiniteRun, block_metrics, count_below, factorize_near, keep_converged,
-                          lanczos_run, rayleigh_ritz_definite, refactorize_at, try_refactorize_at,
-                          verify_and_rescue, TRUSTWORTHY)
+from . import config as cfgmod
+from .driver_core import (Block, Ctx, DefiniteRun, FactorHeld, PhaseClock, block_metrics, count_below,
+                          factorize_near, keep_converged, lanczos_run, rayleigh_ritz_definite, refactorize_at,
+                          try_refactorize_at, verify_and_rescue, TRUSTWORTHY)
 from .errors import InvalidInputError, NotConvergedError
 from .krylov import LARGEST_ALGEBRAIC, LARGEST_MAGNITUDE, SMALLEST_ALGEBRAIC
 from .nearest import rerun_block_size
@@ -77,8 +84,11 @@ def slice_block_size(cfg: dict, nev: int) -> int:
 
 
 class WindowSolver:
-    def __init__(self, ctx: Ctx):
-        self.c = ctx
+    def __init__(self, ctx: Ctx, streams: int = 1):
+        self._main = ctx
+        self._tls = threading.local()      # a slice worker's own context
+        self._lock = threading.Lock()      # the count cache
+        self.streams = max(1, int(streams))
         self.n = ctx.pencil.n
         self.scale = ctx.scale
         self.known: list[CountPoint] = []
@@ -90,6 +100,11 @@ class WindowSolver:
         self.pool_slice = []
         self.restarts = 0
         self.applies = 0
+
+    @property
+    def c(self) -> Ctx:
+        """The context of the calling thread: a slice worker's own, else the solve's."""
+        return getattr(self._tls, "ctx", None) or self._main
 
     # ------------------------------------------------------------------ counts
     def minus_inf(self):
@@ -107,15 +122,17 @@ class WindowSolver:
         self.known.sort(key=lambda q: q.s)
 
     def count(self, s: float, outward: float) -> CountPoint:
-        for p in self.known:
-            if p.s == s:
-                return p
+        with self._lock:
+            for p in self.known:
+                if p.s == s:
+                    return p
         step = 1e-10 * max(self.scale, abs(s))
         nu, s_used, off, near = count_below(self.c, s, outward, step, s_is_gap=False)
         p = CountPoint(s, nu, off, near)
-        self.remember(p)
-        if s_used != s:
-            self.remember(CountPoint(s_used, nu, off, near))
+        with self._lock:
+            self.remember(p)
+            if s_used != s:
+                self.remember(CountPoint(s_used, nu, off, near))
         self.c.event("count", s=s, nu=nu, near_eigenvalue=near)
         return p
 
@@ -400,22 +417,113 @@ class WindowSolver:
         sr["passed"] = inside - st["near_in"] <= count <= inside + st["near_out"]
         sr["restarts"] = run.restarts
         sr["op_applies"] = run.applies
-        self.restarts += run.restarts
-        self.applies += run.applies
         c.event("slice", slice=sid, lower=s.lo.s, upper=s.hi.s, sigma=sigma, which=which, count=count,
                 found=inside, passed=sr["passed"], reruns=sr["reruns"], restarts=run.restarts,
                 op_applies=run.applies)
         if not sr["passed"]:
             c.warn(f"slice [{s.lo.s:.17g}, {s.hi.s:.17g}]: inertia {s.required()} vs found {inside}")
-        while len(self.reports) <= sid:
-            self.reports.append(None)
         self.reports[sid] = sr
         sel = st["sel"]
-        for j in range(sel.size):
-            self.pool_x.append(sel.x[j:j + 1])
-            self.pool_lam.append(float(sel.lam[j]))
-            self.pool_metrics.append(st["metrics"][j])
-            self.pool_slice.append(sid)
+        self.slice_pool[sid] = ([sel.x[j:j + 1] for j in range(sel.size)], [float(l) for l in sel.lam[:sel.size]],
+                                list(st["metrics"]))
+
+    # ------------------------------------------------------------------ slices
+    def run_slices(self):
+        """Every slice, then their pairs pooled in slice order."""
+        ns = len(self.slices)
+        self.reports = [None] * ns
+        self.slice_pool = [None] * ns
+        workers = min(ns, self.streams)
+        if workers <= 1:
+            for i, sl in enumerate(self.slices):
+                self.run_slice(sl, i)
+        else:
+            self.run_slices_concurrently(workers)
+        for sid in range(ns):
+            xs, lams, mets = self.slice_pool[sid]
+            self.pool_x.extend(xs)
+            self.pool_lam.extend(lams)
+            self.pool_metrics.extend(mets)
+            self.pool_slice.extend([sid] * len(xs))
+            self.restarts += self.reports[sid]["restarts"]
+            self.applies += self.reports[sid]["op_applies"]
+
+    def run_slices_concurrently(self, workers: int):
+        """Slices handed out in order to `workers` threads, each with its own stream and its own
+        factorization object (worker 0 keeps the solve's). Each slice runs in a context of its own
+        (report fields, events, held factorization); those are merged in slice order afterwards. The
+        waiting is charged to the iterate phase."""
+        main = self._main
+        torch.cuda.synchronize()
+        facs = [main.fac] + [main.fac.clone() for _ in range(workers - 1)]
+        streams = [torch.cuda.Stream() for _ in range(workers)]
+        held = [main.held] + [FactorHeld() for _ in range(workers - 1)]
+        ns = len(self.slices)
+        ctxs = [None] * ns
+        errors = [None] * ns
+        nxt = [0]
+        order_lock = threading.Lock()
+
+        def worker(w: int):
+            with torch.cuda.stream(streams[w]):
+                while True:
+                    with order_lock:
+                        sid = nxt[0]
+                        nxt[0] += 1
+                        if any(e is not None for e in errors):
+                            sid = ns  # a slice failed: hand out nothing more
+                    if sid >= ns:
+                        break
+                    rep = cfgmod.new_report()
+                    rep["certificate"]["note"] = ""
+                    ctx = dataclasses.replace(main, fac=facs[w], held=held[w], clock=PhaseClock(), events=[],
+                                              rep=rep, certified=[])
+                    ctxs[sid] = ctx
+                    self._tls.ctx = ctx
+                    try:
+                        self.run_slice(self.slices[sid], sid)
+                    except BaseException as e:  # re-raised in slice order below
+                        errors[sid] = e
+                    finally:
+                        held[w] = ctx.held
+                        self._tls.ctx = None
+                    if errors[sid] is not None:
+                        break
+                streams[w].synchronize()
+
+        with main.clock.phase("iterate"):
+            threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-slice-{w}") for w in range(workers)]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+        torch.cuda.synchronize()
+        main.held = held[0]
+        for f in facs[1:]:
+            st, ms = f.stats, main.fac.stats
+            ms.factorizations += st.factorizations
+            ms.solves += st.solves
+            ms.rhs_solved += st.rhs_solved
+            ms.factorize_s += st.factorize_s
+            ms.solve_s += st.solve_s
+        for sid in range(ns):
+            if errors[sid] is not None:
+                raise errors[sid]
+            ctx = ctxs[sid]
+            if ctx is None:
+                continue
+            r = ctx.rep
+            main.events.extend(ctx.events)
+            main.rep["warnings"].extend(r["warnings"])
+            main.rep["shift_retries"] = max(main.rep["shift_retries"], r["shift_retries"])
+            main.rep["certificate"]["endpoint_moves"] += r["certificate"]["endpoint_moves"]
+            if r["certificate"]["note"] and not main.rep["certificate"]["note"]:
+                main.rep["certificate"]["note"] = r["certificate"]["note"]
+            if sid == 0:
+                main.rep["block_size"] = r["block_size"]
+        par = main.rep["parallel"]
+        par["helper_slices"] = ns
+        par["notes"].append(f"{ns} spectrum slices on {workers} concurrent streams of one device")
 
     def deduplicate(self) -> int:
         """definite.cpp:1522-1556: groups within 2e-9 relative holding pairs of several slices are
@@ -572,8 +680,7 @@ class WindowSolver:
         for w in windows:
             self.slices.extend(self.plan(w, c_target, c_max))
         c.event("slices", slices=len(self.slices))
-        for i, s in enumerate(self.slices):
-            self.run_slice(s, i)
+        self.run_slices()
         with c.clock.phase("verify"):
             self.deduplicate()
             inn, found = self.assign_to_windows(windows)
@@ -611,5 +718,5 @@ class WindowSolver:
         return sel, metrics, ok
 
 
-def solve_window(ctx: Ctx):
-    return WindowSolver(ctx).solve()
+def solve_window(ctx: Ctx, streams: int = 1):
+    return WindowSolver(ctx, streams).solve()

@@ -138,6 +138,13 @@ int hsg_release_factors(hsg_factor* f);
 
 void hsg_factor_destroy(hsg_factor* f);
 
+/* A new factorization object over the analysis of src (shared, not recomputed): its own numeric
+ * storage, the same pattern, ordering and supernodal plan. The counterpart of the reference's
+ * helper processes that each hold the problem's analysis (src/driver/helpers.cpp): here
+ * independent factorizations (spectrum slices, certificate counts) run on different streams of
+ * one device. src must outlive the clone; a clone is destroyed with hsg_factor_destroy. */
+int hsg_factor_clone(const hsg_factor* src, hsg_factor** out);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Layer 2: kernels of the shift-invert operator and the block Krylov solver.                  */
 

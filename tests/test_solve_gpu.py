@@ -116,3 +116,21 @@ def test_report_layout_and_phases(gpu):
     phases = sum(rep[f"time_{p}_s"] for p in ("setup", "factor", "iterate", "verify", "certificate"))
     assert abs(phases - rep["time_total_s"]) <= 1e-6 * max(1.0, rep["time_total_s"])
     assert res.eigenvectors.shape == (A.shape[0], 6) and res.eigenvectors.flags["F_CONTIGUOUS"]
+
+
+def test_concurrent_slices_match_sequential(gpu):
+    # the slices of a window solved on three streams at once give the same pairs as one after the
+    # other (each slice's computation does not depend on the others or on the order they finish)
+    from hsolve_amd import solver
+    A, _, g = yee.cube_cavity(10)
+    ref = yee_exact.pec_box_spectrum(g.cells, g.lengths)
+    cfg = {"target": {"mode": "interval", "lower": 5.0, "upper": 40.0}, "tol": 1e-10, "slice_size": 24}
+    l1, v1, r1, _ = solver.solve(A, None, cfg, streams=1)
+    l3, v3, r3, _ = solver.solve(A, None, cfg, streams=3)
+    print("slices", len(r1["slices"]), "pairs", l1.size, r3["parallel"]["notes"], flush=True)
+    assert len(r1["slices"]) >= 3 and len(r3["slices"]) == len(r1["slices"])
+    assert r1["certificate"]["passed"] and r3["certificate"]["passed"]
+    assert np.array_equal(l1, l3)
+    assert r3["factor_stats"]["factorizations"] == r1["factor_stats"]["factorizations"]
+    out = ev.evaluate(A, None, l3, v3, cfg["target"], ref, ref_complete=True)
+    assert out["passed"], out
