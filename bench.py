@@ -76,11 +76,12 @@ def backward_errors(A_dev, b_diag_dev, lam, X, na, nb):
     return eta.cpu().numpy()
 
 
-PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_bwd_gemv_box48.md")
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_bwd_task_box48.md")
 
 
 def measured_traffic(alg_bytes_per_launch):
-    """HBM bytes per k_bwd_gemv launch from the committed counter summary (FETCH_SIZE and
+    """HBM bytes per backward-sweep launch (k_bwd_task and k_bwd_reduce together) from the committed
+    counter summary (FETCH_SIZE and
     WRITE_SIZE runs scaled by a copy of known size), used when it was taken on the same kernel
     source (hash of solve.hpp) and the same launches (algorithmic bytes per launch within 1 %);
     otherwise None."""
@@ -97,7 +98,7 @@ def measured_traffic(alg_bytes_per_launch):
         return None, "profile taken on another version of the solve kernels: not used"
     if alg_bytes_per_launch <= 0 or abs(a - alg_bytes_per_launch) > 0.01 * alg_bytes_per_launch:
         return None, "profile taken on other launches: not used"
-    return t, "profiles/k_bwd_gemv_box48.md (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
+    return t, "profiles/k_bwd_task_box48.md (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
 
 
 def run_form(name, prepared, cfg, steps, warmup, check, streams):

@@ -59,6 +59,9 @@ SIGNATURES = {
     "hsg_timing_enable": (C.c_int, [C.c_int]),
     "hsg_timing_read": (C.c_int, [C.c_int, _P, _P, _P, _P]),
 }
+# complex variants: the same argument lists (values and vectors as interleaved doubles)
+for _name in ("csr_spmm", "diag_spmm", "shifted_residual", "gram", "block_update", "column_dots"):
+    SIGNATURES["hsg_z" + _name] = SIGNATURES["hsg_" + _name]
 
 _lib = None
 

@@ -266,6 +266,64 @@ int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, in
   });
 }
 
+int hsg_zcsr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values, const double* x,
+                  int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_zcsr_spmm: invalid sizes");
+    hsg::zcsr_spmm(n, rowptr, colind, values, x, ldx, y, ldy, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_zdiag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                   double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_zdiag_spmm: invalid sizes");
+    hsg::zdiag_spmm(n, d, x, ldx, y, ldy, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_zshifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_colind, const double* a_values,
+                          const int64_t* b_rowptr, const int32_t* b_colind, const double* b_values,
+                          const double* b_diag, double sigma, const double* x, int64_t ldx, const double* y,
+                          int64_t ldy, double* r, int64_t ldr, int64_t p, double* rnorm2, double* omega,
+                          void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && ldr >= n && p >= 0, "hsg_zshifted_residual: invalid sizes");
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::zshifted_residual(sc.arena, n, a_rowptr, a_colind, a_values, b_rowptr, b_colind, b_values, b_diag, sigma,
+                           x, ldx, y, ldy, r, ldr, p, rnorm2, omega, as_stream(stream));
+  });
+}
+
+int hsg_zgram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w, int64_t ldw,
+              int64_t p, double* c, int64_t ldc, void* stream) {
+  return guarded([&] {
+    require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_zgram: invalid sizes");
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::zgram(sc.arena, n, v, ldv, k, w_diag, w, ldw, p, c, ldc, as_stream(stream));
+  });
+}
+
+int hsg_zblock_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                      int64_t ldw, int64_t p, double alpha, double beta, void* stream) {
+  return guarded([&] {
+    require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_zblock_update: invalid sizes");
+    hsg::zblock_update(n, v, ldv, k, c, ldc, w, ldw, p, alpha, beta, as_stream(stream));
+  });
+}
+
+int hsg_zcolumn_dots(int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                     const double* w_diag, double* out, void* stream) {
+  return guarded([&] {
+    require(ldx >= n && ldy >= n && p >= 0, "hsg_zcolumn_dots: invalid sizes");
+    StreamScratch& sc = scratch_for(as_stream(stream));
+    std::lock_guard<std::mutex> lock(sc.mu);
+    hsg::zcolumn_dots(sc.arena, n, x, ldx, y, ldy, p, w_diag, out, as_stream(stream));
+  });
+}
+
 int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
                   const double* lambda, double* out, void* stream) {
   return guarded([&] {

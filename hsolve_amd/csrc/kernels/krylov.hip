@@ -29,19 +29,53 @@ void check(hipError_t e, const char* what) {
 
 }  // namespace
 
+// Scalars of the kernels: double, or zval (complex, interleaved real and imaginary parts as in
+// std::complex<double>). The real instantiations perform exactly the operations of plain double
+// code (conj is the identity, |a| is fabs).
+struct zval {
+  double re, im;
+};
+__device__ __forceinline__ zval operator+(zval a, zval b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ zval operator-(zval a, zval b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ zval operator*(zval a, zval b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ zval operator*(double a, zval b) { return {a * b.re, a * b.im}; }
+__device__ __forceinline__ zval& operator+=(zval& a, zval b) {
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+__device__ __forceinline__ double conj_of(double a) { return a; }
+__device__ __forceinline__ zval conj_of(zval a) { return {a.re, -a.im}; }
+__device__ __forceinline__ double abs_of(double a) { return fabs(a); }
+__device__ __forceinline__ double abs_of(zval a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ double abs2_of(double a) { return a * a; }
+__device__ __forceinline__ double abs2_of(zval a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ bool finite_of(double a) { return isfinite(a); }
+__device__ __forceinline__ bool finite_of(zval a) { return isfinite(a.re) && isfinite(a.im); }
+__device__ __forceinline__ zval operator*(zval a, double b) { return {a.re * b, a.im * b}; }
+template <class S>
+__device__ __forceinline__ S zero_of() {
+  return S{};
+}
+__device__ __forceinline__ double shfl_xor_of(double v, int off) { return __shfl_xor(v, off); }
+__device__ __forceinline__ zval shfl_xor_of(zval v, int off) { return {__shfl_xor(v.re, off), __shfl_xor(v.im, off)}; }
+
 // ------------------------------------------------------------------------------ sparse kernels
 
 // Y = alpha M X + beta Y, one thread per row (rows of Yee pencils hold ~13 entries).
+template <class S>
 __global__ void k_csr_spmm(int64_t n, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                           const double* __restrict__ v, const double* __restrict__ x, int64_t ldx,
-                           double* __restrict__ y, int64_t ldy, int p, double alpha, double beta) {
+                           const S* __restrict__ v, const S* __restrict__ x, int64_t ldx, S* __restrict__ y,
+                           int64_t ldy, int p, double alpha, double beta) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = rp[i], e = rp[i + 1];
     for (int c0 = 0; c0 < p; c0 += 8) {
       const int pc = min(8, p - c0);
-      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      S acc[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = zero_of<S>();
       for (int64_t q = b; q < e; ++q) {
-        const double a = v[q];
+        const S a = v[q];
         const int64_t j = ci[q];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -50,22 +84,24 @@ __global__ void k_csr_spmm(int64_t n, const int64_t* __restrict__ rp, const int3
 #pragma unroll
       for (int c = 0; c < 8; ++c)
         if (c < pc) {
-          double* yy = y + i + (c0 + c) * ldy;
+          S* yy = y + i + (c0 + c) * ldy;
           *yy = beta == 0.0 ? alpha * acc[c] : alpha * acc[c] + beta * (*yy);
         }
     }
   }
 }
 
-__global__ void k_diag_spmm(int64_t n, const double* __restrict__ d, const double* __restrict__ x, int64_t ldx,
-                            double* __restrict__ y, int64_t ldy, int p, double alpha, double beta) {
+// Y = alpha diag(d) X + beta Y (d real; NULL = identity).
+template <class S>
+__global__ void k_diag_spmm(int64_t n, const double* __restrict__ d, const S* __restrict__ x, int64_t ldx,
+                            S* __restrict__ y, int64_t ldy, int p, double alpha, double beta) {
   const int64_t total = n * p;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = idx % n, c = idx / n;
     const double dv = d ? d[i] : 1.0;
-    double* yy = y + i + c * ldy;
-    const double t = alpha * dv * x[i + c * ldx];
+    S* yy = y + i + c * ldy;
+    const S t = (alpha * dv) * x[i + c * ldx];
     *yy = beta == 0.0 ? t : t + beta * (*yy);
   }
 }
@@ -74,12 +110,12 @@ __global__ void k_diag_spmm(int64_t n, const double* __restrict__ d, const doubl
 // componentwise quotient |R_i| / (|B| |W| + |A| |Y|)_i, W = X + sigma Y (0/0 = 0, r/0 = inf).
 constexpr int kResThreads = 256;
 
+template <class S>
 __global__ void __launch_bounds__(kResThreads)
     k_shifted_residual(int64_t n, const int64_t* __restrict__ arp, const int32_t* __restrict__ aci,
-                       const double* __restrict__ av, const int64_t* __restrict__ brp,
-                       const int32_t* __restrict__ bci, const double* __restrict__ bv,
-                       const double* __restrict__ bd, double sigma, const double* __restrict__ x, int64_t ldx,
-                       const double* __restrict__ y, int64_t ldy, double* __restrict__ r, int64_t ldr, int p,
+                       const S* __restrict__ av, const int64_t* __restrict__ brp, const int32_t* __restrict__ bci,
+                       const S* __restrict__ bv, const double* __restrict__ bd, double sigma, const S* __restrict__ x,
+                       int64_t ldx, const S* __restrict__ y, int64_t ldy, S* __restrict__ r, int64_t ldr, int p,
                        double* __restrict__ part_sum, double* __restrict__ part_max) {
   __shared__ double ssum[kResThreads / 64][16];
   __shared__ double smax[kResThreads / 64][16];
@@ -95,32 +131,35 @@ __global__ void __launch_bounds__(kResThreads)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
       for (int c = 0; c < pc; ++c) {
         const int64_t cc = c0 + c;
-        double ay = 0.0, aay = 0.0;
+        S ay = zero_of<S>();
+        double aay = 0.0;
         for (int64_t q = arp[i]; q < arp[i + 1]; ++q) {
-          const double a = av[q], yy = y[aci[q] + cc * ldy];
+          const S a = av[q], yy = y[aci[q] + cc * ldy];
           ay += a * yy;
-          aay += fabs(a) * fabs(yy);
+          aay += abs_of(a) * abs_of(yy);
         }
-        double bw = 0.0, bbw = 0.0;
+        S bw = zero_of<S>();
+        double bbw = 0.0;
         if (brp) {
           for (int64_t q = brp[i]; q < brp[i + 1]; ++q) {
             const int64_t j = bci[q];
-            const double w = x[j + cc * ldx] + sigma * y[j + cc * ldy];
+            const S w = x[j + cc * ldx] + sigma * y[j + cc * ldy];
             bw += bv[q] * w;
-            bbw += fabs(bv[q]) * fabs(w);
+            bbw += abs_of(bv[q]) * abs_of(w);
           }
         } else {
           const double dv = bd ? bd[i] : 1.0;
-          const double w = x[i + cc * ldx] + sigma * y[i + cc * ldy];
+          const S w = x[i + cc * ldx] + sigma * y[i + cc * ldy];
           bw = dv * w;
-          bbw = fabs(dv) * fabs(w);
+          bbw = fabs(dv) * abs_of(w);
         }
-        const double ri = bw - ay;
+        const S ri = bw - ay;
         r[i + cc * ldr] = ri;
-        lsum[c] += ri * ri;
+        lsum[c] += abs2_of(ri);
         const double s = bbw + aay;
-        const double q = s > 0.0 ? fabs(ri) / s : (ri == 0.0 ? 0.0 : INFINITY);
-        lmax[c] = (isfinite(ri) && isfinite(s)) ? fmax(lmax[c], q) : INFINITY;
+        const double ra = abs_of(ri);
+        const double q = s > 0.0 ? ra / s : (ra == 0.0 ? 0.0 : INFINITY);
+        lmax[c] = (finite_of(ri) && isfinite(s)) ? fmax(lmax[c], q) : INFINITY;
       }
     }
     for (int c = 0; c < pc; ++c) {
@@ -169,43 +208,51 @@ __global__ void k_reduce_partials(int groups, int p, const double* __restrict__ 
 
 // ------------------------------------------------------------------------------ dense kernels
 
-// Partial Gram C_g = V(rows_g, k0:k0+64)^T diag(w) W(rows_g, :) per workgroup g (row range) and
-// column tile k0; a row tile of 64 rows is staged through LDS with coalesced loads.
-constexpr int kGramRows = 64;
+// Partial Gram C_g = V(rows_g, k0:k0+64)^H diag(w) W(rows_g, :) per workgroup g (row range) and
+// column tile k0; a row tile (64 rows of doubles, 32 of complex values) is staged through LDS with
+// coalesced loads.
 constexpr int kGramCols = 64;
 constexpr int kGramP = 16;
+template <class S>
+constexpr int gram_rows() {
+  return sizeof(S) == sizeof(double) ? 64 : 32;
+}
 
+template <class S>
 __global__ void __launch_bounds__(256)
-    k_gram_partial(int64_t n, const double* __restrict__ v, int64_t ldv, int k, const double* __restrict__ wd,
-                   const double* __restrict__ w, int64_t ldw, int p, double* __restrict__ part) {
-  __shared__ double Vt[kGramRows][kGramCols + 1];
-  __shared__ double Wt[kGramRows][kGramP + 1];
+    k_gram_partial(int64_t n, const S* __restrict__ v, int64_t ldv, int k, const double* __restrict__ wd,
+                   const S* __restrict__ w, int64_t ldw, int p, S* __restrict__ part) {
+  constexpr int kRows = gram_rows<S>();
+  __shared__ S Vt[kRows][kGramCols + 1];
+  __shared__ S Wt[kRows][kGramP + 1];
   const int k0 = blockIdx.y * kGramCols;
   const int kc = min(kGramCols, k - k0);
   const int tid = threadIdx.x;
   const int kk = tid % 64, cg = tid / 64;  // column of V, group of 4 columns of W
   for (int c0 = 0; c0 < p; c0 += kGramP) {
     const int pc = min(kGramP, p - c0);
-    double acc[4] = {0, 0, 0, 0};
-    for (int64_t r0 = blockIdx.x * (int64_t)kGramRows; r0 < n; r0 += (int64_t)gridDim.x * kGramRows) {
-      const int rn = (int)min<int64_t>(kGramRows, n - r0);
-      for (int idx = tid; idx < kGramRows * kGramCols; idx += 256) {
-        const int r = idx % kGramRows, j = idx / kGramRows;
-        Vt[r][j] = (r < rn && j < kc) ? v[r0 + r + (int64_t)(k0 + j) * ldv] : 0.0;
+    S acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = zero_of<S>();
+    for (int64_t r0 = blockIdx.x * (int64_t)kRows; r0 < n; r0 += (int64_t)gridDim.x * kRows) {
+      const int rn = (int)min<int64_t>(kRows, n - r0);
+      for (int idx = tid; idx < kRows * kGramCols; idx += 256) {
+        const int r = idx % kRows, j = idx / kRows;
+        Vt[r][j] = (r < rn && j < kc) ? conj_of(v[r0 + r + (int64_t)(k0 + j) * ldv]) : zero_of<S>();
       }
-      for (int idx = tid; idx < kGramRows * kGramP; idx += 256) {
-        const int r = idx % kGramRows, c = idx / kGramRows;
-        double val = 0.0;
+      for (int idx = tid; idx < kRows * kGramP; idx += 256) {
+        const int r = idx % kRows, c = idx / kRows;
+        S val = zero_of<S>();
         if (r < rn && c < pc) {
           val = w[r0 + r + (int64_t)(c0 + c) * ldw];
-          if (wd) val *= wd[r0 + r];
+          if (wd) val = wd[r0 + r] * val;
         }
         Wt[r][c] = val;
       }
       __syncthreads();
 #pragma unroll 4
-      for (int r = 0; r < kGramRows; ++r) {
-        const double a = Vt[r][kk];
+      for (int r = 0; r < kRows; ++r) {
+        const S a = Vt[r][kk];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] += a * Wt[r][cg * 4 + q];
       }
@@ -220,7 +267,8 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// C(i, c) = sum over g of part[g][c][i] in fixed order.
+// C(i, c) = sum over g of part[g][c][i] in fixed order (doubles: a complex partial is reduced as
+// its two parts, k counting doubles).
 __global__ void k_gram_reduce(int groups, int k, int p, const double* __restrict__ part, double* __restrict__ c,
                               int64_t ldc) {
   const int64_t total = (int64_t)k * p;
@@ -234,58 +282,62 @@ __global__ void k_gram_reduce(int groups, int k, int p, const double* __restrict
 }
 
 // W = beta W + alpha V C; one thread per row, 16 columns of W at a time, C staged in LDS.
+template <class S>
 __global__ void __launch_bounds__(256)
-    k_block_update(int64_t n, const double* __restrict__ v, int64_t ldv, int k, const double* __restrict__ cmat,
-                   int64_t ldc, double* __restrict__ w, int64_t ldw, int p, double alpha, double beta) {
-  extern __shared__ double cs[];  // k x 16
+    k_block_update(int64_t n, const S* __restrict__ v, int64_t ldv, int k, const S* __restrict__ cmat, int64_t ldc,
+                   S* __restrict__ w, int64_t ldw, int p, double alpha, double beta) {
+  extern __shared__ __align__(16) unsigned char cs_raw[];
+  S* cs = reinterpret_cast<S*>(cs_raw);  // k x 16
   for (int c0 = 0; c0 < p; c0 += 16) {
     const int pc = min(16, p - c0);
     __syncthreads();
     for (int idx = threadIdx.x; idx < k * 16; idx += blockDim.x) {
       const int j = idx % k, c = idx / k;
-      cs[idx] = c < pc ? cmat[j + (int64_t)(c0 + c) * ldc] : 0.0;
+      cs[idx] = c < pc ? cmat[j + (int64_t)(c0 + c) * ldc] : zero_of<S>();
     }
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      double acc[16];
+      S acc[16];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) acc[c] = 0.0;
+      for (int c = 0; c < 16; ++c) acc[c] = zero_of<S>();
       for (int j = 0; j < k; ++j) {
-        const double a = v[i + (int64_t)j * ldv];
+        const S a = v[i + (int64_t)j * ldv];
 #pragma unroll
         for (int c = 0; c < 16; ++c) acc[c] += a * cs[j + c * k];
       }
       for (int c = 0; c < pc; ++c) {
-        double* ww = w + i + (int64_t)(c0 + c) * ldw;
+        S* ww = w + i + (int64_t)(c0 + c) * ldw;
         *ww = beta == 0.0 ? alpha * acc[c] : alpha * acc[c] + beta * (*ww);
       }
     }
   }
 }
 
-// out[c] = x_c^T diag(w) y_c per column (w may be NULL), partials per workgroup then an ordered sum.
+// out[c] = x_c^H diag(w) y_c per column (w may be NULL), partials per workgroup then an ordered
+// sum (a complex partial is reduced as its two parts).
+template <class S>
 __global__ void __launch_bounds__(256)
-    k_column_dots(int64_t n, const double* __restrict__ x, int64_t ldx, const double* __restrict__ y, int64_t ldy,
-                  int p, const double* __restrict__ wd, double* __restrict__ part) {
-  __shared__ double red[4][16];
+    k_column_dots(int64_t n, const S* __restrict__ x, int64_t ldx, const S* __restrict__ y, int64_t ldy, int p,
+                  const double* __restrict__ wd, S* __restrict__ part) {
+  __shared__ S red[4][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int c0 = 0; c0 < p; c0 += 16) {
     const int pc = min(16, p - c0);
-    double acc[16];
+    S acc[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = 0.0;
+    for (int c = 0; c < 16; ++c) acc[c] = zero_of<S>();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
       const double w = wd ? wd[i] : 1.0;
-      for (int c = 0; c < pc; ++c) acc[c] += x[i + (c0 + c) * ldx] * w * y[i + (c0 + c) * ldy];
+      for (int c = 0; c < pc; ++c) acc[c] += (conj_of(x[i + (c0 + c) * ldx]) * w) * y[i + (c0 + c) * ldy];
     }
     for (int c = 0; c < pc; ++c) {
-      double v = acc[c];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      S v = acc[c];
+      for (int off = 32; off > 0; off >>= 1) v += shfl_xor_of(v, off);
       if (lane == 0) red[wave][c] = v;
     }
     __syncthreads();
     if (threadIdx.x < pc) {
-      double v = 0.0;
+      S v = zero_of<S>();
       for (int w = 0; w < 4; ++w) v += red[w][threadIdx.x];
       part[(int64_t)blockIdx.x * p + c0 + threadIdx.x] = v;
     }
@@ -359,30 +411,35 @@ Scratch::~Scratch() {
   if (ptr_) (void)hipFree(ptr_);
 }
 
-void csr_spmm(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* x, int64_t ldx,
-              double* y, int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st) {
+namespace {
+
+template <class S>
+void csr_spmm_t(int64_t n, const int64_t* rp, const int32_t* ci, const S* v, const S* x, int64_t ldx, S* y,
+                int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st) {
   if (n <= 0 || p <= 0) return;
-  hipLaunchKernelGGL(k_csr_spmm, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, rp, ci, v, x, ldx, y, ldy,
+  hipLaunchKernelGGL(k_csr_spmm<S>, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, rp, ci, v, x, ldx, y, ldy,
                      (int)p, alpha, beta);
   check(hipGetLastError(), "csr_spmm");
 }
 
-void diag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
-               double alpha, double beta, hipStream_t st) {
+template <class S>
+void diag_spmm_t(int64_t n, const double* d, const S* x, int64_t ldx, S* y, int64_t ldy, int64_t p, double alpha,
+                 double beta, hipStream_t st) {
   if (n <= 0 || p <= 0) return;
-  hipLaunchKernelGGL(k_diag_spmm, dim3(grid_for(n * p, 256, 16384)), dim3(256), 0, st, n, d, x, ldx, y, ldy, (int)p,
-                     alpha, beta);
+  hipLaunchKernelGGL(k_diag_spmm<S>, dim3(grid_for(n * p, 256, 16384)), dim3(256), 0, st, n, d, x, ldx, y, ldy,
+                     (int)p, alpha, beta);
   check(hipGetLastError(), "diag_spmm");
 }
 
-void shifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const double* av,
-                      const int64_t* brp, const int32_t* bci, const double* bv, const double* bd, double sigma,
-                      const double* x, int64_t ldx, const double* y, int64_t ldy, double* r, int64_t ldr, int64_t p,
-                      double* rnorm2, double* omega, hipStream_t st) {
+template <class S>
+void shifted_residual_t(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const S* av,
+                        const int64_t* brp, const int32_t* bci, const S* bv, const double* bd, double sigma, const S* x,
+                        int64_t ldx, const S* y, int64_t ldy, S* r, int64_t ldr, int64_t p, double* rnorm2,
+                        double* omega, hipStream_t st) {
   if (n <= 0 || p <= 0) return;
   const int groups = grid_for(n, kResThreads, 1024);
   double* part = static_cast<double*>(scratch.get(sizeof(double) * 2 * (size_t)groups * p));
-  hipLaunchKernelGGL(k_shifted_residual, dim3(groups), dim3(kResThreads), 0, st, n, arp, aci, av, brp, bci, bv, bd,
+  hipLaunchKernelGGL(k_shifted_residual<S>, dim3(groups), dim3(kResThreads), 0, st, n, arp, aci, av, brp, bci, bv, bd,
                      sigma, x, ldx, y, ldy, r, ldr, (int)p, part, part + (size_t)groups * p);
   if (rnorm2) hipLaunchKernelGGL(k_reduce_partials, dim3(p), dim3(256), 0, st, groups, (int)p, part, rnorm2, 0);
   if (omega)
@@ -391,27 +448,120 @@ void shifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int
   check(hipGetLastError(), "shifted_residual");
 }
 
-void gram(Scratch& scratch, int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w,
-          int64_t ldw, int64_t p, double* c, int64_t ldc, hipStream_t st) {
+template <class S>
+void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, const double* wd, const S* w, int64_t ldw,
+            int64_t p, S* c, int64_t ldc, hipStream_t st) {
   if (k <= 0 || p <= 0) return;
+  constexpr int parts = sizeof(S) / sizeof(double);  // doubles per scalar
   const int kt = (int)((k + kGramCols - 1) / kGramCols);
   // row groups: enough workgroups to fill the chip, each over many row tiles
-  int groups = grid_for(n, kGramRows * 4, std::max(32, 2048 / kt));
-  double* part = static_cast<double*>(scratch.get(sizeof(double) * (size_t)groups * p * k));
-  hipLaunchKernelGGL(k_gram_partial, dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw, (int)p, part);
-  hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(k * p, 256, 1024)), dim3(256), 0, st, groups, (int)k, (int)p, part,
-                     c, ldc);
+  int groups = grid_for(n, gram_rows<S>() * 4, std::max(32, 2048 / kt));
+  S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p * k));
+  hipLaunchKernelGGL(k_gram_partial<S>, dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw, (int)p,
+                     part);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(parts * k * p, 256, 1024)), dim3(256), 0, st, groups,
+                     (int)(parts * k), (int)p, reinterpret_cast<const double*>(part), reinterpret_cast<double*>(c),
+                     parts * ldc);
   check(hipGetLastError(), "gram");
+}
+
+template <class S>
+void column_dots_t(Scratch& scratch, int64_t n, const S* x, int64_t ldx, const S* y, int64_t ldy, int64_t p,
+                   const double* wd, S* out, hipStream_t st) {
+  if (p <= 0) return;
+  constexpr int parts = sizeof(S) / sizeof(double);
+  const int groups = grid_for(n, 256, 512);
+  S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p));
+  hipLaunchKernelGGL(k_column_dots<S>, dim3(groups), dim3(256), 0, st, n, x, ldx, y, ldy, (int)p, wd, part);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(parts * p), dim3(256), 0, st, groups, (int)(parts * p),
+                     reinterpret_cast<const double*>(part), reinterpret_cast<double*>(out), 0);
+  check(hipGetLastError(), "column_dots");
+}
+
+template <class S>
+void block_update_t(int64_t n, const S* v, int64_t ldv, int64_t k, const S* c, int64_t ldc, S* w, int64_t ldw,
+                    int64_t p, double alpha, double beta, hipStream_t st) {
+  if (n <= 0 || p <= 0) return;
+  if (k <= 0) {
+    if (beta != 1.0) diag_spmm_t<S>(n, nullptr, w, ldw, w, ldw, p, beta, 0.0, st);
+    return;
+  }
+  const size_t shm = sizeof(S) * (size_t)k * 16;
+  if (shm > 150 * 1024) throw std::runtime_error("block_update: k too large");
+  hipLaunchKernelGGL(k_block_update<S>, dim3(grid_for(n, 256, 4096)), dim3(256), shm, st, n, v, ldv, (int)k, c, ldc, w,
+                     ldw, (int)p, alpha, beta);
+  check(hipGetLastError(), "block_update");
+}
+
+const zval* zc(const double* p) { return reinterpret_cast<const zval*>(p); }
+zval* zm(double* p) { return reinterpret_cast<zval*>(p); }
+
+}  // namespace
+
+void csr_spmm(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* x, int64_t ldx,
+              double* y, int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st) {
+  csr_spmm_t<double>(n, rp, ci, v, x, ldx, y, ldy, p, alpha, beta, st);
+}
+
+void diag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+               double alpha, double beta, hipStream_t st) {
+  diag_spmm_t<double>(n, d, x, ldx, y, ldy, p, alpha, beta, st);
+}
+
+void shifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const double* av,
+                      const int64_t* brp, const int32_t* bci, const double* bv, const double* bd, double sigma,
+                      const double* x, int64_t ldx, const double* y, int64_t ldy, double* r, int64_t ldr, int64_t p,
+                      double* rnorm2, double* omega, hipStream_t st) {
+  shifted_residual_t<double>(scratch, n, arp, aci, av, brp, bci, bv, bd, sigma, x, ldx, y, ldy, r, ldr, p, rnorm2,
+                             omega, st);
+}
+
+void gram(Scratch& scratch, int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w,
+          int64_t ldw, int64_t p, double* c, int64_t ldc, hipStream_t st) {
+  gram_t<double>(scratch, n, v, ldv, k, wd, w, ldw, p, c, ldc, st);
 }
 
 void column_dots(Scratch& scratch, int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
                  const double* wd, double* out, hipStream_t st) {
-  if (p <= 0) return;
-  const int groups = grid_for(n, 256, 512);
-  double* part = static_cast<double*>(scratch.get(sizeof(double) * (size_t)groups * p));
-  hipLaunchKernelGGL(k_column_dots, dim3(groups), dim3(256), 0, st, n, x, ldx, y, ldy, (int)p, wd, part);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(p), dim3(256), 0, st, groups, (int)p, part, out, 0);
-  check(hipGetLastError(), "column_dots");
+  column_dots_t<double>(scratch, n, x, ldx, y, ldy, p, wd, out, st);
+}
+
+void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                  int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st) {
+  block_update_t<double>(n, v, ldv, k, c, ldc, w, ldw, p, alpha, beta, st);
+}
+
+void zcsr_spmm(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* x, int64_t ldx,
+               double* y, int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st) {
+  csr_spmm_t<zval>(n, rp, ci, zc(v), zc(x), ldx, zm(y), ldy, p, alpha, beta, st);
+}
+
+void zdiag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                double alpha, double beta, hipStream_t st) {
+  diag_spmm_t<zval>(n, d, zc(x), ldx, zm(y), ldy, p, alpha, beta, st);
+}
+
+void zshifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const double* av,
+                       const int64_t* brp, const int32_t* bci, const double* bv, const double* bd, double sigma,
+                       const double* x, int64_t ldx, const double* y, int64_t ldy, double* r, int64_t ldr, int64_t p,
+                       double* rnorm2, double* omega, hipStream_t st) {
+  shifted_residual_t<zval>(scratch, n, arp, aci, zc(av), brp, bci, zc(bv), bd, sigma, zc(x), ldx, zc(y), ldy, zm(r),
+                           ldr, p, rnorm2, omega, st);
+}
+
+void zgram(Scratch& scratch, int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w,
+           int64_t ldw, int64_t p, double* c, int64_t ldc, hipStream_t st) {
+  gram_t<zval>(scratch, n, zc(v), ldv, k, wd, zc(w), ldw, p, zm(c), ldc, st);
+}
+
+void zcolumn_dots(Scratch& scratch, int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                  const double* wd, double* out, hipStream_t st) {
+  column_dots_t<zval>(scratch, n, zc(x), ldx, zc(y), ldy, p, wd, zm(out), st);
+}
+
+void zblock_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                   int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st) {
+  block_update_t<zval>(n, zc(v), ldv, k, zc(c), ldc, zm(w), ldw, p, alpha, beta, st);
 }
 
 void pair_sums(Scratch& scratch, int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
@@ -429,20 +579,6 @@ void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* 
   if (n <= 0 || p <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * p, 256, 16384)), dim3(256), 0, st, n, a, x, ldx, b, y, ldy, (int)p);
   check(hipGetLastError(), "axpby");
-}
-
-void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
-                  int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st) {
-  if (n <= 0 || p <= 0) return;
-  if (k <= 0) {
-    if (beta != 1.0) diag_spmm(n, nullptr, w, ldw, w, ldw, p, beta, 0.0, st);
-    return;
-  }
-  const size_t shm = sizeof(double) * (size_t)k * 16;
-  if (shm > 150 * 1024) throw std::runtime_error("block_update: k too large");
-  hipLaunchKernelGGL(k_block_update, dim3(grid_for(n, 256, 4096)), dim3(256), shm, st, n, v, ldv, (int)k, c, ldc, w, ldw,
-                     (int)p, alpha, beta);
-  check(hipGetLastError(), "block_update");
 }
 
 }  // namespace hsg

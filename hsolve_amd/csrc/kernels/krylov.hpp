@@ -41,4 +41,21 @@ void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* 
 void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
                   int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st);
 
+// Complex variants: every value, vector and C array holds interleaved (real, imaginary) doubles; the
+// diagonal weights d / wd stay real. gram and column_dots conjugate their first operand.
+void zcsr_spmm(int64_t n, const int64_t* rp, const int32_t* ci, const double* v, const double* x, int64_t ldx,
+               double* y, int64_t ldy, int64_t p, double alpha, double beta, hipStream_t st);
+void zdiag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                double alpha, double beta, hipStream_t st);
+void zshifted_residual(Scratch& scratch, int64_t n, const int64_t* arp, const int32_t* aci, const double* av,
+                       const int64_t* brp, const int32_t* bci, const double* bv, const double* bd, double sigma,
+                       const double* x, int64_t ldx, const double* y, int64_t ldy, double* r, int64_t ldr, int64_t p,
+                       double* rnorm2, double* omega, hipStream_t st);
+void zgram(Scratch& scratch, int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w,
+           int64_t ldw, int64_t p, double* c, int64_t ldc, hipStream_t st);
+void zcolumn_dots(Scratch& scratch, int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                  const double* wd, double* out, hipStream_t st);
+void zblock_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                   int64_t ldw, int64_t p, double alpha, double beta, hipStream_t st);
+
 }  // namespace hsg

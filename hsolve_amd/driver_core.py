@@ -122,9 +122,9 @@ def estimate_norm2(ops: DeviceOps, apply, steps: int = 30, seed: int = 2026) -> 
     n = ops.n
     steps = min(steps, n)
     rng = np.random.default_rng([seed, 0x6E6F726D])
-    v = torch.from_numpy(rng.uniform(-1.0, 1.0, (1, n))).to(ops.dev)
+    v = ops.block(rng.uniform(-1.0, 1.0, (1, n)))
     v.mul_(1.0 / math.sqrt(float(ops.dots(v, v).cpu()[0])))
-    basis = torch.empty((steps + 1, n), dtype=torch.float64, device=ops.dev)
+    basis = torch.empty((steps + 1, n), dtype=ops.dtype, device=ops.dev)
     basis[0:1].copy_(v)
     alpha, beta = [], []
     for j in range(steps):
@@ -156,7 +156,7 @@ class Block:
     def __init__(self, ops: DeviceOps, x: torch.Tensor | None = None, lam=None):
         self.ops = ops
         n = ops.n
-        self.x = x if x is not None else torch.empty((0, n), dtype=torch.float64, device=ops.dev)
+        self.x = x if x is not None else torch.empty((0, n), dtype=ops.dtype, device=ops.dev)
         self.lam = np.asarray(lam if lam is not None else [], dtype=np.float64)
         self.ax = None
         self.bx = None
@@ -189,7 +189,7 @@ class Block:
 
 def _replace_by_product(ops: DeviceOps, m: torch.Tensor, coef: np.ndarray) -> torch.Tensor:
     out = ops.empty(coef.shape[1])
-    ops.update(m, torch.from_numpy(np.ascontiguousarray(coef.T)).to(ops.dev), out, 1.0, 0.0)
+    ops.update(m, ops.coef(coef), out, 1.0, 0.0)
     return out
 
 
@@ -203,14 +203,14 @@ def rayleigh_ritz_definite(blk: Block, drop_tol: float = DEPENDENT_GRAM) -> int:
     ops = blk.ops
     g = ops.gram(blk.x, blk.ax).cpu().numpy().T
     h = ops.gram(blk.x, blk.bx).cpu().numpy().T
-    g = 0.5 * (g + g.T)
-    h = 0.5 * (h + h.T)
+    g = 0.5 * (g + g.conj().T)
+    h = 0.5 * (h + h.conj().T)
     mu, V = hostla.eigh(h)
     keep = [q for q in range(m) if mu[q] > drop_tol * mu[-1] and mu[q] > 0.0]
     r = len(keep)
     Y = V[:, keep] / np.sqrt(mu[keep])
-    gr = Y.T @ g @ Y
-    gr = 0.5 * (gr + gr.T)
+    gr = Y.conj().T @ g @ Y
+    gr = 0.5 * (gr + gr.conj().T)
     theta, Z = hostla.eigh(gr)
     coef = Y @ Z
     blk.x = _replace_by_product(ops, blk.x, coef)

@@ -83,17 +83,34 @@ class FactorStats:
 
 
 class Pencil:
-    """A and B (B = I when None) on the host and on the device, with the union pattern."""
+    """A and B (B = I when None) on the host and on the device, with the union pattern.
+
+    Complex Hermitian pencils (complex128 A or B) are factorized through their real embedding of
+    order 2n (the reference's MUMPS input, src/factor/mumps_input.cpp:43-150), here with the real
+    and imaginary parts of unknown i interleaved as rows 2i, 2i+1:
+        E(2i, 2j) = E(2i+1, 2j+1) = Re K(i, j),  E(2i, 2j+1) = -Im K(i, j),  E(2i+1, 2j) = Im K(i, j),
+    so that E acting on the real view of a complex block (re, im interleaved, as complex128 is
+    stored) is K acting on the block: solves run in place on that view. Every eigenvalue of K
+    appears twice in E. Positions whose real (or, off the diagonal, imaginary) part is zero in A
+    and in B are left out of E (alpha and beta are real, so they stay zero for every shift). The
+    values of E are gathered from the extended arrays [Re v, Im v, -Im v] of A's and B's values.
+    f_n, f_rowptr, f_col, f_a_index, f_b_index describe the pattern the factorization analyses
+    (the union pattern itself for a real pencil)."""
 
     def __init__(self, A, B=None):
-        A = _csr(A, "A").astype(np.float64)
+        A = _csr(A, "A")
+        Bc = None if B is None else _csr(B, "B")
+        cplx = bool(np.iscomplexobj(A.data) or (Bc is not None and np.iscomplexobj(Bc.data)))
+        dt = np.complex128 if cplx else np.float64
+        A = A.astype(dt)
         n = A.shape[0]
         self.n = n
+        self.is_complex = cplx
         self.b_is_identity = B is None
         if B is None:
-            Bm = sp.identity(n, format="csr", dtype=np.float64)
+            Bm = sp.identity(n, format="csr", dtype=dt)
         else:
-            Bm = _csr(B, "B").astype(np.float64)
+            Bm = Bc.astype(dt)
             if Bm.shape != A.shape:
                 raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {Bm.shape}")
         self.A = A
@@ -101,8 +118,14 @@ class Pencil:
         self._Bfull = Bm
         diag_b = Bm.nnz == n and np.array_equal(Bm.indices, np.arange(n)) and np.array_equal(
             Bm.indptr, np.arange(n + 1))
-        self.b_is_diagonal = bool(diag_b)
+        # a diagonal B of a Hermitian pencil is real (its imaginary parts are zero)
+        self.b_is_diagonal = bool(diag_b) and not (cplx and np.any(Bm.data.imag != 0.0))
         self._build_union(A, Bm)
+        if cplx:
+            self._build_embedding(A, Bm)
+        else:
+            self.f_n, self.f_rowptr, self.f_col = n, self.u_rowptr, self.u_col
+            self.f_a_index, self.f_b_index = self.a_index, self.b_index
         self._dev = None
 
     def _build_union(self, A, Bm):
@@ -122,27 +145,74 @@ class Pencil:
         self.a_index[np.searchsorted(keys, ka)] = np.arange(ka.size, dtype=np.int64)
         self.b_index[np.searchsorted(keys, kb)] = np.arange(kb.size, dtype=np.int64)
 
+    def _build_embedding(self, A, Bm):
+        n = self.n
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(self.u_rowptr))
+        cols = self.u_col.astype(np.int64)
+        ia, ib = self.a_index, self.b_index
+        na, nb = A.nnz, Bm.nnz
+
+        def part_nonzero(data, idx, f):
+            out = np.zeros(idx.size, dtype=bool)
+            has = idx >= 0
+            out[has] = f(data[idx[has]]) != 0.0
+            return out
+
+        diag = rows == cols
+        re_nz = diag | part_nonzero(A.data, ia, np.real) | part_nonzero(Bm.data, ib, np.real)
+        im_nz = ~diag & (part_nonzero(A.data, ia, np.imag) | part_nonzero(Bm.data, ib, np.imag))
+
+        def src(idx, offset):
+            return np.where(idx >= 0, idx + offset, -1)
+
+        r, c = rows[re_nz], cols[re_nz]
+        ra, rb = ia[re_nz], ib[re_nz]
+        q, d = rows[im_nz], cols[im_nz]
+        qa, qb = ia[im_nz], ib[im_nz]
+        er = np.concatenate((2 * r, 2 * r + 1, 2 * q, 2 * q + 1))
+        ec = np.concatenate((2 * c, 2 * c + 1, 2 * d + 1, 2 * d))
+        ea = np.concatenate((ra, ra, src(qa, 2 * na), src(qa, na)))      # Re, Re, -Im, +Im
+        eb = np.concatenate((rb, rb, src(qb, 2 * nb), src(qb, nb)))
+        n2 = 2 * n
+        key = er * n2 + ec
+        order = np.argsort(key, kind="stable")
+        er = er[order]
+        self.f_n = n2
+        self.f_rowptr = np.zeros(n2 + 1, dtype=np.int64)
+        np.add.at(self.f_rowptr, er + 1, 1)
+        np.cumsum(self.f_rowptr, out=self.f_rowptr)
+        self.f_col = ec[order].astype(np.int32)
+        self.f_a_index = np.ascontiguousarray(ea[order], dtype=np.int64)
+        self.f_b_index = np.ascontiguousarray(eb[order], dtype=np.int64)
+
     @property
     def nnz(self) -> int:
         return int(self.u_col.size)
 
     def to_device(self):
-        """Device copies: A (CSR), B (CSR values, and its diagonal when B is diagonal)."""
+        """Device copies: A (CSR), B (CSR values, and its diagonal when B is diagonal), and the value
+        arrays the factorization gathers from (f_a, f_b)."""
         if self._dev is None:
             dev = device()
             d = {
                 "a_rowptr": torch.from_numpy(self.A.indptr.astype(np.int64)).to(dev),
                 "a_col": torch.from_numpy(self.A.indices.astype(np.int32)).to(dev),
-                "a_val": torch.from_numpy(self.A.data.astype(np.float64)).to(dev),
-                "b_val": torch.from_numpy(self._Bfull.data.astype(np.float64)).to(dev),
+                "a_val": torch.from_numpy(self.A.data).to(dev),
+                "b_val": torch.from_numpy(self._Bfull.data).to(dev),
             }
             if self.b_is_diagonal:
-                d["b_diag"] = d["b_val"]
+                d["b_diag"] = torch.from_numpy(np.ascontiguousarray(self._Bfull.data.real)).to(dev)
                 d["b_rowptr"] = d["b_col"] = None
             else:
                 d["b_diag"] = None
                 d["b_rowptr"] = torch.from_numpy(self._Bfull.indptr.astype(np.int64)).to(dev)
                 d["b_col"] = torch.from_numpy(self._Bfull.indices.astype(np.int32)).to(dev)
+            if self.is_complex:
+                def ext(v):
+                    return torch.cat((v.real, v.imag, -v.imag)).contiguous()
+                d["f_a"], d["f_b"] = ext(d["a_val"]), ext(d["b_val"])
+            else:
+                d["f_a"], d["f_b"] = d["a_val"], d["b_val"]
             self._dev = d
         return self._dev
 
@@ -172,8 +242,8 @@ class GpuFactorization:
         t0 = time.perf_counter()
         h = C.c_void_p(0)
         order_ptr = host_ptr(self._given_order) if self._given_order is not None else C.c_void_p(0)
-        _lib.call("hsg_analyze", p.n, host_ptr(p.u_rowptr), host_ptr(p.u_col), host_ptr(p.a_index),
-                  host_ptr(p.b_index), order_ptr, C.byref(self._opts), C.byref(h))
+        _lib.call("hsg_analyze", p.f_n, host_ptr(p.f_rowptr), host_ptr(p.f_col), host_ptr(p.f_a_index),
+                  host_ptr(p.f_b_index), order_ptr, C.byref(self._opts), C.byref(h))
         self._handle = h
         self.stats.analyses += 1
         self.stats.analyze_s += time.perf_counter() - t0
@@ -186,7 +256,9 @@ class GpuFactorization:
         self.stats.max_front = int(info.max_front)
 
     def order(self) -> np.ndarray:
-        out = np.empty(self.pencil.n, dtype=np.int32)
+        """Elimination order (new -> old) of the analysed pattern (of the embedding, 2n entries, for a
+        complex pencil)."""
+        out = np.empty(self.pencil.f_n, dtype=np.int32)
         _lib.call("hsg_analysis_get_order", self._handle, host_ptr(out))
         return out
 
@@ -219,17 +291,25 @@ class GpuFactorization:
         inert = _lib.Inertia()
         sing = C.c_int32(0)
         t0 = time.perf_counter()
-        _lib.call("hsg_factorize", self._handle, float(alpha), float(beta), ptr(dev["a_val"]), ptr(dev["b_val"]),
+        _lib.call("hsg_factorize", self._handle, float(alpha), float(beta), ptr(dev["f_a"]), ptr(dev["f_b"]),
                   code, tol, stream_ptr(), C.byref(inert), C.byref(sing))
         self.stats.factorize_s += time.perf_counter() - t0
         self.stats.factorizations += 1
         self.stats.null_pivots = int(inert.zero)
         self.stats.two_by_two_pivots = int(inert.two_by_two)
-        self._inertia = Inertia(int(inert.neg), int(inert.zero), int(inert.pos), bool(inert.valid),
-                                int(inert.two_by_two))
+        neg, zero, pos, valid = int(inert.neg), int(inert.zero), int(inert.pos), bool(inert.valid)
+        if self.pencil.is_complex:
+            # mumps_factorization.cpp:367-380: every eigenvalue of K appears twice in E; an odd
+            # negative count means the two copies of one were split by the shift (K singular to
+            # working precision), treated like a null pivot
+            split = neg % 2 != 0
+            neg, zero = neg // 2, (zero + 1) // 2
+            pos = max(0, self.pencil.n - neg - zero)
+            valid = valid and not split
+        self._inertia = Inertia(neg, zero, pos, valid, int(inert.two_by_two))
         self.factored_precision = prec
-        self.sigma_held = (alpha, beta, prec, self.detection) if inert.valid else None
-        return bool(inert.valid)
+        self.sigma_held = (alpha, beta, prec, self.detection) if valid else None
+        return valid
 
     def inertia(self) -> Inertia:
         return self._inertia
@@ -241,17 +321,19 @@ class GpuFactorization:
         return self._inertia
 
     def solve(self, x: torch.Tensor) -> torch.Tensor:
-        """In place: x (p, n) float64 contiguous on the device, rows = right-hand sides."""
+        """In place: x (p, n) contiguous on the device (float64, or complex128 for a complex pencil,
+        solved through its real view with the embedding), rows = right-hand sides."""
         if not self._inertia.valid:
             raise RuntimeError("solve without a usable factorization")
         if x.dim() == 1:
             x2 = x.view(1, -1)
         else:
             x2 = x
-        if x2.dtype != torch.float64 or not x2.is_contiguous() or x2.shape[1] != self.pencil.n:
-            raise InvalidInputError("solve expects a contiguous float64 (p, n) device tensor")
+        want = torch.complex128 if self.pencil.is_complex else torch.float64
+        if x2.dtype != want or not x2.is_contiguous() or x2.shape[1] != self.pencil.n:
+            raise InvalidInputError(f"solve expects a contiguous {want} (p, n) device tensor")
         t0 = time.perf_counter()
-        _lib.call("hsg_solve", self._handle, ptr(x2), x2.shape[0], self.pencil.n, stream_ptr())
+        _lib.call("hsg_solve", self._handle, ptr(x2), x2.shape[0], self.pencil.f_n, stream_ptr())
         self.stats.solve_s += time.perf_counter() - t0
         self.stats.solves += 1
         self.stats.rhs_solved += int(x2.shape[0])

@@ -159,11 +159,12 @@ class BlockLanczos:
             self.exact_cycle = False
         self.ncv, self.keep, self.first_probe = ncv, keep, first_probe
         cols = self.d + ncv + self.p
-        self.V = torch.zeros((cols, n), dtype=torch.float64, device=ops.dev)
+        self.V = torch.zeros((cols, n), dtype=ops.dtype, device=ops.dev)
         if self.d:
             self.V[: self.d].copy_(deflation)
-        self.H = np.zeros((ncv, ncv))
-        self.rlast = np.zeros((self.p, self.p))
+        self.dt = ops.np_dtype  # projected matrices: real symmetric or complex Hermitian
+        self.H = np.zeros((ncv, ncv), dtype=self.dt)
+        self.rlast = np.zeros((self.p, self.p), dtype=self.dt)
         self.restarts = 0
         self.m = 0
         self.rr = None
@@ -175,15 +176,17 @@ class BlockLanczos:
 
     # ------------------------------------------------------------ helpers
     def _random(self, cols: int) -> torch.Tensor:
-        out = np.empty((cols, self.n))
+        out = np.empty((cols, self.n), dtype=self.dt)
         for c in range(cols):
             rng = np.random.default_rng([self.opt.seed, self.next_stream])
             out[c] = rng.uniform(-1.0, 1.0, self.n)
+            if self.ops.is_complex:
+                out[c] += 1j * rng.uniform(-1.0, 1.0, self.n)
             self.next_stream += 1
-        return torch.from_numpy(out).to(self.ops.dev)
+        return self.ops.block(out)
 
     def _project_out(self, k: int, w: torch.Tensor, coef: np.ndarray | None = None):
-        """w -= V[:k] (V[:k]^T B w), twice (CGS2); coefficients accumulated into coef (k x p)."""
+        """w -= V[:k] (V[:k]^H B w), twice (CGS2); coefficients accumulated into coef (k x p)."""
         if k == 0:
             return
         vk = self.V[:k]
@@ -198,13 +201,13 @@ class BlockLanczos:
         (coef k x p, R p x p). Breakdown columns are replaced by random vectors (coupling 0)."""
         ops = self.ops
         p = w.shape[0]
-        coef = np.zeros((k, p))
+        coef = np.zeros((k, p), dtype=self.dt)
         self._project_out(k, w, coef)
         G = ops.b_gram(w, w).cpu().numpy().T
-        G = 0.5 * (G + G.T)
+        G = 0.5 * (G + G.conj().T)
         ok = True
-        R = np.eye(p)
-        if np.all(np.diag(G) > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
+        R = np.eye(p, dtype=self.dt)
+        if np.all(np.diag(G).real > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
             for _ in range(2):
                 try:
                     Rk = hostla.cholesky_upper(G)
@@ -217,25 +220,25 @@ class BlockLanczos:
                     break
                 Rinv = hostla.inv(Rk)
                 tmp = ops.empty(p)
-                ops.update(w, torch.from_numpy(np.ascontiguousarray(Rinv.T)).to(ops.dev), tmp, 1.0, 0.0)
+                ops.update(w, ops.coef(Rinv), tmp, 1.0, 0.0)
                 w.copy_(tmp)
                 R = Rk @ R
                 G = ops.b_gram(w, w).cpu().numpy().T
-                G = 0.5 * (G + G.T)
+                G = 0.5 * (G + G.conj().T)
         else:
             ok = False
         if ok:
             return coef, R
         # column-by-column fallback (block_ortho.cpp column_fallback)
         self.fallbacks += 1
-        R = np.zeros((p, p))
+        R = np.zeros((p, p), dtype=self.dt)
         for c in range(p):
             col = w[c:c + 1]
-            extra = np.zeros((k, 1))
+            extra = np.zeros((k, 1), dtype=self.dt)
             self._project_out(k, col, extra)
             coef[:, c] += extra[:, 0]
             if c:
-                rc = np.zeros((c, 1))
+                rc = np.zeros((c, 1), dtype=self.dt)
                 blk = w[:c]
                 for _ in range(2):
                     cc = ops.b_gram(blk, col)
@@ -280,11 +283,11 @@ class BlockLanczos:
             dst.copy_(y)
             coef, R = self._orthonormalize(d + j + p, dst, BREAKDOWN_TOL, norms)
             A = coef[d + j:d + j + p, :]
-            A = 0.5 * (A + A.T)
+            A = 0.5 * (A + A.conj().T)
             self.H[j:j + p, j:j + p] = A
             if j + p < self.ncv:
                 self.H[j + p:j + 2 * p, j:j + p] = np.triu(R)
-                self.H[j:j + p, j + p:j + 2 * p] = np.triu(R).T
+                self.H[j:j + p, j + p:j + 2 * p] = np.triu(R).conj().T
             self.m = j + p
             self.rlast = np.triu(R)
             threshold = self.first_probe if self.restarts == 0 else self.keep + p
@@ -297,7 +300,7 @@ class BlockLanczos:
     def _rayleigh_ritz(self):
         m, p = self.m, self.p
         Hm = self.H[:m, :m]
-        Hm = 0.5 * (Hm + Hm.T)
+        Hm = 0.5 * (Hm + Hm.conj().T)
         theta, S = hostla.eigh(Hm)
         self.coupling = self.rlast @ S[m - p:m, :]
         est = np.linalg.norm(self.coupling, axis=0)
@@ -315,7 +318,7 @@ class BlockLanczos:
                 b[:, l] = 0.0
         vm = self.V[d:d + m]
         tmp = self.ops.empty(k)
-        self.ops.update(vm, torch.from_numpy(np.ascontiguousarray(sk.T)).to(self.ops.dev), tmp, 1.0, 0.0)
+        self.ops.update(vm, self.ops.coef(sk), tmp, 1.0, 0.0)
         resid = self.V[d + m:d + m + p].clone()
         self.V[d:d + k].copy_(tmp)
         self.V[d + k:d + k + p].copy_(resid)
@@ -323,7 +326,7 @@ class BlockLanczos:
         for l, i in enumerate(sel):
             self.H[l, l] = rr.theta[i]
             self.H[k:k + p, l] = b[:, l]
-            self.H[l, k:k + p] = b[:, l]
+            self.H[l, k:k + p] = b[:, l].conj()
 
     def run(self) -> EigResult:
         self.applies0 = getattr(self.op, "applications", 0)
@@ -349,8 +352,7 @@ class BlockLanczos:
         sel = [i for i in sel if rr.conv[i]] + [i for i in sel if not rr.conv[i]]
         ssel = rr.s[:, sel]
         vec = self.ops.empty(nev)
-        self.ops.update(self.V[self.d:self.d + m], torch.from_numpy(np.ascontiguousarray(ssel.T)).to(self.ops.dev),
-                        vec, 1.0, 0.0)
+        self.ops.update(self.V[self.d:self.d + m], self.ops.coef(ssel), vec, 1.0, 0.0)
         nconv = int(np.count_nonzero(rr.conv[sel]))
         return EigResult(theta=rr.theta[sel].copy(), estimate=rr.est[sel].copy(), vectors=vec,
                          ritz_all=rr.theta.copy(), nconv=nconv, converged=nconv >= nev, restarts=self.restarts,

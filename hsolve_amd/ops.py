@@ -1,8 +1,10 @@
 """Device kernels of the operator and Krylov layers, and the shift-invert operator.
 
-Blocks of vectors are torch float64 tensors of shape (p, n) on the GPU (row c = column c of the
-column-major n x p block of the C ABI). Every arithmetic kernel is a libhsg entry point; torch
-only allocates, copies and indexes.
+Blocks of vectors are torch tensors of shape (p, n) on the GPU (row c = column c of the
+column-major n x p block of the C ABI), float64 for a real pencil and complex128 for a complex
+Hermitian one (the hsg_z* kernels; kernels without a complex variant run on the real view, 2n
+doubles per row). Every arithmetic kernel is a libhsg entry point; torch only allocates, copies and
+indexes.
 
 ShiftInvertOperator restates src/ops/shift_invert_operator.cpp:83-177: Y = K^-1 B X with
 K = A - sigma B factorized by GpuFactorization; with an inexact (single-precision) factor every
@@ -34,13 +36,29 @@ class DeviceOps:
         self.n = pencil.n
         self.dev = device()
         self.d = pencil.to_device()
+        self.is_complex = pencil.is_complex
+        self.dtype = torch.complex128 if self.is_complex else torch.float64
+        self.np_dtype = np.complex128 if self.is_complex else np.float64
+        self._k = "hsg_z" if self.is_complex else "hsg_"
 
     # --------------------------------------------------------------- helpers
     def empty(self, p: int) -> torch.Tensor:
-        return torch.empty((p, self.n), dtype=torch.float64, device=self.dev)
+        return torch.empty((p, self.n), dtype=self.dtype, device=self.dev)
 
     def zeros(self, p: int) -> torch.Tensor:
-        return torch.zeros((p, self.n), dtype=torch.float64, device=self.dev)
+        return torch.zeros((p, self.n), dtype=self.dtype, device=self.dev)
+
+    def block(self, a: np.ndarray) -> torch.Tensor:
+        """(p, n) device block of this pencil's scalar type from a host array."""
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=self.np_dtype)).to(self.dev)
+
+    def coef(self, m: np.ndarray) -> torch.Tensor:
+        """A host k x p coefficient matrix as the (p, k) device tensor update() takes."""
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(m).T, dtype=self.np_dtype)).to(self.dev)
+
+    def _rows(self) -> int:
+        """Doubles per row of a block (the real view of a complex block has 2n)."""
+        return 2 * self.n if self.is_complex else self.n
 
     # --------------------------------------------------------------- products
     def apply_a(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha=1.0, beta=0.0) -> torch.Tensor:
@@ -48,8 +66,8 @@ class DeviceOps:
         if y is None:
             y = self.empty(p)
         d = self.d
-        _lib.call("hsg_csr_spmm", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]), ptr(x), self.n,
-                  ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
+        _lib.call(self._k + "csr_spmm", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]), ptr(x),
+                  self.n, ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
         return y
 
     def apply_b(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha=1.0, beta=0.0) -> torch.Tensor:
@@ -59,11 +77,11 @@ class DeviceOps:
         d = self.d
         if self.pencil.b_is_diagonal:
             # B = I (b_diag holds ones) or a diagonal mass matrix
-            _lib.call("hsg_diag_spmm", self.n, ptr(d["b_diag"]), ptr(x), self.n, ptr(y), self.n, p, float(alpha),
-                      float(beta), stream_ptr())
+            _lib.call(self._k + "diag_spmm", self.n, ptr(d["b_diag"]), ptr(x), self.n, ptr(y), self.n, p,
+                      float(alpha), float(beta), stream_ptr())
         else:
-            _lib.call("hsg_csr_spmm", self.n, ptr(d["b_rowptr"]), ptr(d["b_col"]), ptr(d["b_val"]), ptr(x), self.n,
-                      ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
+            _lib.call(self._k + "csr_spmm", self.n, ptr(d["b_rowptr"]), ptr(d["b_col"]), ptr(d["b_val"]), ptr(x),
+                      self.n, ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
         return y
 
     def b_weight(self):
@@ -76,16 +94,16 @@ class DeviceOps:
 
     # --------------------------------------------------------------- dense
     def gram(self, v: torch.Tensor, w: torch.Tensor, weight=None) -> torch.Tensor:
-        """C = V^T diag(weight) W as a (p, k) tensor (column-major k x p)."""
+        """C = V^H diag(weight) W as a (p, k) tensor (column-major k x p)."""
         k, p = v.shape[0], w.shape[0]
-        c = torch.empty((p, k), dtype=torch.float64, device=self.dev)
+        c = torch.empty((p, k), dtype=self.dtype, device=self.dev)
         if k == 0 or p == 0:
             return c
-        _lib.call("hsg_gram", self.n, ptr(v), self.n, k, ptr(weight), ptr(w), self.n, p, ptr(c), k, stream_ptr())
+        _lib.call(self._k + "gram", self.n, ptr(v), self.n, k, ptr(weight), ptr(w), self.n, p, ptr(c), k, stream_ptr())
         return c
 
     def b_gram(self, v: torch.Tensor, w: torch.Tensor, bw: torch.Tensor | None = None) -> torch.Tensor:
-        """V^T B W; bw = B W when already formed (general B)."""
+        """V^H B W; bw = B W when already formed (general B)."""
         if self.pencil.b_is_diagonal:
             return self.gram(v, w, self.b_weight())
         if bw is None:
@@ -97,17 +115,19 @@ class DeviceOps:
         k, p = v.shape[0], w.shape[0]
         if p == 0:
             return w
-        _lib.call("hsg_block_update", self.n, ptr(v), self.n, k, ptr(c), k, ptr(w), self.n, p, float(alpha),
+        _lib.call(self._k + "block_update", self.n, ptr(v), self.n, k, ptr(c), k, ptr(w), self.n, p, float(alpha),
                   float(beta), stream_ptr())
         return w
 
     def dots(self, x: torch.Tensor, y: torch.Tensor, weight=None) -> torch.Tensor:
+        """x_c^H diag(weight) y_c per row pair; real parts (the values the driver uses: norms and the
+        Rayleigh quotients of Hermitian pencils, whose imaginary parts vanish)."""
         p = x.shape[0]
-        out = torch.empty(p, dtype=torch.float64, device=self.dev)
+        out = torch.empty(p, dtype=self.dtype, device=self.dev)
         if p:
-            _lib.call("hsg_column_dots", self.n, ptr(x), self.n, ptr(y), self.n, p, ptr(weight), ptr(out),
+            _lib.call(self._k + "column_dots", self.n, ptr(x), self.n, ptr(y), self.n, p, ptr(weight), ptr(out),
                       stream_ptr())
-        return out
+        return out.real if self.is_complex else out
 
     def b_dots(self, x, y, by=None):
         if self.pencil.b_is_diagonal:
@@ -123,13 +143,15 @@ class DeviceOps:
             return np.zeros((0, 3))
         ld = torch.from_numpy(np.ascontiguousarray(lam, dtype=np.float64)).to(self.dev)
         out = torch.empty(3 * p, dtype=torch.float64, device=self.dev)
-        _lib.call("hsg_pair_sums", self.n, ptr(ax), ptr(bx), ptr(x), self.n, p, ptr(ld), ptr(out), stream_ptr())
+        nr = self._rows()  # real lambda: the sums of a complex block are those of its real view
+        _lib.call("hsg_pair_sums", nr, ptr(ax), ptr(bx), ptr(x), nr, p, ptr(ld), ptr(out), stream_ptr())
         return out.cpu().numpy().reshape(p, 3)
 
     def axpby(self, a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
         p = x.shape[0]
         if p:
-            _lib.call("hsg_axpby", self.n, float(a), ptr(x), self.n, float(b), ptr(y), self.n, p, stream_ptr())
+            nr = self._rows()  # real a, b: a complex block is updated through its real view
+            _lib.call("hsg_axpby", nr, float(a), ptr(x), nr, float(b), ptr(y), nr, p, stream_ptr())
         return y
 
     def shifted_residual(self, x: torch.Tensor, y: torch.Tensor, sigma: float, r: torch.Tensor | None = None):
@@ -140,7 +162,7 @@ class DeviceOps:
         rn = torch.empty(p, dtype=torch.float64, device=self.dev)
         om = torch.empty(p, dtype=torch.float64, device=self.dev)
         d = self.d
-        _lib.call("hsg_shifted_residual", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]),
+        _lib.call(self._k + "shifted_residual", self.n, ptr(d["a_rowptr"]), ptr(d["a_col"]), ptr(d["a_val"]),
                   ptr(d["b_rowptr"]), ptr(d["b_col"]), ptr(d["b_val"]), ptr(d["b_diag"]), float(sigma), ptr(x),
                   self.n, ptr(y), self.n, ptr(r), self.n, p, ptr(rn), ptr(om), stream_ptr())
         return r, rn, om

@@ -41,20 +41,17 @@ def hermitian_input(M, name: str, tol: float):
         raise InvalidInputError(f"{name} must be square, got {M.shape}")
     if M.nnz and not np.all(np.isfinite(M.data)):
         raise InvalidInputError(f"{name} has non-finite entries")
-    if np.iscomplexobj(M.data):
-        if M.nnz and np.all(M.data.imag == 0):
-            M = sp.csr_matrix(M.real)
-        else:
-            raise UnsupportedError(f"{name}: complex Hermitian pencils are not supported by this engine yet")
-    M = M.astype(np.float64)
+    # complex input keeps complex scalars (the reference's complex template), real input stays real
+    M = M.astype(np.complex128 if np.iscomplexobj(M.data) else np.float64)
     fro = sp.linalg.norm(M) if M.nnz else 0.0
     if fro == 0.0:
         raise InvalidInputError(f"{name} is zero")
-    D = (M - M.T).tocsr()
+    D = (M - M.conj().T).tocsr()
+    D.eliminate_zeros()
     defect = (sp.linalg.norm(D) / fro) if D.nnz else 0.0
     if defect > tol:
         raise NotHermitianError(f"{name} is not Hermitian: ||M - M^H||_F / ||M||_F = {defect:.3g} > {tol:.3g}")
-    H = ((M + M.T) * 0.5).tocsr()
+    H = ((M + M.conj().T) * 0.5).tocsr()
     H.sum_duplicates()
     H.eliminate_zeros()
     H.sort_indices()
@@ -99,7 +96,7 @@ def prepare(A, B=None, config: dict | None = None) -> Prepared:
             raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {B.shape}")
         if bk == "diagonal":
             B = sp.diags(B.diagonal(), format="csr")
-    if B is not None and bk == "diagonal" and np.any(B.diagonal() <= 0):
+    if B is not None and bk == "diagonal" and np.any(B.diagonal().real <= 0):
         raise UnsupportedError("B is diagonal but not positive definite (GHIEP): not supported by this engine")
     pencil = Pencil(A, B)
     pencil.to_device()
@@ -123,7 +120,8 @@ def slice_streams(fac: GpuFactorization, requested: int | None) -> int:
 def solve(A=None, B=None, config: dict | None = None, *, prepared: Prepared | None = None, vectors: str = "host",
           streams: int | None = None):
     """Returns (eigenvalues ndarray, eigenvectors, report dict, config dict). The eigenvectors are an
-    (n, m) Fortran-order ndarray (vectors="host") or an (m, n) float64 tensor on the device whose
+    (n, m) Fortran-order ndarray (vectors="host") or an (m, n) tensor on the device (float64, complex128
+    for a complex pencil) whose
     row j is eigenvector j (vectors="device"). `prepared` (from prepare() on the same matrices)
     skips the input stage and the uploads. `streams`: concurrent streams for the spectrum slices
     of a window target (default: automatic, see slice_streams; 1 = one slice after the other)."""
@@ -171,6 +169,7 @@ def _solve(A, B, config, prepared, vectors, streams):
     rep["scale"] = scale
     rep["problem"] = "standard" if B is None else "generalized_definite"
     rep["method"] = "shift_invert_lanczos"
+    rep["complex_scalars"] = bool(pencil.is_complex)
     clock.pop()
     with clock.phase("factor"):
         fac = GpuFactorization(pencil)
@@ -206,7 +205,7 @@ def _solve(A, B, config, prepared, vectors, streams):
             ctx.warn(f"single-precision attempt abandoned ({e}); solving again with double factors")
             keep = {k: rep[k] for k in ("warnings", "precision", "hermitian_defect_a", "hermitian_defect_b", "n",
                                         "nnz_a", "nnz_b", "norm2_a", "norm2_b", "scale", "problem", "method",
-                                        "sigma_requested")}
+                                        "sigma_requested", "complex_scalars")}
             rep.clear()
             rep.update(cfgmod.new_report())
             rep.update(keep)
@@ -221,9 +220,10 @@ def _solve(A, B, config, prepared, vectors, streams):
     with clock.phase("verify"):
         lam = np.asarray(sel.lam, dtype=np.float64)
         if vectors == "device":
-            vecs = sel.x.detach() if sel.size else torch.zeros((0, n), dtype=torch.float64, device=ops.dev)
+            vecs = sel.x.detach() if sel.size else torch.zeros((0, n), dtype=ops.dtype, device=ops.dev)
         else:
-            vecs = sel.x.detach().cpu().numpy().T.copy(order="F") if sel.size else np.zeros((n, 0), order="F")
+            vecs = (sel.x.detach().cpu().numpy().T.copy(order="F") if sel.size else
+                    np.zeros((n, 0), dtype=ops.np_dtype, order="F"))
     all_ok = ok
     for j, m in enumerate(metrics):
         acc = m.backward_error <= cfg["tol"]

@@ -17,7 +17,10 @@ without the zero band (zero_filter_tol < 0, the default) and without helper proc
 The reference hands slices to helper processes (definite.cpp:1586-1837); here independent slices
 run at the same time on separate streams of the one device, each on its own factorization object
 over the shared analysis, and their results are merged in slice order (so they do not depend on
-which slice finished first).
+which slice finished first). With more than one stream the two end counts of an interval run at
+the same time, and so do counts at the cuts plan() is expected to choose (linear interpolation of
+the end counts); plan() then finds them remembered. Every count is exact, so the certificate is
+the same; only the cut positions may differ from the one-stream sequence.
 """
 from __future__ import annotations
 
@@ -89,6 +92,7 @@ class WindowSolver:
         self._tls = threading.local()      # a slice worker's own context
         self._lock = threading.Lock()      # the count cache
         self.streams = max(1, int(streams))
+        self._slot_list = []
         self.n = ctx.pencil.n
         self.scale = ctx.scale
         self.known: list[CountPoint] = []
@@ -449,67 +453,94 @@ class WindowSolver:
             self.applies += self.reports[sid]["op_applies"]
 
     def run_slices_concurrently(self, workers: int):
-        """Slices handed out in order to `workers` threads, each with its own stream and its own
-        factorization object (worker 0 keeps the solve's). Each slice runs in a context of its own
-        (report fields, events, held factorization); those are merged in slice order afterwards. The
-        waiting is charged to the iterate phase."""
+        """Slices handed out in order to the worker slots; the block size of slice 0 is the solve's."""
+        ctxs = self._concurrently(len(self.slices), lambda sid: self.run_slice(self.slices[sid], sid), "iterate")
         main = self._main
+        if ctxs and ctxs[0] is not None:
+            main.rep["block_size"] = ctxs[0].rep["block_size"]
+        par = main.rep["parallel"]
+        par["helper_slices"] = len(self.slices)
+        par["notes"].append(f"{len(self.slices)} spectrum slices on {workers} concurrent streams of one device")
+
+    # ------------------------------------------------------------------ workers
+    def _slots(self, k: int):
+        """k worker slots {fac, stream, held}. Slot 0 is the solve's own factorization object; the
+        others are clones over the same analysis, made once per solve and reused by every
+        concurrent phase (counts, slices), each with a stream of its own."""
+        main = self._main
+        if not self._slot_list:
+            self._slot_list.append({"fac": main.fac, "stream": torch.cuda.Stream(), "held": FactorHeld()})
+        while len(self._slot_list) < k:
+            self._slot_list.append({"fac": main.fac.clone(), "stream": torch.cuda.Stream(), "held": FactorHeld()})
+        return self._slot_list[:k]
+
+    def _concurrently(self, njobs: int, job, phase: str):
+        """Runs job(i) for i < njobs on min(njobs, streams) threads, one worker slot each, jobs handed
+        out in order. Every job runs in a context of its own (self.c inside the job: the slot's
+        factorization, a fresh report and event list); the jobs' events, warnings and certificate
+        fields are merged into the solve's in job order afterwards, so the outcome does not depend on
+        which job finished first. The first failure in job order is re-raised. The waiting is charged
+        to `phase` of the solve's clock. Returns the job contexts."""
+        main = self._main
+        workers = max(1, min(njobs, self.streams))
+        slots = self._slots(workers)
+        slots[0]["held"] = main.held
         torch.cuda.synchronize()
-        facs = [main.fac] + [main.fac.clone() for _ in range(workers - 1)]
-        streams = [torch.cuda.Stream() for _ in range(workers)]
-        held = [main.held] + [FactorHeld() for _ in range(workers - 1)]
-        ns = len(self.slices)
-        ctxs = [None] * ns
-        errors = [None] * ns
+        ctxs = [None] * njobs
+        errors = [None] * njobs
         nxt = [0]
         order_lock = threading.Lock()
 
         def worker(w: int):
-            with torch.cuda.stream(streams[w]):
+            slot = slots[w]
+            with torch.cuda.stream(slot["stream"]):
                 while True:
                     with order_lock:
-                        sid = nxt[0]
+                        i = nxt[0]
                         nxt[0] += 1
                         if any(e is not None for e in errors):
-                            sid = ns  # a slice failed: hand out nothing more
-                    if sid >= ns:
+                            i = njobs  # a job failed: hand out nothing more
+                    if i >= njobs:
                         break
                     rep = cfgmod.new_report()
                     rep["certificate"]["note"] = ""
-                    ctx = dataclasses.replace(main, fac=facs[w], held=held[w], clock=PhaseClock(), events=[],
+                    ctx = dataclasses.replace(main, fac=slot["fac"], held=slot["held"], clock=PhaseClock(), events=[],
                                               rep=rep, certified=[])
-                    ctxs[sid] = ctx
+                    ctxs[i] = ctx
                     self._tls.ctx = ctx
                     try:
-                        self.run_slice(self.slices[sid], sid)
-                    except BaseException as e:  # re-raised in slice order below
-                        errors[sid] = e
+                        job(i)
+                    except BaseException as e:  # re-raised in job order below
+                        errors[i] = e
                     finally:
-                        held[w] = ctx.held
+                        slot["held"] = ctx.held
                         self._tls.ctx = None
-                    if errors[sid] is not None:
+                    if errors[i] is not None:
                         break
-                streams[w].synchronize()
+                slot["stream"].synchronize()
 
-        with main.clock.phase("iterate"):
-            threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-slice-{w}") for w in range(workers)]
+        with main.clock.phase(phase):
+            threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-worker-{w}") for w in range(workers)]
             for t in threads:
                 t.start()
             for t in threads:
                 t.join()
         torch.cuda.synchronize()
-        main.held = held[0]
-        for f in facs[1:]:
-            st, ms = f.stats, main.fac.stats
+        main.held = slots[0]["held"]
+        ms = main.fac.stats
+        for slot in slots[1:]:
+            st = slot["fac"].stats
             ms.factorizations += st.factorizations
             ms.solves += st.solves
             ms.rhs_solved += st.rhs_solved
             ms.factorize_s += st.factorize_s
             ms.solve_s += st.solve_s
-        for sid in range(ns):
-            if errors[sid] is not None:
-                raise errors[sid]
-            ctx = ctxs[sid]
+            st.factorizations = st.solves = st.rhs_solved = 0
+            st.factorize_s = st.solve_s = 0.0
+        for i in range(njobs):
+            if errors[i] is not None:
+                raise errors[i]
+            ctx = ctxs[i]
             if ctx is None:
                 continue
             r = ctx.rep
@@ -519,11 +550,34 @@ class WindowSolver:
             main.rep["certificate"]["endpoint_moves"] += r["certificate"]["endpoint_moves"]
             if r["certificate"]["note"] and not main.rep["certificate"]["note"]:
                 main.rep["certificate"]["note"] = r["certificate"]["note"]
-            if sid == 0:
-                main.rep["block_size"] = r["block_size"]
-        par = main.rep["parallel"]
-        par["helper_slices"] = ns
-        par["notes"].append(f"{ns} spectrum slices on {workers} concurrent streams of one device")
+        return ctxs
+
+    def count_all(self, points):
+        """Counts at several (shift, outward) points; with more than one stream the ones not yet
+        known are counted at the same time on the worker slots (one FP64 factorization each)."""
+        with self._lock:
+            known = {p.s for p in self.known}
+        todo = list(dict.fromkeys(s for s, _ in points if s not in known))
+        if self.streams > 1 and len(todo) > 1:
+            side = {s: o for s, o in points}
+            self._concurrently(len(todo), lambda i: self.count(todo[i], side[todo[i]]), "certificate")
+        return [self.count(s, o) for s, o in points]
+
+    def speculative_cuts(self, w: Slice, c_target: int, c_max: int):
+        """The shifts plan() is likely to count at first: its sequence of eigenvalue-count targets
+        over w, each placed by linear interpolation between the counts at w's ends (the guess plan()
+        itself starts from). Counting them ahead, at the same time, leaves plan() mostly with cache
+        hits; where a guess misses, plan() searches on from the known points as usual."""
+        a, b = w.lo, w.hi
+        if w.want >= 0 or not (math.isfinite(a.s) and math.isfinite(b.s)) or b.nu <= a.nu:
+            return []
+        out = []
+        cur = a.nu
+        while b.nu - cur > c_max:
+            remaining = b.nu - cur
+            cur += (remaining + 1) // 2 if remaining <= 2 * c_target else c_target
+            out.append(a.s + (b.s - a.s) * (cur - a.nu) / (b.nu - a.nu))
+        return out
 
     def deduplicate(self) -> int:
         """definite.cpp:1522-1556: groups within 2e-9 relative holding pairs of several slices are
@@ -638,8 +692,7 @@ class WindowSolver:
         cert = rep["certificate"]
         k = t["k"]
         if mode == "interval":
-            lo = self.count(t["lower"], -1.0)
-            hi = self.count(t["upper"], +1.0)
+            lo, hi = self.count_all([(t["lower"], -1.0), (t["upper"], +1.0)])
             windows.append(Slice(lo, hi))
             cert["lower"], cert["upper"] = lo.s, hi.s
         elif mode == "smallest":
@@ -677,6 +730,8 @@ class WindowSolver:
             cert["note"] = cert["note"] or "window is empty"
             empty = Block(c.ops, None, [])
             return empty, [], mode == "interval"
+        if self.streams > 1:
+            self.count_all([(x, +1.0) for w in windows for x in self.speculative_cuts(w, c_target, c_max)])
         for w in windows:
             self.slices.extend(self.plan(w, c_target, c_max))
         c.event("slices", slices=len(self.slices))

@@ -198,6 +198,31 @@ int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x
 int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
               void* stream);
 
+/* Complex Hermitian pencils: the same kernels for T = std::complex<double> (the reference's
+ * templates above instantiated for cplx = std::complex<double>, include/hsolve/types.hpp:14).
+ * Every matrix value, vector and
+ * C array holds interleaved (real, imaginary) doubles, leading dimensions count complex entries,
+ * alpha / beta / sigma and the diagonal weights stay real. hsg_zgram forms C = V^H diag(w) W and
+ * hsg_zcolumn_dots out[c] = x_c^H diag(w) y_c (p complex values); hsg_zshifted_residual uses
+ * complex moduli in the componentwise quotient. hsg_pair_sums and hsg_axpby serve complex blocks
+ * unchanged through their real view (2n doubles per column). */
+int hsg_zcsr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values,
+                  const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta,
+                  void* stream);
+int hsg_zdiag_spmm(int64_t n, const double* d, const double* x, int64_t ldx, double* y, int64_t ldy, int64_t p,
+                   double alpha, double beta, void* stream);
+int hsg_zshifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_colind, const double* a_values,
+                          const int64_t* b_rowptr, const int32_t* b_colind, const double* b_values,
+                          const double* b_diag, double sigma, const double* x, int64_t ldx, const double* y,
+                          int64_t ldy, double* r, int64_t ldr, int64_t p, double* rnorm2, double* omega,
+                          void* stream);
+int hsg_zgram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w,
+              int64_t ldw, int64_t p, double* c, int64_t ldc, void* stream);
+int hsg_zblock_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
+                      int64_t ldw, int64_t p, double alpha, double beta, void* stream);
+int hsg_zcolumn_dots(int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
+                     const double* w_diag, double* out, void* stream);
+
 /* ---------------------------------------------------------------- measurement
  * No counterpart in the reference (its timings are host wall-clock phases, SolveReport time_*_s,
  * include/hsolve/solver.hpp). Phase timing of factorizations and solves with HIP events recorded

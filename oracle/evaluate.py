@@ -122,7 +122,7 @@ def evaluate(A, B, lam, X, target: dict, ref: np.ndarray | None, tol: float = 1e
     pm = pair_metrics(A, Bm, lam, X, n2a, n2b, tol)
     max_eta = max((p["backward_error"] for p in pm), default=0.0)
     BX = X if Bm is None else Bm @ X
-    G = np.asarray(X).T @ np.asarray(BX)
+    G = np.asarray(X).conj().T @ np.asarray(BX)
     orth = float(np.max(np.abs(G - np.eye(G.shape[0])))) if G.size else 0.0
     out = {"max_eta": max_eta, "all_accepted": all(p["accepted"] for p in pm), "orthogonality": orth,
            "missing": [], "deficit": 0, "spurious": [], "distinct": int(lam.size)}

@@ -119,18 +119,23 @@ def test_report_layout_and_phases(gpu):
 
 
 def test_concurrent_slices_match_sequential(gpu):
-    # the slices of a window solved on three streams at once give the same pairs as one after the
-    # other (each slice's computation does not depend on the others or on the order they finish)
+    # the slices of a window solved on three streams at once: the result does not depend on which
+    # slice or count finishes first (two concurrent runs agree bitwise), and it holds the same
+    # eigenvalues as the one-stream solve (whose cuts may differ: concurrent runs also count ahead
+    # at the cuts the planner is expected to choose)
     from hsolve_amd import solver
     A, _, g = yee.cube_cavity(10)
     ref = yee_exact.pec_box_spectrum(g.cells, g.lengths)
     cfg = {"target": {"mode": "interval", "lower": 5.0, "upper": 40.0}, "tol": 1e-10, "slice_size": 24}
     l1, v1, r1, _ = solver.solve(A, None, cfg, streams=1)
     l3, v3, r3, _ = solver.solve(A, None, cfg, streams=3)
-    print("slices", len(r1["slices"]), "pairs", l1.size, r3["parallel"]["notes"], flush=True)
-    assert len(r1["slices"]) >= 3 and len(r3["slices"]) == len(r1["slices"])
+    l3b, v3b, r3b, _ = solver.solve(A, None, cfg, streams=3)
+    print("slices", len(r1["slices"]), len(r3["slices"]), "pairs", l1.size, r3["parallel"]["notes"], flush=True)
+    assert len(r1["slices"]) >= 3 and len(r3["slices"]) >= 3
     assert r1["certificate"]["passed"] and r3["certificate"]["passed"]
-    assert np.array_equal(l1, l3)
-    assert r3["factor_stats"]["factorizations"] == r1["factor_stats"]["factorizations"]
+    assert np.array_equal(l3, l3b) and np.array_equal(v3, v3b)
+    assert [s["lower"] for s in r3["slices"]] == [s["lower"] for s in r3b["slices"]]
+    assert l3.size == l1.size
+    np.testing.assert_allclose(l3, l1, rtol=1e-9, atol=0)
     out = ev.evaluate(A, None, l3, v3, cfg["target"], ref, ref_complete=True)
     assert out["passed"], out
