@@ -19,7 +19,7 @@ sweep's transposed products, timed with HIP events on its stream during the time
 the CPU baseline (the oracle's SuperLU elimination of the same pencil in the same fill-reducing
 order plus one 8-RHS solve, scaled by the operation counts of a full solve).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-plain | --plain-only]
 """
 from __future__ import annotations
 
@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="run the plain form alone (for counter runs of the kernels the roofline is taken on)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES bytes (the reference kernel that "
                          "hardware byte counters are scaled by)")
@@ -203,12 +205,15 @@ def main():
 
     # chosen: the engine's own choices (working precision, concurrent slice streams);
     # plain: FP64 factors, one slice after the other
-    chosen = run_form("chosen", prepared, solver_cfg(chosen_cfg), args.steps, args.warmup, check, None)
+    if args.plain_only:
+        chosen_cfg, args.no_plain = plain_cfg, False
+    chosen = run_form("plain" if args.plain_only else "chosen", prepared, solver_cfg(chosen_cfg), args.steps,
+                      args.warmup, check, 1 if args.plain_only else None)
     cs = summary(chosen)
     rep = chosen["reps"][-1]
     same_form = rep["precision"]["used"] == "double"
     plain, plain_s = None, None
-    if not args.no_plain:
+    if not args.no_plain and not args.plain_only:
         plain = run_form("plain", prepared, solver_cfg(plain_cfg), args.steps, args.warmup, check, 1)
         plain_s = summary(plain)
 
@@ -253,7 +258,7 @@ def main():
         "config": {"workload": "box48: PEC box GHEP, 48x24x36 Yee cells, n=117036, jittered diagonal B (seed 2026), "
                                f"all {COUNT} eigenvalues in [40, b] by inertia bisection",
                    "n": int(A.shape[0]), "lower": LOWER, "upper": upper, "pairs": COUNT, "tol": TOL,
-                   "form": "chosen", "precision_used": rep["precision"]["used"],
+                   "form": "plain" if args.plain_only else "chosen", "precision_used": rep["precision"]["used"],
                    "slice_streams": rep["parallel"]["notes"][-1] if rep["parallel"]["notes"] else "1 stream"},
         "latency_p50_ms": cs["latency_p50_ms"], "latency_p99_ms": cs["latency_p99_ms"],
         "certified_per_step": cs["certified_per_step"],

@@ -42,23 +42,17 @@ void permuted_lower(int32_t n, const Graph& g, const std::vector<int32_t>& order
   }
 }
 
-// Elimination tree from the lower CSC pattern: for each row i, walk from every column j < i with
-// L(i, j) structurally present up to its current root (path halving) and hang it below i.
-std::vector<int32_t> etree_lower(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& ri) {
-  // Row-wise access to the upper part (= columns j of row i with j < i): transpose.
-  std::vector<int64_t> rp(static_cast<size_t>(n) + 1, 0);
-  for (int64_t p = 0; p < cp[n]; ++p) ++rp[ri[p] + 1];
-  for (int32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
-  std::vector<int32_t> cj(static_cast<size_t>(cp[n]));
-  {
-    std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
-    for (int32_t j = 0; j < n; ++j)
-      for (int64_t p = cp[j]; p < cp[j + 1]; ++p) cj[fill[ri[p]]++] = j;
-  }
+// Elimination tree of the pattern in the new numbering (Liu's algorithm with path halving): row
+// i's entries left of the diagonal are the neighbours of order[i] numbered before i. The algorithm
+// gives the elimination tree whatever order a row's columns are visited in, so no sorted lower
+// pattern is needed.
+std::vector<int32_t> etree_from_graph(int32_t n, const Graph& g, const std::vector<int32_t>& order,
+                                      const std::vector<int32_t>& iorder) {
   std::vector<int32_t> parent(n, -1), anc(n, -1);
   for (int32_t i = 0; i < n; ++i) {
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
-      int32_t j = cj[p];
+    const int32_t v = order[i];
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+      int32_t j = iorder[g.adj[p]];
       while (j != -1 && j < i) {
         const int32_t next = anc[j];
         anc[j] = i;
@@ -155,20 +149,25 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
       throw std::runtime_error("analyze_pattern: invalid ordering");
     iorder[order[k]] = k;
   }
+  std::vector<int32_t> parent = etree_from_graph(n, g, order, iorder);
+  // postorder and compose; a postorder is a topological order of the tree, so the tree of the
+  // postordered pattern is the same tree relabelled (no second elimination-tree pass)
+  {
+    const std::vector<int32_t> post = postorder(n, parent);
+    std::vector<int32_t> ipost(n), order2(n), parent2(n);
+    for (int32_t k = 0; k < n; ++k) ipost[post[k]] = k;
+    for (int32_t k = 0; k < n; ++k) {
+      order2[k] = order[post[k]];
+      const int32_t pa = parent[post[k]];
+      parent2[k] = pa < 0 ? -1 : ipost[pa];
+    }
+    order.swap(order2);
+    parent.swap(parent2);
+    for (int32_t k = 0; k < n; ++k) iorder[order[k]] = k;
+  }
   std::vector<int64_t> cp;
   std::vector<int32_t> ri;
   permuted_lower(n, g, order, iorder, cp, ri);
-  std::vector<int32_t> parent = etree_lower(n, cp, ri);
-  // postorder and compose
-  {
-    const std::vector<int32_t> post = postorder(n, parent);
-    std::vector<int32_t> order2(n);
-    for (int32_t k = 0; k < n; ++k) order2[k] = order[post[k]];
-    order.swap(order2);
-    for (int32_t k = 0; k < n; ++k) iorder[order[k]] = k;
-    permuted_lower(n, g, order, iorder, cp, ri);
-    parent = etree_lower(n, cp, ri);
-  }
   const std::vector<int64_t> cc = column_counts(n, parent, cp, ri);
   // number of children per column
   std::vector<int32_t> nchild(n, 0);

@@ -468,16 +468,30 @@ __device__ __forceinline__ double wave_max_abs(double v) {
   return __longlong_as_double((long long)wave_max_key((unsigned long long)__double_as_longlong(fabs(v))));
 }
 
+// Value of v in lane src (src uniform across the wave).
+__device__ __forceinline__ double readlane_of(double v, int src) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, src);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane_of(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 // Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, four 64-lane waves per
 // front: lane i of every wave stands for row i of the block (LDS, padded leading dimension).
-// Every wave runs the same pivot search on the same data (DPP wave reductions; argmax = lowest
-// lane holding the maximum, by ballot), so all reach the same decision without a barrier; wave w
-// then updates the columns j = w (mod 4) of the trailing triangle, with the step's multipliers
-// formed from the pivot column (still unscaled) in registers. Wave 0 alone swaps rows and
-// columns, writes the multipliers into the pivot column after the update, keeps the counts and
-// the pivot data. One barrier per step (four when rows are swapped). Also forms the inverse of
-// the block's unit lower L11 (per-front scratch, read by the panel solve), split over the waves
-// the same way.
+// Every wave reads the pivot column into a register and runs the same pivot search on it (DPP
+// wave reductions; argmax = lowest lane holding the maximum, by ballot), so all reach the same
+// decision without a barrier. Wave w then updates the columns j = w (mod 4) of the trailing
+// triangle with the step's multipliers formed from the pivot column (broadcast LDS reads), and
+// every lane stores all of its columns (the new value where it changes, the old one elsewhere),
+// so the update has no branches. The pivot columns of F keep their unscaled values; their
+// multipliers go to a separate array Lm, written by wave 0 in the same step (nobody reads Lm during
+// a step, and a stored-back old value of F is never stale). Wave 0 alone swaps rows and
+// columns (Lm for the columns already pivoted, F for the rest), keeps the counts and the pivot
+// data. One barrier per step (four when rows are swapped). Also forms the inverse of the block's
+// unit lower L11 (per-front scratch, read by the panel solve), split over the waves the same way.
 constexpr int kBkLd = 65;
 constexpr int kBkWaves = 4;
 constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
@@ -486,10 +500,13 @@ template <class T>
 __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                            int p, int nb, int32_t* __restrict__ bperm, double null_tol,
                                                            int detect, T* __restrict__ Linv) {
-  __shared__ T F[64 * kBkLd];
-  __shared__ T Xi[64 * kBkLd];
+  __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D);
+                                // after the write-back, the inverse of L11 (Xi)
+  __shared__ T Lm[64 * kBkLd];  // multipliers of the pivot columns (strictly lower L11)
+  T* const Xi = F;
   __shared__ int perm[64];
   __shared__ int8_t ptl[64];
+  __shared__ int8_t pairc[64];  // 1: column j starts a 2x2 step, F(j + 1, j) belongs to D
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -497,7 +514,8 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   if (c0 >= ncol) return;
   const int w = min(nb, ncol - c0);
   T* P = N.L + S.panel_off[s];
-  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const bool lead = wv == 0;
   {
     // branch-free loads (row index clamped into the block) so that they are issued together
@@ -508,7 +526,10 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       F[lane + j * kBkLd] = (lane >= j && lane < w) ? v : (T)0;
     }
   }
-  if (lead) perm[lane] = lane;
+  if (lead) {
+    perm[lane] = lane;
+    pairc[lane] = 0;
+  }
   __syncthreads();
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   const bool det = detect != 0;
@@ -518,16 +539,18 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   int k = 0;
   while (k < w) {
     // ---- pivot search (column k, candidates k..w-1), the same in every wave
-    // LDS reads below are unconditional (every index lies inside F) so that a step's reads are
-    // issued together; conditions only select values and mask stores
-    const double absakk = fabs((double)F[k + k * kBkLd]);
     const T fk = F[lane + k * kBkLd];
+    const double absakk = fabs((double)readlane_of(fk, k));
     const double v = (lane > k && lane < w) ? fabs((double)fk) : 0.0;
     const double colmax = wave_max_abs(v);
     const unsigned long long at_max = __ballot(lane > k && lane < w && v == colmax);
     const int imax = (colmax > 0.0 && at_max) ? __ffsll((long long)at_max) - 1 : -1;
     int kind = 1, kp = k;
-    if (!(absakk >= kBkAlpha * colmax) && imax > k) {
+    // the row search reads entries of row imax that this step's update changes: when it runs and no
+    // interchange follows, a barrier keeps every wave's reads ahead of the first update store (the
+    // condition depends on column k only, which no wave changes in the step, so it is uniform)
+    const bool row_search = !(absakk >= kBkAlpha * colmax) && imax > k;
+    if (row_search) {
       const T ra = F[imax + lane * kBkLd], rb = F[lane + imax * kBkLd];
       const double r = (lane >= k && lane < w && lane != imax) ? fabs((double)(lane < imax ? ra : rb)) : 0.0;
       const double rowmax = wave_max_abs(r);
@@ -543,14 +566,17 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       }
     }
     const int kk = kind == 1 ? k : k + 1;
-    if (kp != kk) {
+    const bool swapped = kp != kk;
+    if (row_search && !swapped) __syncthreads();
+    if (swapped) {
       const int pp = kk, q = kp;
       __syncthreads();  // every wave has finished reading the block for its search
       if (lead) {
-        if (lane < pp) {  // rows p, q of the columns before p
-          const T a = F[pp + lane * kBkLd];
-          F[pp + lane * kBkLd] = F[q + lane * kBkLd];
-          F[q + lane * kBkLd] = a;
+        if (lane < pp) {  // rows p, q of the columns before p: multipliers, and column k of F when p = k + 1
+          T* X = lane < k ? Lm : F;
+          const T a = X[pp + lane * kBkLd];
+          X[pp + lane * kBkLd] = X[q + lane * kBkLd];
+          X[q + lane * kBkLd] = a;
         } else if (lane > pp && lane < q) {  // A(j, p) <-> A(q, j)
           const T a = F[lane + pp * kBkLd];
           F[lane + pp * kBkLd] = F[q + lane * kBkLd];
@@ -572,20 +598,25 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       }
       __syncthreads();
     }
+    // pivot column(s) after the interchange, in registers
+    const T c1 = swapped ? F[lane + k * kBkLd] : fk;
+    const T c2 = kind == 2 ? F[lane + (k + 1) * kBkLd] : (T)0;
     // ---- remaining column maxima for the null-pivot test (rows of the block below the pivot,
     //      then lazily the panel rows below the block)
     const int s0 = kind == 1 ? k + 1 : k + 2;
     double m01 = colmax;  // kind 1 without a swap: the same column below the pivot
-    if (kind == 2 || kp != kk) {
+    if (kind == 2 || swapped) {
       double mv = 0.0;
       if (lane >= s0 && lane < w) {
-        mv = fabs((double)F[lane + k * kBkLd]);
-        if (kind == 2) mv = fmax(mv, fabs((double)F[lane + (k + 1) * kBkLd]));
+        mv = fabs((double)c1);
+        if (kind == 2) mv = fmax(mv, fabs((double)c2));
       }
       m01 = wave_max_abs(mv);
     }
+    const bool own = lead;  // wave 0 writes the multipliers of this step
+    T* row = F + lane;
     if (kind == 1) {
-      const double d = (double)F[k + k * kBkLd];
+      const double d = (double)readlane_of(c1, k);
       bool null_piv = d == 0.0;
       if (det && fabs(d) <= tau && m01 <= tau) {
         double mb = 0.0;
@@ -607,9 +638,9 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         my_dof = (T)0;
       }
       const bool below = lane > k && lane < w;
-      const T fk2 = F[lane + k * kBkLd];  // after a swap column k has changed
-      const T wi = below ? fk2 : (T)0;
-      // this wave's multipliers l_j = F(j, k) / d, j = wv (mod 4)
+      const T wi = below ? c1 : (T)0;
+      // this wave's multipliers l_j = F(j, k) / d, j = wv (mod 4) (broadcast LDS reads: column k of
+      // F is not written during the step)
       T lv[kBkCols];
 #pragma unroll
       for (int q = 0; q < kBkCols; ++q) {
@@ -617,22 +648,20 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         const T fj = F[j + k * kBkLd];
         lv[q] = (j > k && j < w) ? fj * inv : (T)0;
       }
-      // rank-1 update of the lower triangle: row `lane`, this wave's columns k+1..lane
-      const bool upd = inv != (T)0 && below;
-      T* row = F + lane;
+      // rank-1 update: row `lane`, this wave's columns. lv vanishes for j <= k and wi outside the
+      // rows below the pivot, so the other entries keep their values; where j > lane the entry is
+      // in the upper triangle, which nothing reads, so no per-lane mask is needed
       T old[kBkCols];
 #pragma unroll
       for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
 #pragma unroll
-      for (int q = 0; q < kBkCols; ++q) {
-        const int j = wv + kBkWaves * q;
-        if (upd && j > k && j <= lane) row[j * kBkLd] = old[q] - wi * lv[q];
-      }
+      for (int q = 0; q < kBkCols; ++q) row[(wv + kBkWaves * q) * kBkLd] = old[q] - wi * lv[q];
+      if (own && below) Lm[lane + k * kBkLd] = wi * inv;
       __syncthreads();
-      if (lead && below) F[lane + k * kBkLd] = wi * inv;
       k += 1;
     } else {
-      const double a = (double)F[k + k * kBkLd], b = (double)F[k + 1 + k * kBkLd], c = (double)F[k + 1 + (k + 1) * kBkLd];
+      const double a = (double)readlane_of(c1, k), b = (double)readlane_of(c1, k + 1),
+                   c = (double)readlane_of(c2, k + 1);
       const double dt = a * c - b * b;
       const double mean = 0.5 * (a + c), rad = sqrt(0.25 * (a - c) * (a - c) + b * b);
       const double e1 = mean - rad, e2 = mean + rad;
@@ -652,6 +681,7 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         neg += nn;
         pos += np;
         ++two;
+        pairc[k] = 1;
       }
       if (lane == k) {
         my_pt = dead ? kPivNull : kPiv2First;
@@ -664,7 +694,6 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         my_dof = (T)0;
       }
       const bool below = lane > k + 1 && lane < w;
-      const T c1 = F[lane + k * kBkLd], c2 = F[lane + (k + 1) * kBkLd];
       const T w1 = below ? c1 : (T)0;
       const T w2 = below ? c2 : (T)0;
       // this wave's multipliers of the 2x2 step, j = wv (mod 4)
@@ -678,28 +707,27 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         l1v[q] = a1 * i11 + a2 * i12;
         l2v[q] = a1 * i12 + a2 * i22;
       }
-      const bool upd = !dead && below;
-      T* row = F + lane;
+      // rank-2 update, unmasked as in the 1x1 step (l1v, l2v vanish for j <= k + 1 and when dead)
       T old[kBkCols];
 #pragma unroll
       for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
 #pragma unroll
-      for (int q = 0; q < kBkCols; ++q) {
-        const int j = wv + kBkWaves * q;
-        if (upd && j > k + 1 && j <= lane) row[j * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
+      for (int q = 0; q < kBkCols; ++q)
+        row[(wv + kBkWaves * q) * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
+      if (own && below) {
+        Lm[lane + k * kBkLd] = w1 * i11 + w2 * i12;
+        Lm[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
       }
       __syncthreads();
-      if (lead && below) {
-        F[lane + k * kBkLd] = w1 * i11 + w2 * i12;
-        F[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
-      }
       k += 2;
     }
   }
   __syncthreads();
   // ---- write back L11 and D, pivot data
   for (int j = wv; j < w; j += kBkWaves)
-    if (lane >= j && lane < w) P[(int64_t)(c0 + j) * m + c0 + lane] = F[lane + j * kBkLd];
+    if (lane >= j && lane < w)
+      P[(int64_t)(c0 + j) * m + c0 + lane] =
+          (lane == j || (lane == j + 1 && pairc[j])) ? F[lane + j * kBkLd] : Lm[lane + j * kBkLd];
   if (lead && lane < w) {
     N.dval[first + c0 + lane] = my_dv;
     N.doff[first + c0 + lane] = my_dof;
@@ -711,25 +739,26 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   }
   // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L), outer-product
   //      form: X = I, then for t = 0.. every row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t); wave w
-  //      takes the columns j = w (mod 4), row t is read into registers first
+  //      takes the columns j = w (mod 4), row t is read into registers first; every lane stores all
+  //      of its columns (rows of X are written only by the wave that owns the column). X takes F's
+  //      storage once every wave has written its part of the block back.
+  __syncthreads();
   for (int j = wv; j < w; j += kBkWaves) Xi[lane + j * kBkLd] = lane == j ? (T)1 : (T)0;
   __syncthreads();
   for (int t = 0; t + 1 < w; ++t) {
     T rt[kBkCols], old[kBkCols];
-    T* row = Xi + lane;
+    T* xrow = Xi + lane;
 #pragma unroll
     for (int q = 0; q < kBkCols; ++q) {
       rt[q] = Xi[t + (wv + kBkWaves * q) * kBkLd];
-      old[q] = row[(wv + kBkWaves * q) * kBkLd];
+      old[q] = xrow[(wv + kBkWaves * q) * kBkLd];
     }
     const bool pair = lane == t + 1 && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
-    const T ft = F[lane + t * kBkLd];
+    const T ft = Lm[lane + t * kBkLd];
     const T lit = (lane > t && lane < w && !pair) ? ft : (T)0;
+    // row t of X vanishes beyond column t, so columns j > t keep their values
 #pragma unroll
-    for (int q = 0; q < kBkCols; ++q) {
-      const int j = wv + kBkWaves * q;
-      if (lit != (T)0 && j <= t) row[j * kBkLd] = old[q] - lit * rt[q];
-    }
+    for (int q = 0; q < kBkCols; ++q) xrow[(wv + kBkWaves * q) * kBkLd] = old[q] - lit * rt[q];
     __syncthreads();
   }
   T* out = Linv + (int64_t)f * nb * nb;

@@ -267,17 +267,40 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// C(i, c) = sum over g of part[g][c][i] in fixed order (doubles: a complex partial is reduced as
-// its two parts, k counting doubles).
-__global__ void k_gram_reduce(int groups, int k, int p, const double* __restrict__ part, double* __restrict__ c,
-                              int64_t ldc) {
-  const int64_t total = (int64_t)k * p;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % k), col = (int)(idx / k);
-    double s = 0.0;
-    for (int g = 0; g < groups; ++g) s += part[((int64_t)g * p + col) * k + i];
-    c[i + col * ldc] = s;
+// C(i, c) = sum over g of part[g][c][i] (doubles: a complex partial is reduced as its two parts, k
+// counting doubles). A workgroup takes 64 consecutive outputs o = i + c k (lane = output, so each
+// group's 64 partials are one coalesced load) and its 16 waves split the groups, wave w summing
+// g = w, w + 16, ... in increasing order; the 16 sums are then added in wave order. Fixed order:
+// bitwise reproducible.
+constexpr int kGramRedWaves = 16;
+
+__global__ void __launch_bounds__(64 * kGramRedWaves)
+    k_gram_reduce(int groups, int k, int p, const double* __restrict__ part, double* __restrict__ c, int64_t ldc) {
+  __shared__ double red[kGramRedWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t outs = (int64_t)k * p;
+  const int64_t o = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t oc = o < outs ? o : outs - 1;  // clamped: loads stay in bounds, the store is masked
+  const int64_t stride = (int64_t)p * k;
+  double s = 0.0;
+  int g = wave;
+#pragma unroll 1
+  for (; g + 3 * kGramRedWaves < groups; g += 4 * kGramRedWaves) {
+    const double a0 = part[(int64_t)g * stride + oc];
+    const double a1 = part[(int64_t)(g + kGramRedWaves) * stride + oc];
+    const double a2 = part[(int64_t)(g + 2 * kGramRedWaves) * stride + oc];
+    const double a3 = part[(int64_t)(g + 3 * kGramRedWaves) * stride + oc];
+    s = (((s + a0) + a1) + a2) + a3;
+  }
+  for (; g < groups; g += kGramRedWaves) s += part[(int64_t)g * stride + oc];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && o < outs) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kGramRedWaves; ++w) t += red[w][lane];
+    const int i = (int)(o % k), col = (int)(o / k);
+    c[i + col * ldc] = t;
   }
 }
 
@@ -459,7 +482,7 @@ void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, con
   S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p * k));
   hipLaunchKernelGGL(k_gram_partial<S>, dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw, (int)p,
                      part);
-  hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(parts * k * p, 256, 1024)), dim3(256), 0, st, groups,
+  hipLaunchKernelGGL(k_gram_reduce, dim3((int)((parts * k * p + 63) / 64)), dim3(64 * kGramRedWaves), 0, st, groups,
                      (int)(parts * k), (int)p, reinterpret_cast<const double*>(part), reinterpret_cast<double*>(c),
                      parts * ldc);
   check(hipGetLastError(), "gram");

@@ -21,7 +21,7 @@ from __future__ import annotations
 import math
 import resource
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import scipy.sparse as sp
@@ -34,15 +34,23 @@ from .factor import GpuFactorization, Pencil
 from .ops import DeviceOps, InexactFactorFailure
 
 
-def hermitian_input(M, name: str, tol: float):
-    """Checks and Hermitian part (solver.cpp:70-114): returns (csr, defect)."""
+def _nonzero_imag(M) -> bool:
+    return M is not None and np.iscomplexobj(M.data) and bool(np.any(sp.csr_matrix(M).data.imag != 0))
+
+
+def hermitian_input(M, name: str, tol: float, complex_problem: bool | None = None):
+    """Checks and Hermitian part (solver.cpp:70-114): returns (csr, defect). The values are complex
+    when the pencil is a complex problem (default: when M has a nonzero imaginary part), else real
+    (python/src/module.cpp:279-291: complex dtypes with all-zero imaginary parts are solved as
+    real)."""
     M = sp.csr_matrix(M)
     if M.shape[0] != M.shape[1]:
         raise InvalidInputError(f"{name} must be square, got {M.shape}")
     if M.nnz and not np.all(np.isfinite(M.data)):
         raise InvalidInputError(f"{name} has non-finite entries")
-    # complex input keeps complex scalars (the reference's complex template), real input stays real
-    M = M.astype(np.complex128 if np.iscomplexobj(M.data) else np.float64)
+    if complex_problem is None:
+        complex_problem = _nonzero_imag(M)
+    M = M.astype(np.complex128) if complex_problem else (M.real if np.iscomplexobj(M.data) else M).astype(np.float64)
     fro = sp.linalg.norm(M) if M.nnz else 0.0
     if fro == 0.0:
         raise InvalidInputError(f"{name} is zero")
@@ -80,6 +88,7 @@ class Prepared:
     defect_a: float
     defect_b: float
     pencil: Pencil
+    notes: list = field(default_factory=list)   # input remarks, reported as warnings "input: ..."
 
 
 def prepare(A, B=None, config: dict | None = None) -> Prepared:
@@ -87,11 +96,16 @@ def prepare(A, B=None, config: dict | None = None) -> Prepared:
     exact Hermitian parts of A and B with their defects, the B kind, the union pattern and the
     device copies of the values. A Prepared pencil can be solved many times (solve(prepared=...))."""
     cfg = cfgmod.parse_config(config or {})
-    A, def_a = hermitian_input(A, "A", cfg["hermitian_tol"])
+    # a complex problem when A or B has a nonzero imaginary part; complex A with real B promotes B
+    # (and vice versa), complex dtypes with all-zero imaginary parts are solved as real
+    cplx = _nonzero_imag(A) or _nonzero_imag(B)
+    notes = [f"{name}: real: complex input has all-zero imaginary parts; solved as real"
+             for name, M in (("A", A), ("B", B)) if M is not None and not cplx and np.iscomplexobj(M.data)]
+    A, def_a = hermitian_input(A, "A", cfg["hermitian_tol"], cplx)
     bk = _b_kind(B)
     def_b = 0.0
     if B is not None:
-        B, def_b = hermitian_input(B, "B", cfg["hermitian_tol"])
+        B, def_b = hermitian_input(B, "B", cfg["hermitian_tol"], cplx)
         if B.shape != A.shape:
             raise InvalidInputError(f"A and B differ in shape: {A.shape} vs {B.shape}")
         if bk == "diagonal":
@@ -100,7 +114,7 @@ def prepare(A, B=None, config: dict | None = None) -> Prepared:
         raise UnsupportedError("B is diagonal but not positive definite (GHIEP): not supported by this engine")
     pencil = Pencil(A, B)
     pencil.to_device()
-    return Prepared(A, B, bk, def_a, def_b, pencil)
+    return Prepared(A, B, bk, def_a, def_b, pencil, notes)
 
 
 def slice_streams(fac: GpuFactorization, requested: int | None) -> int:
@@ -151,6 +165,7 @@ def _solve(A, B, config, prepared, vectors, streams):
     if prepared is None:
         prepared = prepare(A, B, cfg)
     A, B, bk, pencil = prepared.A, prepared.B, prepared.b_kind, prepared.pencil
+    rep["warnings"].extend("input: " + note for note in prepared.notes)
     rep["hermitian_defect_a"] = prepared.defect_a
     if B is not None:
         rep["hermitian_defect_b"] = prepared.defect_b
