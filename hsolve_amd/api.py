@@ -153,12 +153,15 @@ def _mirror(M: sp.csr_matrix, storage: str) -> sp.csr_matrix:
     return (T + strict.conj().T).tocsr()
 
 
-def _matrix(m, name: str, storage: str):
+def _matrix(m, name: str, storage: str, notes: list | None = None, hermitian_tol: float = 1e-10):
     if m is None:
         return None
     if isinstance(m, (str, bytes, os.PathLike)):
         from .io import load_matrix
-        return load_matrix(os.fsdecode(m), storage=storage)
+        loaded = load_matrix(os.fsdecode(m), storage=storage, hermitian_tol=hermitian_tol)
+        if notes is not None:
+            notes.extend(f"{name}: {n}" for n in loaded.notes)
+        return loaded.matrix
     if sp.issparse(m):
         M = sp.csr_matrix(m)
     else:
@@ -195,13 +198,15 @@ def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = _Defaul
         raise InvalidInputError(f"storage = {storage!r} is not one of {_STORAGES}")
     cfg = merge_config(config, sigma, k, mode, lower, upper, tol, threads)
     _integer(verbose, "verbose")
-    Am = _matrix(A, "A", storage)
-    Bm = _matrix(B, "B", storage)
+    notes: list = []
+    Am = _matrix(A, "A", storage, notes, cfg["hermitian_tol"])
+    Bm = _matrix(B, "B", storage, notes, cfg["hermitian_tol"])
     complex_dtype = np.iscomplexobj(Am.data) or (Bm is not None and np.iscomplexobj(Bm.data))
     from .solver import solve as engine_solve
     with _SOLVE_LOCK:
         w, v, report, cfg_out = engine_solve(Am, Bm, cfg)
     report.pop("_events", None)
+    report["warnings"].extend("input: " + n for n in notes)
     if complex_dtype:
         v = v.astype(np.complex128, order="F")
     res = SolveResult(np.asarray(w, dtype=np.float64), v, np.zeros(0, dtype=np.complex128),
