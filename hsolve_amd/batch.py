@@ -1,0 +1,89 @@
+"""Independent pencils solved together on one device (the Bloch k-points of a photonic cell, the
+requests of a serving loop).
+
+Each pencil is an ordinary solve (hsolve_amd.solver) with its own configuration, certificate and
+report. What the batch adds:
+  * one symbolic analysis per distinct factorized pattern: pencils whose union pattern and value
+    maps coincide (k-points of one cell whose Bloch phases are complex in the same directions)
+    factorize on factorization objects cloned from one analysis (GpuFactorization.clone_for), the
+    counterpart of the reference's helper processes holding the problem's analysis
+    (src/driver/helpers.cpp); only the first solve of a pattern pays the analysis;
+  * concurrency: `workers` host threads, each with a stream of its own, take the pencils in order;
+    a pencil's result does not depend on which worker solved it or on the others (each solve owns
+    its factorization, Krylov scratch and report).
+The dense host work of every solve runs on one BLAS thread (as in solver.solve).
+"""
+from __future__ import annotations
+
+import threading
+
+import torch
+
+from . import config as cfgmod
+from .factor import GpuFactorization, same_pattern
+from .solver import _one_blas_thread, _solve, prepare
+
+
+def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host", share_analysis: bool = True):
+    """pencils: a list of (A, B) (B may be None) or of solver.Prepared; config: one configuration
+    for all, or a list with one per pencil. Returns a list of (eigenvalues, eigenvectors, report,
+    config) in the pencils' order. Every factorization keeps its numeric storage until the batch
+    ends (a free in the middle would synchronize the device and stall the other workers)."""
+    n = len(pencils)
+    cfgs = config if isinstance(config, list) else [config] * n
+    if len(cfgs) != n:
+        raise ValueError("solve_batch: one configuration per pencil")
+    cfgs = [cfgmod.parse_config(c or {}) for c in cfgs]
+    prepared = [p if hasattr(p, "pencil") else prepare(p[0], p[1], cfgs[i]) for i, p in enumerate(pencils)]
+    # one analysis per distinct factorized pattern, clones for the other pencils of that pattern
+    heads: list[GpuFactorization] = []
+    facs = []
+    for p in prepared:
+        head = next((f for f in heads if same_pattern(f.pencil, p.pencil)), None) if share_analysis else None
+        if head is None:
+            head = GpuFactorization(p.pencil)
+            heads.append(head)
+            facs.append(head)
+        else:
+            facs.append(head.clone_for(p.pencil))
+    results = [None] * n
+    errors = [None] * n
+    nxt = [0]
+    lock = threading.Lock()
+    workers = max(1, min(int(workers), n))
+    streams = [torch.cuda.Stream() for _ in range(workers)]
+
+    def worker(w: int):
+        with torch.cuda.stream(streams[w]):
+            while True:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                    if any(e is not None for e in errors):
+                        i = n
+                if i >= n:
+                    break
+                try:
+                    results[i] = _solve(None, None, cfgs[i], prepared[i], vectors, 1, facs[i])
+                except BaseException as e:  # re-raised in pencil order below
+                    errors[i] = e
+            streams[w].synchronize()
+
+    with _one_blas_thread():
+        torch.cuda.synchronize()
+        threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-batch-{w}") for w in range(workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+    for e in errors:
+        if e is not None:
+            raise e
+    for f in facs:
+        f.release_factors()
+    for i, r in enumerate(results):
+        r[2].pop("_events", None)
+        r[2]["parallel"]["notes"].append(
+            f"batch of {n} pencils on {workers} concurrent streams; {len(heads)} analyses shared by pattern")
+    return results

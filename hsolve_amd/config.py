@@ -12,6 +12,8 @@ import copy
 import json
 import math
 
+import numpy as np
+
 from .errors import InvalidInputError
 
 VERSION = "0.1.0"
@@ -251,11 +253,13 @@ def report_to_json(rep: dict, indent: int = 2) -> str:
     """JSON text of a report; non-finite window ends become null (serialization.cpp:110)."""
 
     def clean(o):
+        if isinstance(o, np.generic):   # numpy scalars (bool_, integers, floats) as Python values
+            o = o.item()
         if isinstance(o, float):
             return o if math.isfinite(o) else None
         if isinstance(o, dict):
             return {k: clean(v) for k, v in o.items()}
-        if isinstance(o, list):
+        if isinstance(o, (list, tuple)):
             return [clean(v) for v in o]
         return o
 

@@ -170,6 +170,7 @@ int64_t cut_of(const WGraph& g, const std::vector<int8_t>& part) {
 
 // Fiduccia-Mattheyses style refinement of a 2-way edge partition with a balance bound; several
 // passes, each moving vertices by best gain with rollback to the best prefix.
+constexpr size_t kFmStall = 100;
 void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int passes) {
   const int32_t n = g.n;
   const int64_t maxw = static_cast<int64_t>(max_frac * static_cast<double>(g.total_vw)) + 1;
@@ -193,7 +194,9 @@ void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int 
     int64_t cur = 0, best = 0;
     size_t best_len = 0;
     const size_t limit = static_cast<size_t>(std::max<int32_t>(50, n / 10));
-    while (seq.size() < limit) {
+    // a pass also ends after kFmStall moves in a row that did not improve on the best cut: on
+    // large graphs the moves beyond that are almost always rolled back
+    while (seq.size() < limit && seq.size() - best_len <= kFmStall) {
       // pick the side to move from: the heavier side first if it has candidates
       int from = pw[0] >= pw[1] ? 0 : 1;
       int32_t v = -1;

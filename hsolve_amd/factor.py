@@ -217,6 +217,13 @@ class Pencil:
         return self._dev
 
 
+def same_pattern(p: Pencil, q: Pencil) -> bool:
+    """Whether one analysis serves both pencils: the same factorized pattern and value maps."""
+    return (p.f_n == q.f_n and p.is_complex == q.is_complex and np.array_equal(p.f_rowptr, q.f_rowptr)
+            and np.array_equal(p.f_col, q.f_col) and np.array_equal(p.f_a_index, q.f_a_index)
+            and np.array_equal(p.f_b_index, q.f_b_index))
+
+
 class GpuFactorization:
     """Sparse LDL^T of the pencil on the GPU (SparseFactorization contract)."""
 
@@ -359,6 +366,16 @@ class GpuFactorization:
         h = C.c_void_p(0)
         _lib.call("hsg_factor_clone", self._handle, C.byref(h))
         other._handle = h
+        return other
+
+    def clone_for(self, pencil: Pencil) -> "GpuFactorization":
+        """A factorization object for another pencil of the same pattern (e.g. the Bloch k-points of
+        one cell): this object's analysis, the other pencil's values. The factorized pattern and
+        the value maps must be identical (same_pattern)."""
+        if not same_pattern(self.pencil, pencil):
+            raise InvalidInputError("clone_for: the pencil's factorization pattern differs from the analysed one")
+        other = self.clone()
+        other.pencil = pencil
         return other
 
     def release_factors(self) -> bool:

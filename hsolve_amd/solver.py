@@ -157,7 +157,7 @@ def _one_blas_thread():
         return contextlib.nullcontext()
 
 
-def _solve(A, B, config, prepared, vectors, streams):
+def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | None = None):
     cfg = cfgmod.parse_config(config or {})
     rep = cfgmod.new_report()
     clock = PhaseClock()
@@ -187,7 +187,10 @@ def _solve(A, B, config, prepared, vectors, streams):
     rep["complex_scalars"] = bool(pencil.is_complex)
     clock.pop()
     with clock.phase("factor"):
-        fac = GpuFactorization(pencil)
+        if fac is None:
+            fac = GpuFactorization(pencil)
+        elif fac.pencil is not pencil:
+            raise InvalidInputError("solve: the given factorization object belongs to another pencil")
     if B is not None and bk == "general":
         with clock.phase("setup"):
             fac.set_null_pivot_detection(True)
