@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "analysis.hpp"
@@ -360,20 +361,43 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
 
 std::vector<int64_t> assembly_map(const Symbolic& S, int32_t n, const int64_t* rowptr, const int32_t* colind) {
   std::vector<int64_t> dst(static_cast<size_t>(rowptr[n]), -1);
-  for (int32_t i = 0; i < n; ++i) {
-    const int32_t in = S.iorder[i];
-    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-      const int32_t jn = S.iorder[colind[p]];
-      if (in < jn) continue;  // mirrored copy above the diagonal
-      const int32_t s = S.col_sn[jn];
-      const int32_t* r = &S.rows[S.sn_rowoff[s]];
-      const int32_t m = S.sn_m[s];
-      const int32_t* it = std::lower_bound(r, r + m, in);
-      if (it == r + m || *it != in) throw std::runtime_error("assembly_map: entry outside the front");
-      const int64_t lr = it - r, lc = jn - S.sn_first[s];
-      dst[p] = S.panel_off[s] + lc * m + lr;
+  // rows are independent: contiguous row ranges on up to 16 threads (each entry is written once)
+  auto rows = [&](int32_t i0, int32_t i1, bool& bad) {
+    for (int32_t i = i0; i < i1; ++i) {
+      const int32_t in = S.iorder[i];
+      for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+        const int32_t jn = S.iorder[colind[p]];
+        if (in < jn) continue;  // mirrored copy above the diagonal
+        const int32_t s = S.col_sn[jn];
+        const int32_t* r = &S.rows[S.sn_rowoff[s]];
+        const int32_t m = S.sn_m[s];
+        const int32_t* it = std::lower_bound(r, r + m, in);
+        if (it == r + m || *it != in) {
+          bad = true;
+          return;
+        }
+        const int64_t lr = it - r, lc = jn - S.sn_first[s];
+        dst[p] = S.panel_off[s] + lc * m + lr;
+      }
     }
+  };
+  const int nt = n < 8192 ? 1 : std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+  std::vector<char> bad(nt, 0);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      bool b = false;
+      rows((int32_t)((int64_t)n * t / nt), (int32_t)((int64_t)n * (t + 1) / nt), b);
+      bad[t] = b;
+    });
+  {
+    bool b = false;
+    rows(0, (int32_t)((int64_t)n / nt), b);
+    bad[0] = b;
   }
+  for (auto& th : pool) th.join();
+  for (char b : bad)
+    if (b) throw std::runtime_error("assembly_map: entry outside the front");
   return dst;
 }
 
