@@ -212,35 +212,37 @@ __global__ void k_reduce_partials(int groups, int p, const double* __restrict__ 
 // column tile k0; a row tile (64 rows of doubles, 32 of complex values) is staged through LDS with
 // coalesced loads.
 constexpr int kGramCols = 64;
-constexpr int kGramP = 16;
 template <class S>
 constexpr int gram_rows() {
   return sizeof(S) == sizeof(double) ? 64 : 32;
 }
 
-template <class S>
+// PC columns of W per pass (4, 8 or 16 by p): wave g of the workgroup takes columns
+// g PC/4 .. (g+1) PC/4 - 1 of the pass, lane kk column kk of the V tile.
+template <class S, int PC>
 __global__ void __launch_bounds__(256)
     k_gram_partial(int64_t n, const S* __restrict__ v, int64_t ldv, int k, const double* __restrict__ wd,
                    const S* __restrict__ w, int64_t ldw, int p, S* __restrict__ part) {
   constexpr int kRows = gram_rows<S>();
+  constexpr int kPer = PC / 4;  // W columns per thread
   __shared__ S Vt[kRows][kGramCols + 1];
-  __shared__ S Wt[kRows][kGramP + 1];
+  __shared__ S Wt[kRows][PC + 1];
   const int k0 = blockIdx.y * kGramCols;
   const int kc = min(kGramCols, k - k0);
   const int tid = threadIdx.x;
-  const int kk = tid % 64, cg = tid / 64;  // column of V, group of 4 columns of W
-  for (int c0 = 0; c0 < p; c0 += kGramP) {
-    const int pc = min(kGramP, p - c0);
-    S acc[4];
+  const int kk = tid % 64, cg = tid / 64;  // column of V, group of kPer columns of W
+  for (int c0 = 0; c0 < p; c0 += PC) {
+    const int pc = min(PC, p - c0);
+    S acc[kPer];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = zero_of<S>();
+    for (int q = 0; q < kPer; ++q) acc[q] = zero_of<S>();
     for (int64_t r0 = blockIdx.x * (int64_t)kRows; r0 < n; r0 += (int64_t)gridDim.x * kRows) {
       const int rn = (int)min<int64_t>(kRows, n - r0);
       for (int idx = tid; idx < kRows * kGramCols; idx += 256) {
         const int r = idx % kRows, j = idx / kRows;
         Vt[r][j] = (r < rn && j < kc) ? conj_of(v[r0 + r + (int64_t)(k0 + j) * ldv]) : zero_of<S>();
       }
-      for (int idx = tid; idx < kRows * kGramP; idx += 256) {
+      for (int idx = tid; idx < kRows * PC; idx += 256) {
         const int r = idx % kRows, c = idx / kRows;
         S val = zero_of<S>();
         if (r < rn && c < pc) {
@@ -254,14 +256,14 @@ __global__ void __launch_bounds__(256)
       for (int r = 0; r < kRows; ++r) {
         const S a = Vt[r][kk];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += a * Wt[r][cg * 4 + q];
+        for (int q = 0; q < kPer; ++q) acc[q] += a * Wt[r][cg * kPer + q];
       }
       __syncthreads();
     }
     // partial layout: part[g][c][k] with g = blockIdx.x, c in [0, p), k in [0, k)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = cg * 4 + q;
+    for (int q = 0; q < kPer; ++q) {
+      const int c = cg * kPer + q;
       if (kk < kc && c < pc) part[((int64_t)blockIdx.x * p + c0 + c) * k + k0 + kk] = acc[q];
     }
   }
@@ -304,29 +306,41 @@ __global__ void __launch_bounds__(64 * kGramRedWaves)
   }
 }
 
-// W = beta W + alpha V C; one thread per row, 16 columns of W at a time, C staged in LDS.
-template <class S>
+// W = beta W + alpha V C; one thread per row, NC columns of W at a time (NC = 4, 8 or 16 by p),
+// C staged in LDS row by row (C(j, c0..c0+NC) contiguous, read as vectors), four rows of V in
+// flight per thread. Each column sums j = 0..k-1 in order.
+template <class S, int NC>
 __global__ void __launch_bounds__(256)
     k_block_update(int64_t n, const S* __restrict__ v, int64_t ldv, int k, const S* __restrict__ cmat, int64_t ldc,
                    S* __restrict__ w, int64_t ldw, int p, double alpha, double beta) {
   extern __shared__ __align__(16) unsigned char cs_raw[];
-  S* cs = reinterpret_cast<S*>(cs_raw);  // k x 16
-  for (int c0 = 0; c0 < p; c0 += 16) {
-    const int pc = min(16, p - c0);
+  S* cs = reinterpret_cast<S*>(cs_raw);  // k x NC
+  for (int c0 = 0; c0 < p; c0 += NC) {
+    const int pc = min(NC, p - c0);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < k * 16; idx += blockDim.x) {
-      const int j = idx % k, c = idx / k;
+    for (int idx = threadIdx.x; idx < k * NC; idx += blockDim.x) {
+      const int j = idx / NC, c = idx % NC;
       cs[idx] = c < pc ? cmat[j + (int64_t)(c0 + c) * ldc] : zero_of<S>();
     }
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      S acc[16];
+      S acc[NC];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) acc[c] = zero_of<S>();
-      for (int j = 0; j < k; ++j) {
+      for (int c = 0; c < NC; ++c) acc[c] = zero_of<S>();
+      int j = 0;
+      for (; j + 4 <= k; j += 4) {
+        S a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = v[i + (int64_t)(j + u) * ldv];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] += a[u] * cs[(j + u) * NC + c];
+      }
+      for (; j < k; ++j) {
         const S a = v[i + (int64_t)j * ldv];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) acc[c] += a * cs[j + c * k];
+        for (int c = 0; c < NC; ++c) acc[c] += a * cs[j * NC + c];
       }
       for (int c = 0; c < pc; ++c) {
         S* ww = w + i + (int64_t)(c0 + c) * ldw;
@@ -480,8 +494,15 @@ void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, con
   // row groups: enough workgroups to fill the chip, each over many row tiles
   int groups = grid_for(n, gram_rows<S>() * 4, std::max(32, 2048 / kt));
   S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p * k));
-  hipLaunchKernelGGL(k_gram_partial<S>, dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw, (int)p,
-                     part);
+  if (p <= 4)
+    hipLaunchKernelGGL((k_gram_partial<S, 4>), dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw,
+                       (int)p, part);
+  else if (p <= 8)
+    hipLaunchKernelGGL((k_gram_partial<S, 8>), dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw,
+                       (int)p, part);
+  else
+    hipLaunchKernelGGL((k_gram_partial<S, 16>), dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw,
+                       (int)p, part);
   hipLaunchKernelGGL(k_gram_reduce, dim3((int)((parts * k * p + 63) / 64)), dim3(64 * kGramRedWaves), 0, st, groups,
                      (int)(parts * k), (int)p, reinterpret_cast<const double*>(part), reinterpret_cast<double*>(c),
                      parts * ldc);
@@ -509,10 +530,19 @@ void block_update_t(int64_t n, const S* v, int64_t ldv, int64_t k, const S* c, i
     if (beta != 1.0) diag_spmm_t<S>(n, nullptr, w, ldw, w, ldw, p, beta, 0.0, st);
     return;
   }
-  const size_t shm = sizeof(S) * (size_t)k * 16;
+  const int nc = p <= 4 ? 4 : (p <= 8 ? 8 : 16);
+  const size_t shm = sizeof(S) * (size_t)k * nc;
   if (shm > 150 * 1024) throw std::runtime_error("block_update: k too large");
-  hipLaunchKernelGGL(k_block_update<S>, dim3(grid_for(n, 256, 4096)), dim3(256), shm, st, n, v, ldv, (int)k, c, ldc, w,
-                     ldw, (int)p, alpha, beta);
+  const dim3 grid(grid_for(n, 256, 4096)), block(256);
+  if (nc == 4)
+    hipLaunchKernelGGL((k_block_update<S, 4>), grid, block, shm, st, n, v, ldv, (int)k, c, ldc, w, ldw, (int)p, alpha,
+                       beta);
+  else if (nc == 8)
+    hipLaunchKernelGGL((k_block_update<S, 8>), grid, block, shm, st, n, v, ldv, (int)k, c, ldc, w, ldw, (int)p, alpha,
+                       beta);
+  else
+    hipLaunchKernelGGL((k_block_update<S, 16>), grid, block, shm, st, n, v, ldv, (int)k, c, ldc, w, ldw, (int)p, alpha,
+                       beta);
   check(hipGetLastError(), "block_update");
 }
 
