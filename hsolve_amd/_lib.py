@@ -63,10 +63,31 @@ SIGNATURES = {
 for _name in ("csr_spmm", "diag_spmm", "shifted_residual", "gram", "block_update", "column_dots"):
     SIGNATURES["hsg_z" + _name] = SIGNATURES["hsg_" + _name]
 
+# Entry points that run long on the host (analysis, the numeric factorization and its inertia, a
+# solve's level-by-level launches, factor copies) release the GIL while they run. Every other entry
+# point only enqueues one or two kernels on a stream and returns within microseconds: those keep the
+# GIL. Concurrent spectrum slices drive the device from several Python threads, and a thread that
+# gave up the GIL for a short call could wait a full interpreter switch interval to get it back.
+RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_solve", "hsg_factor_clone", "hsg_release_factors",
+                          "hsg_factor_destroy"})
+
+
+class Bound:
+    """libhsg entry points with their ctypes signatures, one attribute per exported name."""
+
+    def __init__(self, path: Path):
+        releasing, holding = C.CDLL(str(path)), C.PyDLL(str(path))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(releasing if name in RELEASES_GIL else holding, name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+
 _lib = None
 
 
-def lib() -> C.CDLL:
+def lib() -> Bound:
     """The loaded library; raises ImportError if it is missing (build with python -m hsolve_amd.build)."""
     global _lib
     if _lib is None:
@@ -74,12 +95,7 @@ def lib() -> C.CDLL:
         if not path.exists():
             raise ImportError(f"libhsg.so not found at {path}: build it with `python -m hsolve_amd.build` "
                               "(the HIP engine has no CPU fallback)")
-        handle = C.CDLL(str(path))
-        for name, (res, args) in SIGNATURES.items():
-            fn = getattr(handle, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = handle
+        _lib = Bound(path)
     return _lib
 
 

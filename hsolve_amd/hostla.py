@@ -1,27 +1,22 @@
-"""Host dense linear algebra of the driver (projected eigenproblems, Cholesky factors of small
-Gram matrices), serialized across threads: spectrum slices solved concurrently call it from
-several threads, and LAPACK/BLAS builds are not all safe to enter concurrently. The calls are
-small (order <= a few hundred) and run while other slices keep the device busy."""
+"""Host dense linear algebra of the driver: projected eigenproblems and the Cholesky factors and
+inverses of small Gram matrices (order <= a few hundred). Spectrum slices solved concurrently call
+these from several threads at once, each on one BLAS thread (the solver limits the BLAS pool while it
+runs). The LAPACK builds bundled with numpy and scipy are reentrant (per-thread work buffers, LAPACK
+compiled recursive), so the calls run in parallel: a lock here would make a slice's 8 x 8 Cholesky
+factor wait behind another slice's Rayleigh-Ritz eigensolve."""
 from __future__ import annotations
-
-import threading
 
 import numpy as np
 import scipy.linalg as sla
 
-_LOCK = threading.Lock()
-
 
 def eigh(a: np.ndarray):
-    with _LOCK:
-        return np.linalg.eigh(a)
+    return np.linalg.eigh(a)
 
 
 def cholesky_upper(a: np.ndarray) -> np.ndarray:
-    with _LOCK:
-        return sla.cholesky(a, lower=False)
+    return sla.cholesky(a, lower=False)
 
 
 def inv(a: np.ndarray) -> np.ndarray:
-    with _LOCK:
-        return np.linalg.inv(a)
+    return np.linalg.inv(a)

@@ -196,19 +196,35 @@ class BlockLanczos:
             if coef is not None:
                 coef += c.cpu().numpy().T
 
-    def _orthonormalize(self, k: int, w: torch.Tensor, breakdown_tol: float, norms_in: np.ndarray):
+    def _orthonormalize(self, k: int, w: torch.Tensor, breakdown_tol: float):
         """Block CGS2 of w (p, n) against V[:k], then Cholesky-QR2 within the block. Returns
-        (coef k x p, R p x p). Breakdown columns are replaced by random vectors (coupling 0)."""
+        (coef k x p, R p x p). Breakdown columns (B-norm below breakdown_tol times the norm w had on
+        entry) are replaced by random vectors (coupling 0). The entry norms, both CGS2 coefficient
+        blocks and the first Gram matrix land in one device buffer fetched with one copy, so a block
+        step waits on the device twice (before each Cholesky factor)."""
         ops = self.ops
         p = w.shape[0]
+        kp = k * p
+        buf = torch.empty(p + 2 * kp + p * p, dtype=ops.dtype, device=ops.dev)
+        ops.b_dots(w, w, out=buf[:p])
+        if k:
+            vk = self.V[:k]
+            for i in range(2):
+                c = buf[p + i * kp:p + (i + 1) * kp].view(p, k)
+                ops.b_gram(vk, w, out=c)
+                ops.update(vk, c, w, alpha=-1.0, beta=1.0)
+        ops.b_gram(w, w, out=buf[p + 2 * kp:].view(p, p))
+        h = buf.cpu().numpy()
+        norms_in = np.sqrt(np.maximum(h[:p].real, 0.0))
         coef = np.zeros((k, p), dtype=self.dt)
-        self._project_out(k, w, coef)
-        G = ops.b_gram(w, w).cpu().numpy().T
+        for i in range(2 if k else 0):
+            coef += h[p + i * kp:p + (i + 1) * kp].reshape(p, k).T
+        G = h[p + 2 * kp:].reshape(p, p).T
         G = 0.5 * (G + G.conj().T)
         ok = True
         R = np.eye(p, dtype=self.dt)
         if np.all(np.diag(G).real > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
-            for _ in range(2):
+            for it in range(2):
                 try:
                     Rk = hostla.cholesky_upper(G)
                 except np.linalg.LinAlgError:
@@ -223,8 +239,9 @@ class BlockLanczos:
                 ops.update(w, ops.coef(Rinv), tmp, 1.0, 0.0)
                 w.copy_(tmp)
                 R = Rk @ R
-                G = ops.b_gram(w, w).cpu().numpy().T
-                G = 0.5 * (G + G.conj().T)
+                if it == 0:
+                    G = ops.b_gram(w, w).cpu().numpy().T
+                    G = 0.5 * (G + G.conj().T)
         else:
             ok = False
         if ok:
@@ -269,8 +286,7 @@ class BlockLanczos:
         d, p = self.d, self.p
         blk = self.V[d:d + p]
         blk.copy_(self._random(p))
-        norms = torch.sqrt(self.ops.b_dots(blk, blk)).cpu().numpy()
-        self._orthonormalize(d, blk, START_MIN_RATIO, norms)
+        self._orthonormalize(d, blk, START_MIN_RATIO)
 
     # ------------------------------------------------------------ steps
     def _expand(self, first: int) -> bool:
@@ -278,10 +294,8 @@ class BlockLanczos:
         for j in range(first, self.ncv, p):
             src = self.V[d + j:d + j + p]
             dst = self.V[d + j + p:d + j + 2 * p]
-            y = self.op(src)
-            norms = torch.sqrt(torch.clamp(self.ops.b_dots(y, y), min=0.0)).cpu().numpy()
-            dst.copy_(y)
-            coef, R = self._orthonormalize(d + j + p, dst, BREAKDOWN_TOL, norms)
+            dst.copy_(self.op(src))
+            coef, R = self._orthonormalize(d + j + p, dst, BREAKDOWN_TOL)
             A = coef[d + j:d + j + p, :]
             A = 0.5 * (A + A.conj().T)
             self.H[j:j + p, j:j + p] = A

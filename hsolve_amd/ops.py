@@ -53,8 +53,14 @@ class DeviceOps:
         return torch.from_numpy(np.ascontiguousarray(a, dtype=self.np_dtype)).to(self.dev)
 
     def coef(self, m: np.ndarray) -> torch.Tensor:
-        """A host k x p coefficient matrix as the (p, k) device tensor update() takes."""
-        return torch.from_numpy(np.ascontiguousarray(np.asarray(m).T, dtype=self.np_dtype)).to(self.dev)
+        """A host k x p coefficient matrix as the (p, k) device tensor update() takes. Staged through
+        pinned memory and copied without blocking the host: the copy is ordered on the current
+        stream before the kernels that read it, and the caching host allocator keeps the staging
+        block until the copy has run."""
+        a = np.ascontiguousarray(np.asarray(m).T, dtype=self.np_dtype)
+        h = torch.empty(a.shape, dtype=self.dtype, pin_memory=True)
+        h.numpy()[...] = a
+        return h.to(self.dev, non_blocking=True)
 
     def _rows(self) -> int:
         """Doubles per row of a block (the real view of a complex block has 2n)."""
@@ -93,22 +99,24 @@ class DeviceOps:
         raise NotImplementedError
 
     # --------------------------------------------------------------- dense
-    def gram(self, v: torch.Tensor, w: torch.Tensor, weight=None) -> torch.Tensor:
-        """C = V^H diag(weight) W as a (p, k) tensor (column-major k x p)."""
+    def gram(self, v: torch.Tensor, w: torch.Tensor, weight=None, out: torch.Tensor | None = None) -> torch.Tensor:
+        """C = V^H diag(weight) W as a (p, k) tensor (column-major k x p); `out`, when given, is a
+        contiguous (p, k) tensor that receives C."""
         k, p = v.shape[0], w.shape[0]
-        c = torch.empty((p, k), dtype=self.dtype, device=self.dev)
+        c = torch.empty((p, k), dtype=self.dtype, device=self.dev) if out is None else out
         if k == 0 or p == 0:
             return c
         _lib.call(self._k + "gram", self.n, ptr(v), self.n, k, ptr(weight), ptr(w), self.n, p, ptr(c), k, stream_ptr())
         return c
 
-    def b_gram(self, v: torch.Tensor, w: torch.Tensor, bw: torch.Tensor | None = None) -> torch.Tensor:
+    def b_gram(self, v: torch.Tensor, w: torch.Tensor, bw: torch.Tensor | None = None,
+               out: torch.Tensor | None = None) -> torch.Tensor:
         """V^H B W; bw = B W when already formed (general B)."""
         if self.pencil.b_is_diagonal:
-            return self.gram(v, w, self.b_weight())
+            return self.gram(v, w, self.b_weight(), out)
         if bw is None:
             bw = self.apply_b(w)
-        return self.gram(v, bw, None)
+        return self.gram(v, bw, None, out)
 
     def update(self, v: torch.Tensor, c: torch.Tensor, w: torch.Tensor, alpha=1.0, beta=1.0) -> torch.Tensor:
         """W = beta W + alpha V C with C a (p, k) tensor (column-major k x p)."""
@@ -119,22 +127,24 @@ class DeviceOps:
                   float(beta), stream_ptr())
         return w
 
-    def dots(self, x: torch.Tensor, y: torch.Tensor, weight=None) -> torch.Tensor:
+    def dots(self, x: torch.Tensor, y: torch.Tensor, weight=None, out: torch.Tensor | None = None) -> torch.Tensor:
         """x_c^H diag(weight) y_c per row pair; real parts (the values the driver uses: norms and the
-        Rayleigh quotients of Hermitian pencils, whose imaginary parts vanish)."""
+        Rayleigh quotients of Hermitian pencils, whose imaginary parts vanish). `out`, when given,
+        is a (p,) tensor of the block's scalar type that receives the (complex) sums."""
         p = x.shape[0]
-        out = torch.empty(p, dtype=self.dtype, device=self.dev)
+        if out is None:
+            out = torch.empty(p, dtype=self.dtype, device=self.dev)
         if p:
             _lib.call(self._k + "column_dots", self.n, ptr(x), self.n, ptr(y), self.n, p, ptr(weight), ptr(out),
                       stream_ptr())
         return out.real if self.is_complex else out
 
-    def b_dots(self, x, y, by=None):
+    def b_dots(self, x, y, by=None, out: torch.Tensor | None = None):
         if self.pencil.b_is_diagonal:
-            return self.dots(x, y, self.b_weight())
+            return self.dots(x, y, self.b_weight(), out)
         if by is None:
             by = self.apply_b(y)
-        return self.dots(x, by)
+        return self.dots(x, by, None, out)
 
     def pair_sums(self, ax: torch.Tensor, bx: torch.Tensor, x: torch.Tensor, lam: np.ndarray) -> np.ndarray:
         """(p, 3) host array: ||ax - lambda bx||^2, ||bx||^2, ||x||^2 per column."""
