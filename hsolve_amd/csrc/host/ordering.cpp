@@ -25,6 +25,30 @@
 
 namespace hsg {
 
+namespace {
+
+// Threads for a loop over `items` independent items: one below `min_items`, else up to 16.
+int worker_count(int64_t items, int64_t min_items) {
+  if (items < min_items) return 1;
+  return std::min(16, static_cast<int>(std::max(1u, std::thread::hardware_concurrency())));
+}
+
+// body(i0, i1) over nt contiguous ranges of [0, n), the first on the calling thread.
+template <class F>
+void parallel_ranges(int32_t n, int nt, F&& body) {
+  if (nt <= 1) {
+    body(0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t)
+    pool.emplace_back([&, t] { body(static_cast<int32_t>(int64_t{n} * t / nt), static_cast<int32_t>(int64_t{n} * (t + 1) / nt)); });
+  body(0, static_cast<int32_t>(int64_t{n} / nt));
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
 Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind) {
   // Count both directions, then sort/unique each row.
   std::vector<int64_t> deg(static_cast<size_t>(n) + 1, 0);
@@ -46,17 +70,25 @@ Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind) {
       tmp[fill[i]++] = j;
       tmp[fill[j]++] = i;
     }
+  // rows sorted and deduplicated over row ranges on several threads, concatenated in row order
   Graph g;
   g.n = n;
   g.ptr.assign(static_cast<size_t>(n) + 1, 0);
-  g.adj.reserve(tmp.size() / 2 + 16);
-  for (int32_t i = 0; i < n; ++i) {
-    auto b = tmp.begin() + deg[i], e = tmp.begin() + deg[i + 1];
-    std::sort(b, e);
-    auto last = std::unique(b, e);
-    g.adj.insert(g.adj.end(), b, last);
-    g.ptr[i + 1] = static_cast<int64_t>(g.adj.size());
-  }
+  const int nt = worker_count(n, 32768);
+  std::vector<int64_t> kept(static_cast<size_t>(n) + 1, 0);
+  parallel_ranges(n, nt, [&](int32_t i0, int32_t i1) {
+    for (int32_t i = i0; i < i1; ++i) {
+      auto b = tmp.begin() + deg[i], e = tmp.begin() + deg[i + 1];
+      std::sort(b, e);
+      kept[i + 1] = std::unique(b, e) - b;
+    }
+  });
+  for (int32_t i = 0; i < n; ++i) g.ptr[i + 1] = g.ptr[i] + kept[i + 1];
+  g.adj.resize(static_cast<size_t>(g.ptr[n]));
+  parallel_ranges(n, nt, [&](int32_t i0, int32_t i1) {
+    for (int32_t i = i0; i < i1; ++i)
+      std::copy(tmp.begin() + deg[i], tmp.begin() + deg[i] + kept[i + 1], g.adj.begin() + g.ptr[i]);
+  });
   return g;
 }
 
@@ -124,12 +156,6 @@ bool coarsen(const WGraph& g, WGraph& c, std::vector<int32_t>& cmap, Rng& rng) {
   c.vw.assign(nc, 0);
   for (int32_t v = 0; v < n; ++v) c.vw[cmap[v]] += g.vw[v];
   c.total_vw = g.total_vw;
-  c.ptr.assign(static_cast<size_t>(nc) + 1, 0);
-  c.adj.clear();
-  c.ew.clear();
-  c.adj.reserve(g.adj.size() / 2);
-  c.ew.reserve(g.adj.size() / 2);
-  std::vector<int32_t> slot(nc, -1);  // position of coarse neighbour in the current row
   // Visit coarse vertices in order; the fine members are v and match[v].
   std::vector<int32_t> first(nc, -1), second(nc, -1);
   for (int32_t v = 0; v < n; ++v) {
@@ -137,6 +163,12 @@ bool coarsen(const WGraph& g, WGraph& c, std::vector<int32_t>& cmap, Rng& rng) {
     if (first[cv] < 0) first[cv] = v;
     else second[cv] = v;
   }
+  c.ptr.assign(static_cast<size_t>(nc) + 1, 0);
+  c.adj.clear();
+  c.ew.clear();
+  c.adj.reserve(g.adj.size() / 2);
+  c.ew.reserve(g.adj.size() / 2);
+  std::vector<int32_t> slot(nc, -1);  // position of coarse neighbour in the current row
   for (int32_t cv = 0; cv < nc; ++cv) {
     const int64_t row_start = static_cast<int64_t>(c.adj.size());
     for (int k = 0; k < 2; ++k) {
