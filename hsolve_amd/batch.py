@@ -20,6 +20,7 @@ import threading
 import torch
 
 from . import config as cfgmod
+from . import streams as stream_pool
 from .factor import GpuFactorization, same_pattern
 from .solver import _one_blas_thread, _solve, prepare
 
@@ -51,7 +52,7 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
     nxt = [0]
     lock = threading.Lock()
     workers = max(1, min(int(workers), n))
-    streams = [torch.cuda.Stream() for _ in range(workers)]
+    streams = stream_pool.checkout(workers)
 
     def worker(w: int):
         with torch.cuda.stream(streams[w]):
@@ -69,14 +70,17 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
                     errors[i] = e
             streams[w].synchronize()
 
-    with _one_blas_thread():
-        torch.cuda.synchronize()
-        threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-batch-{w}") for w in range(workers)]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-        torch.cuda.synchronize()
+    try:
+        with _one_blas_thread():
+            torch.cuda.synchronize()
+            threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-batch-{w}") for w in range(workers)]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+            torch.cuda.synchronize()
+    finally:
+        stream_pool.checkin(streams)
     for e in errors:
         if e is not None:
             raise e

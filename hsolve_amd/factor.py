@@ -384,6 +384,14 @@ class GpuFactorization:
         self.sigma_held = None
         return True
 
+    def close(self) -> None:
+        """Frees the factorization object and its device storage now (hipFree waits for the
+        device, so solves free their objects when they end rather than leaving it to the garbage
+        collector, which could run in the middle of a later solve). Idempotent."""
+        self.__del__()
+        self._inertia = Inertia(self._inertia.neg, self._inertia.zero, self._inertia.pos, False)
+        self.sigma_held = None
+
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and h.value:

@@ -186,6 +186,7 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     rep["method"] = "shift_invert_lanczos"
     rep["complex_scalars"] = bool(pencil.is_complex)
     clock.pop()
+    own_fac = fac is None
     with clock.phase("factor"):
         if fac is None:
             fac = GpuFactorization(pencil)
@@ -262,6 +263,8 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     })
     rep["factor_stats"]["estimate"].update({"factor_entries": est["factor_entries"], "incore_mb": est["incore_mb"],
                                             "flops": est["flops"]})
+    if own_fac:
+        fac.close()  # the device storage is freed here, when the solve's work is done
     total = clock.total()
     rep["time_total_s"] = total
     for ph in clock.PHASES:

@@ -33,6 +33,7 @@ import numpy as np
 import torch
 
 from . import config as cfgmod
+from . import streams as stream_pool
 from .driver_core import (Block, Ctx, DefiniteRun, FactorHeld, PhaseClock, block_metrics, count_below,
                           factorize_near, keep_converged, lanczos_run, rayleigh_ritz_definite, refactorize_at,
                           try_refactorize_at, verify_and_rescue, TRUSTWORTHY)
@@ -468,11 +469,21 @@ class WindowSolver:
         others are clones over the same analysis, made once per solve and reused by every
         concurrent phase (counts, slices), each with a stream of its own."""
         main = self._main
-        if not self._slot_list:
-            self._slot_list.append({"fac": main.fac, "stream": torch.cuda.Stream(), "held": FactorHeld()})
         while len(self._slot_list) < k:
-            self._slot_list.append({"fac": main.fac.clone(), "stream": torch.cuda.Stream(), "held": FactorHeld()})
+            (st,) = stream_pool.checkout(1)
+            fac = main.fac if not self._slot_list else main.fac.clone()
+            self._slot_list.append({"fac": fac, "stream": st, "held": FactorHeld()})
         return self._slot_list[:k]
+
+    def release_slots(self):
+        """The end of the solve: the clones' factorizations are freed and the slots' streams go back
+        to the process pool (slot 0's factorization belongs to the caller)."""
+        torch.cuda.synchronize()
+        for s in self._slot_list[1:]:
+            s["fac"].close()
+        stream_pool.checkin([s["stream"] for s in self._slot_list])
+        for s in self._slot_list:
+            s["stream"] = None
 
     def _concurrently(self, njobs: int, job, phase: str):
         """Runs job(i) for i < njobs on min(njobs, streams) threads, one worker slot each, jobs handed
@@ -774,4 +785,8 @@ class WindowSolver:
 
 
 def solve_window(ctx: Ctx, streams: int = 1):
-    return WindowSolver(ctx, streams).solve()
+    w = WindowSolver(ctx, streams)
+    try:
+        return w.solve()
+    finally:
+        w.release_slots()
