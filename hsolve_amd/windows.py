@@ -588,6 +588,11 @@ class WindowSolver:
             remaining = b.nu - cur
             cur += (remaining + 1) // 2 if remaining <= 2 * c_target else c_target
             out.append(a.s + (b.s - a.s) * (cur - a.nu) / (b.nu - a.nu))
+        # where the density grows over the window the cuts hold fewer eigenvalues than their
+        # targets, more than c_max remain after the last one and plan() splits that stretch once
+        # more: its midpoint is counted ahead as well
+        if out and b.nu - cur > c_target // 2:
+            out.append(0.5 * (out[-1] + b.s))
         return out
 
     def deduplicate(self) -> int:
