@@ -33,17 +33,19 @@ int worker_count(int64_t items, int64_t min_items) {
   return std::min(16, static_cast<int>(std::max(1u, std::thread::hardware_concurrency())));
 }
 
-// body(i0, i1) over nt contiguous ranges of [0, n), the first on the calling thread.
+// body(t, i0, i1) for the nt contiguous ranges [i0, i1) of [0, n), range 0 on the calling thread.
 template <class F>
 void parallel_ranges(int32_t n, int nt, F&& body) {
+  auto range = [&](int t) {
+    body(t, static_cast<int32_t>(int64_t{n} * t / nt), static_cast<int32_t>(int64_t{n} * (t + 1) / nt));
+  };
   if (nt <= 1) {
-    body(0, n);
+    range(0);
     return;
   }
   std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t)
-    pool.emplace_back([&, t] { body(static_cast<int32_t>(int64_t{n} * t / nt), static_cast<int32_t>(int64_t{n} * (t + 1) / nt)); });
-  body(0, static_cast<int32_t>(int64_t{n} / nt));
+  for (int t = 1; t < nt; ++t) pool.emplace_back([&, t] { range(t); });
+  range(0);
   for (auto& th : pool) th.join();
 }
 
@@ -76,7 +78,7 @@ Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind) {
   g.ptr.assign(static_cast<size_t>(n) + 1, 0);
   const int nt = worker_count(n, 32768);
   std::vector<int64_t> kept(static_cast<size_t>(n) + 1, 0);
-  parallel_ranges(n, nt, [&](int32_t i0, int32_t i1) {
+  parallel_ranges(n, nt, [&](int, int32_t i0, int32_t i1) {
     for (int32_t i = i0; i < i1; ++i) {
       auto b = tmp.begin() + deg[i], e = tmp.begin() + deg[i + 1];
       std::sort(b, e);
@@ -85,7 +87,7 @@ Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind) {
   });
   for (int32_t i = 0; i < n; ++i) g.ptr[i + 1] = g.ptr[i] + kept[i + 1];
   g.adj.resize(static_cast<size_t>(g.ptr[n]));
-  parallel_ranges(n, nt, [&](int32_t i0, int32_t i1) {
+  parallel_ranges(n, nt, [&](int, int32_t i0, int32_t i1) {
     for (int32_t i = i0; i < i1; ++i)
       std::copy(tmp.begin() + deg[i], tmp.begin() + deg[i] + kept[i + 1], g.adj.begin() + g.ptr[i]);
   });
@@ -163,31 +165,48 @@ bool coarsen(const WGraph& g, WGraph& c, std::vector<int32_t>& cmap, Rng& rng) {
     if (first[cv] < 0) first[cv] = v;
     else second[cv] = v;
   }
-  c.ptr.assign(static_cast<size_t>(nc) + 1, 0);
-  c.adj.clear();
-  c.ew.clear();
-  c.adj.reserve(g.adj.size() / 2);
-  c.ew.reserve(g.adj.size() / 2);
-  std::vector<int32_t> slot(nc, -1);  // position of coarse neighbour in the current row
-  for (int32_t cv = 0; cv < nc; ++cv) {
-    const int64_t row_start = static_cast<int64_t>(c.adj.size());
-    for (int k = 0; k < 2; ++k) {
-      const int32_t v = k == 0 ? first[cv] : second[cv];
-      if (v < 0) continue;
-      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
-        const int32_t cu = cmap[g.adj[p]];
-        if (cu == cv) continue;
-        if (slot[cu] >= 0) {
-          c.ew[slot[cu]] += g.ew[p];
-        } else {
-          slot[cu] = static_cast<int32_t>(c.adj.size());
-          c.adj.push_back(cu);
-          c.ew.push_back(g.ew[p]);
+  // coarse rows are independent: built over ranges of coarse vertices, each range into vectors of
+  // its own thread (moved out at the end, so no two threads write near the same cache line), then
+  // concatenated in order
+  const int nt = std::min(8, worker_count(nc, 24576));
+  std::vector<std::vector<int32_t>> radj(nt), rew(nt);
+  std::vector<int64_t> rdeg(static_cast<size_t>(nc) + 1, 0);
+  parallel_ranges(nc, nt, [&](int t, int32_t b0, int32_t e0) {
+    std::vector<int32_t> A, W;
+    A.reserve(g.adj.size() / 2 / static_cast<size_t>(nt) + 16);
+    W.reserve(A.capacity());
+    std::vector<int32_t> slot(nc, -1);  // position of coarse neighbour in the current row
+    for (int32_t cv = b0; cv < e0; ++cv) {
+      const size_t row_start = A.size();
+      for (int k = 0; k < 2; ++k) {
+        const int32_t v = k == 0 ? first[cv] : second[cv];
+        if (v < 0) continue;
+        for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+          const int32_t cu = cmap[g.adj[p]];
+          if (cu == cv) continue;
+          if (slot[cu] >= 0) {
+            W[slot[cu]] += g.ew[p];
+          } else {
+            slot[cu] = static_cast<int32_t>(A.size());
+            A.push_back(cu);
+            W.push_back(g.ew[p]);
+          }
         }
       }
+      for (size_t q = row_start; q < A.size(); ++q) slot[A[q]] = -1;
+      rdeg[cv + 1] = static_cast<int64_t>(A.size() - row_start);
     }
-    for (int64_t p = row_start; p < static_cast<int64_t>(c.adj.size()); ++p) slot[c.adj[p]] = -1;
-    c.ptr[cv + 1] = static_cast<int64_t>(c.adj.size());
+    radj[t] = std::move(A);
+    rew[t] = std::move(W);
+  });
+  c.ptr.assign(static_cast<size_t>(nc) + 1, 0);
+  for (int32_t cv = 0; cv < nc; ++cv) c.ptr[cv + 1] = c.ptr[cv] + rdeg[cv + 1];
+  c.adj.resize(static_cast<size_t>(c.ptr[nc]));
+  c.ew.resize(static_cast<size_t>(c.ptr[nc]));
+  for (int t = 0; t < nt; ++t) {
+    const int32_t b0 = static_cast<int32_t>(int64_t{nc} * t / nt);
+    std::copy(radj[t].begin(), radj[t].end(), c.adj.begin() + c.ptr[b0]);
+    std::copy(rew[t].begin(), rew[t].end(), c.ew.begin() + c.ptr[b0]);
   }
   return true;
 }
