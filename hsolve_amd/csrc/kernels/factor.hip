@@ -1292,10 +1292,11 @@ Numeric::~Numeric() { free_all(); }
 void Numeric::free_all() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
                   (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
-                  (void*)d_pos_, (void*)d_rpos_})
+                  (void*)d_pos_, (void*)d_rpos_, (void*)d_zpull_})
     dfree(p);
   d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Xp_ = d_Y_ = d_U_ = d_pinv_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
   d_pos_ = d_rpos_ = nullptr;
+  d_zpull_ = nullptr;
   g_ready_ = false;
   d_ptype_ = nullptr;
   d_piv_ = d_ipiv_ = d_bperm_ = nullptr;
@@ -1422,6 +1423,8 @@ void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
   if (!nm.d_Gt_) hip_check(hipMalloc(&nm.d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
   if (!nm.d_T_) hip_check(hipMalloc(&nm.d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
   if (!nm.d_pos_) hip_check(hipMalloc(&nm.d_pos_, (size_t)sym_.n * sizeof(int32_t)), "hipMalloc (pivot positions)");
+  if (!nm.d_zpull_)
+    hip_check(hipMalloc(&nm.d_zpull_, (size_t)sym_.n * 2 * sizeof(int64_t)), "hipMalloc (pivot positions)");
   if (!nm.d_rpos_)
     hip_check(hipMalloc(&nm.d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t)), "hipMalloc (pivot positions)");
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
@@ -1445,7 +1448,8 @@ void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
     ++nl;
   }
   // pivot positions of the rows after this factorization's local pivoting
-  hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, nm.d_pos_);
+  hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, nm.d_pos_,
+                     (const int64_t*)d_cptr_, nm.d_zpull_);
   hipLaunchKernelGGL(k_row_positions, dim3(blocks_for(rows_total_)), dim3(256), 0, st, rows_total_,
                      (const int32_t*)d_rows_orig_, (const int32_t*)nm.d_pos_, nm.d_rpos_);
   nl += 2;
@@ -1488,16 +1492,16 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
                      (const double*)x, ldx, pos, Xp);
   int64_t tok = timing_begin(kPhaseForward, st);
   int64_t nl = 0;
-  // forward: gather z (in place in Xp), then [y1; u] = G z1 + [0; u] per front
+  // forward: [y1; u] = G z1 + [0; u_children] per front, z gathered from the children where read
+  const int64_t* cptr = (const int64_t*)d_cptr_;
+  const int32_t* cidx = (const int32_t*)d_cidx_;
   for (const Level& L : levels_) {
-    hipLaunchKernelGGL(k_fwd_gather<T>, dim3((L.max_m + kGatherRows - 1) / kGatherRows, L.n_all), dim3(kGatherRows), 0,
-                       st, S, L.d_all, p, Xp, U, rpos, (const int64_t*)d_cptr_, (const int32_t*)d_cidx_);
     hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
-                       (const T*)Xp, Y, U, part);
-    nl += 2;
+                       (const T*)Xp, Y, U, part, (const int64_t*)nm.d_zpull_, cptr, cidx);
+    ++nl;
     if (L.fwd_chunks > 1) {
       hipLaunchKernelGGL(k_fwd_reduce<T>, dim3(L.fwd_tiles, L.n_all), dim3(256), 0, st, S, L.d_all, p, L.fwd_chunks, Y,
-                         U, (const T*)part);
+                         U, (const T*)part, cptr, cidx);
       ++nl;
     }
   }

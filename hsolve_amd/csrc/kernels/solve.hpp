@@ -7,13 +7,15 @@
 //     G  = [ G11 ; G21 ] = [ L11^-1 ; -L21 L11^-1 ]   (m x ncol, column-major, the panel layout)
 //     Gt = G stored row-major                         (same offsets, element (i, j) at i*ncol + j)
 // because L^-1 restricted to one front is [L11^-1 0; -L21 L11^-1 I]. Then
-//   forward, bottom-up per level:  z = own right-hand-side rows + children's update vectors
-//                                  (pull gather, fixed order), then one GEMV per front:
-//                                  [ y1 ; u ] = G z1 + [ 0 ; z_cb ]
+//   forward, bottom-up per level:  one GEMV per front, [ y1 ; u ] = G z1 + [ 0 ; z_cb ], where z =
+//                                  own right-hand-side rows + children's update vectors, summed
+//                                  in a fixed order (pull map) where z1 is staged and where the
+//                                  contribution rows are written
 //   diagonal:                      y = D^-1 y (1x1 and 2x2 blocks)
 //   backward, top-down per level:  x1 = G^T [ y1 ; x_cb ] with x_cb the solution at the
 //                                  front's contribution rows (rows of ancestors, final).
-// Two launches per level forward, one backward, and no sequential chain inside a front.
+// One launch per level forward and one backward (each with a second one where long fronts split
+// their products), and no sequential chain inside a front.
 // Inside a solve the right-hand sides live in pivot numbering with the p columns interleaved
 // (Xp[pos * p + c], the same for Y and the update rows U[(front row) * p + c]): a front's rows are
 // contiguous and a gathered row is one contiguous run of p values. The caller's column-major X is
@@ -36,7 +38,6 @@ constexpr int kSolveBlock = 64;   // diagonal block of the first inversion step
 constexpr int kGvTile = 64;       // rows (forward) / columns (backward) per GEMV workgroup
 constexpr int kGvChunk = 128;     // vector entries staged in LDS per pass
 constexpr int kMaxRhsChunk = 8;   // right-hand sides per pass
-constexpr int kGatherRows = 256;  // rows per gather workgroup
 
 // Stages kn rows of right-hand-side values [c0, c0 + pc) into ws (zeros elsewhere): two threads
 // per row, four values each; row_ptr(t) gives the address of row t's value c0. All loads of a
@@ -52,6 +53,40 @@ __device__ __forceinline__ void stage_rows(T (*ws)[kMaxRhsChunk], int kn, int pc
   for (int c = 0; c < 4; ++c) v[c] = src[min(h + c, pc - 1)];
 #pragma unroll
   for (int c = 0; c < 4; ++c) ws[t][h + c] = (valid && h + c < pc) ? v[c] : (T)0;
+}
+
+// Stages kn rows of z for the pivot positions q0 + t: the right-hand side in Xp plus, in child
+// order, the children's update rows mapped onto that row (the forward gather, done where z is
+// read). Two threads per row, four values each; invalid rows read row q0 and add no children.
+template <class T>
+__device__ __forceinline__ void stage_gathered(T (*ws)[kMaxRhsChunk], int kn, int pc, int p, int c0, int64_t q0,
+                                               const T* __restrict__ Xp, const T* __restrict__ U,
+                                               const int64_t* __restrict__ zpull, const int32_t* __restrict__ cidx) {
+  const int t = threadIdx.x >> 1, h = (threadIdx.x & 1) * 4;
+  const bool valid = t < kn;
+  const int64_t q = q0 + (valid ? t : 0);
+  const int64_t b = zpull[2 * q], e = valid ? zpull[2 * q + 1] : b;
+  const T* src = Xp + q * p + c0;
+  T v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = src[min(h + c, pc - 1)];
+  for (int64_t i = b; i < e; ++i) {
+    const T* u = U + (int64_t)cidx[i] * p + c0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] += u[min(h + c, pc - 1)];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) ws[t][h + c] = (valid && h + c < pc) ? v[c] : (T)0;
+}
+
+// Sum, in child order, of value c of the children's update rows mapped onto front row r (index
+// rowoff + local row): the part of a contribution row's update that comes from below.
+template <class T>
+__device__ __forceinline__ T children_sum(int64_t r, int p, int c, const T* __restrict__ U,
+                                          const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
+  T acc = (T)0;
+  for (int64_t i = cptr[r]; i < cptr[r + 1]; ++i) acc += U[(int64_t)cidx[i] * p + c];
+  return acc;
 }
 
 // L(i, j) of a pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair (that slot of
@@ -198,12 +233,20 @@ __global__ void __launch_bounds__(256) k_transpose_g(DevSym S, const int32_t* __
 
 // Pivot position of every original row: pos[rows_orig[rowoff + r]] = first + ipiv[first + r] for
 // the pivot rows r < ncol of each front (one workgroup per front, all fronts).
+// With them, per pivot position q, the range [zpull[2q], zpull[2q + 1]) of cidx holding the
+// children's update rows of the front row pivoted to q (the forward product's pull map).
 template <class T>
-__global__ void __launch_bounds__(256) k_pivot_positions(DevSym S, DevNum<T> N, int32_t* __restrict__ pos) {
+__global__ void __launch_bounds__(256) k_pivot_positions(DevSym S, DevNum<T> N, int32_t* __restrict__ pos,
+                                                         const int64_t* __restrict__ cptr, int64_t* __restrict__ zpull) {
   const int s = blockIdx.x;
   const int ncol = S.sn_ncol[s], first = S.sn_first[s];
   const int64_t roff = S.sn_rowoff[s];
-  for (int r = threadIdx.x; r < ncol; r += blockDim.x) pos[S.rows_orig[roff + r]] = first + N.ipiv[first + r];
+  for (int r = threadIdx.x; r < ncol; r += blockDim.x) {
+    const int q = first + N.ipiv[first + r];
+    pos[S.rows_orig[roff + r]] = q;
+    zpull[2 * (int64_t)q] = cptr[roff + r];
+    zpull[2 * (int64_t)q + 1] = cptr[roff + r + 1];
+  }
 }
 
 // Pivot position of every front row (index rowoff + i): rpos = pos[rows_orig].
@@ -233,40 +276,6 @@ __global__ void k_permute_out(int n, int p, const T* __restrict__ Xp, const int3
   }
 }
 
-// z of each front of the level: a pivot row adds, to its right-hand side in Xp (in place), the
-// update rows of the children mapped onto it; a contribution row's U row becomes the sum of
-// those (cptr / cidx: per front row, the children's U rows, in child order). grid (row chunks,
-// fronts).
-template <class T>
-__global__ void __launch_bounds__(kGatherRows) k_fwd_gather(DevSym S, const int32_t* __restrict__ list, int p, T* Xp,
-                                                             T* U, const int32_t* __restrict__ rpos,
-                                                             const int64_t* __restrict__ cptr,
-                                                             const int32_t* __restrict__ cidx) {
-  const int s = list[blockIdx.y];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s];
-  const int loc = blockIdx.x * kGatherRows + threadIdx.x;
-  if (loc >= m) return;
-  const int64_t roff = S.sn_rowoff[s];
-  const int64_t q0 = cptr[roff + loc], q1 = cptr[roff + loc + 1];
-  const bool own = loc < ncol;
-  T* out = own ? Xp + (int64_t)rpos[roff + loc] * p : U + (roff + loc) * p;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
-    const int pc = min(kMaxRhsChunk, p - c0);
-    T v[kMaxRhsChunk];
-#pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c) v[c] = (own && c < pc) ? out[c0 + c] : (T)0;
-    for (int64_t q = q0; q < q1; ++q) {
-      const T* u = U + (int64_t)cidx[q] * p + c0;
-#pragma unroll
-      for (int c = 0; c < kMaxRhsChunk; ++c)
-        if (c < pc) v[c] += u[c];
-    }
-#pragma unroll
-    for (int c = 0; c < kMaxRhsChunk; ++c)
-      if (c < pc) out[c0 + c] = v[c];
-  }
-}
-
 // Split products: a tile's long dimension (columns of G forward, rows of Gt backward) is cut into
 // chunks, one workgroup each. With more than one chunk, each workgroup stores its partial sums
 // (64 x 8 per group of right-hand sides) in its slot of `part`, and a second launch adds the
@@ -276,14 +285,31 @@ __global__ void __launch_bounds__(kGatherRows) k_fwd_gather(DevSym S, const int3
 constexpr int kGvSplit = 256;
 constexpr int kPartSlot = kGvTile * kMaxRhsChunk;  // values per partial slot
 
-// [y1; u] = G z1 + [0; u] for a tile of 64 front rows and one chunk of G's columns: rows < ncol
-// (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution rows
-// add G21 z1 to the front's U rows. 256 threads = 64 rows x 4 column groups; z1 is staged through
-// LDS, G read along columns. grid (row tiles, fronts, column chunks).
+// src[0] + src[stride] + ... (n terms), added in that order; the loads are issued eight at a time.
+template <class T>
+__device__ __forceinline__ T sum_partials(const T* __restrict__ src, int64_t stride, int n) {
+  T v = (T)0;
+  for (int c0 = 0; c0 < n; c0 += 8) {
+    T x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = c0 + u < n ? src[(int64_t)(c0 + u) * stride] : (T)0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c0 + u < n) v += x[u];
+  }
+  return v;
+}
+
+// [y1; u] = G z1 + [0; u_children] for a tile of 64 front rows and one chunk of G's columns: rows
+// < ncol (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution
+// rows get the children's update rows plus G21 z1 in U. z1 (right-hand side plus the children's
+// update rows, gathered while staging) goes through LDS, G is read along columns. 256 threads =
+// 64 rows x 4 column groups. grid (row tiles, fronts, column chunks).
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict__ G, const int32_t* __restrict__ list,
                                                    int p, const T* __restrict__ Xp, T* __restrict__ Y, T* __restrict__ U,
-                                                   T* __restrict__ part) {
+                                                   T* __restrict__ part, const int64_t* __restrict__ zpull,
+                                                   const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
   __shared__ T zs[kGvChunk][kMaxRhsChunk];
   __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
   const int f = blockIdx.y;
@@ -300,7 +326,6 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
   const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
   const int i = r0 + lane;
   const bool rin = i < m;
-  const T* zf = Xp + (int64_t)first * p;
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
     const int pc = min(kMaxRhsChunk, p - c0);
@@ -312,7 +337,7 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
     for (int k0 = kbeg; k0 < kstop; k0 += kGvChunk) {
       const int kn = min(kGvChunk, kstop - k0);
       __syncthreads();
-      stage_rows<T>(zs, kn, pc, [&](int t) { return zf + (int64_t)(k0 + t) * p + c0; });
+      stage_gathered<T>(zs, kn, pc, p, c0, (int64_t)first + k0, Xp, U, zpull, cidx);
       __syncthreads();
       if (rin) {
         const T* gcol = Gs + (int64_t)k0 * m + i;
@@ -333,8 +358,9 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
       if (nchunk > 1) {
         part[slot * kPartSlot + c * kGvTile + lane] = v;
       } else if (rin) {
+        const int64_t r = S.sn_rowoff[s] + i;
         if (i < ncol) Y[(int64_t)(first + i) * p + c0 + c] = v;
-        else U[(S.sn_rowoff[s] + i) * p + c0 + c] += v;
+        else U[r * p + c0 + c] = children_sum<T>(r, p, c0 + c, U, cptr, cidx) + v;
       }
     }
     __syncthreads();
@@ -346,7 +372,8 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
 // grid (row tiles, fronts), chunks = the forward launch's grid.z.
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __restrict__ list, int p, int chunks,
-                                                     T* __restrict__ Y, T* __restrict__ U, const T* __restrict__ part) {
+                                                     T* __restrict__ Y, T* __restrict__ U, const T* __restrict__ part,
+                                                     const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
   const int f = blockIdx.y;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -362,13 +389,11 @@ __global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __r
     const int i = r0 + lane;
     if (i >= m) continue;
     const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
-    T v = (T)0;
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const int64_t slot = (((int64_t)f * gridDim.x + tile) * chunks + ch) * nrc + q;
-      v += part[slot * kPartSlot + cc * kGvTile + lane];
-    }
+    const int64_t slot0 = ((int64_t)f * gridDim.x + tile) * chunks * nrc + q;
+    const T v = sum_partials<T>(part + slot0 * kPartSlot + cc * kGvTile + lane, (int64_t)nrc * kPartSlot, nchunk);
+    const int64_t r = S.sn_rowoff[s] + i;
     if (i < ncol) Y[(int64_t)(first + i) * p + c] = v;
-    else U[(S.sn_rowoff[s] + i) * p + c] += v;
+    else U[r * p + c] = children_sum<T>(r, p, c, U, cptr, cidx) + v;
   }
 }
 
@@ -481,8 +506,8 @@ __global__ void __launch_bounds__(256) k_bwd_reduce(DevSym S, const BwdTask* __r
   for (int idx = threadIdx.x; idx < tk.jn * p; idx += blockDim.x) {
     const int j2 = idx % tk.jn, c = idx / tk.jn;
     const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
-    T v = (T)0;
-    for (int ch = 0; ch < tk.nchunk; ++ch) v += part[((tk.slot0 + ch) * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + j2];
+    const T v = sum_partials<T>(part + (tk.slot0 * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + j2,
+                                (int64_t)nrc * kPartSlot, tk.nchunk);
     Xp[(int64_t)(first + tk.j0 + j2) * p + c] = v;
   }
 }
