@@ -1143,18 +1143,33 @@ void Engine::plan_solve() {
   }
   n_ttiles_ = (int)(h_ttiles_.size() / 3);
   // backward tasks per level: column tiles of at most 64 (width rounded up to a power of two >= 8:
-  // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group
+  // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group,
+  // or of 16 where the level has split tiles already and fewer than 1024 tasks (a task's time is
+  // mostly its chain of dependent loads: shorter chunks shorten it, at the cost of more partial
+  // sums; a level without splits would also gain a reduction launch)
+  auto tile_width = [](int jn) {
+    int jw = 8;
+    while (jw < jn) jw *= 2;
+    return jw;
+  };
   for (Level& L : levels_) {
     L.bt_begin = (int64_t)h_btasks_.size();
     L.bh_begin = (int64_t)h_bheads_.size();
-    int64_t slots = 0;
+    int64_t slots = 0, tasks32 = 0, splits32 = 0;
+    for (int s : L.h_all)
+      for (int j0 = 0; j0 < sym_.sn_ncol[s]; j0 += kGvTile) {
+        const int rc = 32 * (256 / tile_width(std::min(kGvTile, sym_.sn_ncol[s] - j0)));
+        const int64_t nch = (sym_.sn_m[s] - j0 + rc - 1) / rc;
+        tasks32 += nch;
+        splits32 += nch > 1;
+      }
+    const int rows_per_group = splits32 > 0 && tasks32 < 1024 ? 16 : 32;
     for (int s : L.h_all) {
       const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
       for (int j0 = 0; j0 < ncol; j0 += kGvTile) {
         const int jn = std::min(kGvTile, ncol - j0);
-        int jw = 8;
-        while (jw < jn) jw *= 2;
-        const int rc = 32 * (256 / jw);
+        const int jw = tile_width(jn);
+        const int rc = rows_per_group * (256 / jw);
         const int rows = m - j0;
         const int nchunk = (rows + rc - 1) / rc;
         BwdTask t{};
