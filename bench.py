@@ -35,10 +35,10 @@ import time
 import numpy as np
 
 # The chosen form runs up to 8 spectrum slices and counts on concurrent streams; HIP maps streams
-# onto GPU_MAX_HW_QUEUES hardware queues (default 4), and streams that share a queue run one after
-# the other. This process owns the device, so it asks for 8 queues before HIP initializes (an
-# explicit setting in the environment is kept).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# onto GPU_MAX_HW_QUEUES hardware queues (HIP's default is 4), and streams that share a queue run
+# one after the other. This process owns the device, so it sets 8 queues before HIP initializes
+# (HSG_HW_QUEUES overrides the number; the value used is reported in the JSON line).
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("HSG_HW_QUEUES", "8")
 
 METRIC = "certified eigpairs/s, GPU chosen vs plain @bwd err≤1e-10; p50/p99 latency"
 UNIT = "eigpairs/s"

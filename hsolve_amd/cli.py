@@ -114,8 +114,8 @@ def print_summary(r: dict) -> None:
 
 def main(argv=None) -> int:
     # concurrent spectrum slices need distinct hardware queues (HIP's default is 4); read when HIP
-    # initializes, so it applies when this command owns the process (an explicit setting is kept)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+    # initializes, so it applies when this command owns the process (HSG_HW_QUEUES overrides 8)
+    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("HSG_HW_QUEUES", "8")
     ap = _parser()
     try:
         args = ap.parse_args(argv)
