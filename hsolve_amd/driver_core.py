@@ -396,7 +396,9 @@ def lanczos_run(ctx: Ctx, sigma: float, which: str, nev: int, p: int, deflation:
                        tol=cfg["tol"] * cfg["inner_tol_factor"], seed=cfg["seed"] + run.restarts,
                        block_size=max(p, 1), lead_nev=lead if lead < nev else 0)
     with ctx.clock.phase("iterate"):
-        res = lanczos_krylov_schur(ctx.ops, op.apply, which, ko, deflation)
+        # without refinement an application only enqueues kernels: the next one may be launched
+        # ahead of the host's Rayleigh-Ritz probe
+        res = lanczos_krylov_schur(ctx.ops, op.apply, which, ko, deflation, prefetch=op.ref.max_steps == 0)
         # operator applications are counted by the operator object
         res.op_applies = op.applications
     run.restarts += res.restarts

@@ -133,9 +133,16 @@ class RitzPairs:
 class BlockLanczos:
     """One shift-invert Krylov-Schur run; `op(x)` returns Op x for a (p, n) block."""
 
-    def __init__(self, ops: DeviceOps, op, which: str, opt: KrylovOptions, deflation: torch.Tensor | None = None):
+    def __init__(self, ops: DeviceOps, op, which: str, opt: KrylovOptions, deflation: torch.Tensor | None = None,
+                 prefetch: bool = False):
         self.ops = ops
         self.op = op
+        # the object counting applications (op is usually a bound method of the operator)
+        self._op_obj = getattr(op, "__self__", op)
+        # prefetch: op only enqueues device work (no host synchronization), so the image of the
+        # next block can be launched before the host's Rayleigh-Ritz probe and dropped if the
+        # probe ends the run
+        self.prefetch = prefetch
         self.which = which
         self.opt = opt
         n = ops.n
@@ -289,12 +296,17 @@ class BlockLanczos:
         self._orthonormalize(d, blk, START_MIN_RATIO)
 
     # ------------------------------------------------------------ steps
+    def _applications(self) -> int:
+        return getattr(self._op_obj, "applications", 0)
+
     def _expand(self, first: int) -> bool:
         d, p = self.d, self.p
+        pending = None  # Op of the block just orthonormalized, launched ahead of the probe
         for j in range(first, self.ncv, p):
             src = self.V[d + j:d + j + p]
             dst = self.V[d + j + p:d + j + 2 * p]
-            dst.copy_(self.op(src))
+            dst.copy_(pending if pending is not None else self.op(src))
+            pending = None
             coef, R = self._orthonormalize(d + j + p, dst, BREAKDOWN_TOL)
             A = coef[d + j:d + j + p, :]
             A = 0.5 * (A + A.conj().T)
@@ -306,8 +318,13 @@ class BlockLanczos:
             self.rlast = np.triu(R)
             threshold = self.first_probe if self.restarts == 0 else self.keep + p
             if self.m < self.ncv and self.m >= threshold:
+                if self.prefetch:  # the next step's source is dst, final now
+                    pending = self.op(dst)
                 self._rayleigh_ritz()
                 if self.rr.done(self.nev, self.lead):
+                    discard = getattr(self._op_obj, "discard", None)
+                    if pending is not None and discard is not None:
+                        discard(pending)  # launched but not used
                     return True
         return False
 
@@ -343,13 +360,13 @@ class BlockLanczos:
             self.H[l, k:k + p] = b[:, l].conj()
 
     def run(self) -> EigResult:
-        self.applies0 = getattr(self.op, "applications", 0)
+        self.applies0 = self._applications()
         self._set_start_block()
         first = 0
         while True:
             if not self._expand(first):
                 self._rayleigh_ritz()
-            applies = getattr(self.op, "applications", 0) - self.applies0
+            applies = self._applications() - self.applies0
             self.history.append(self.rr.record(self.restarts, self.nev, applies))
             if self.rr.done(self.nev, self.lead) or self.restarts >= self.opt.max_restarts:
                 break
@@ -370,10 +387,10 @@ class BlockLanczos:
         nconv = int(np.count_nonzero(rr.conv[sel]))
         return EigResult(theta=rr.theta[sel].copy(), estimate=rr.est[sel].copy(), vectors=vec,
                          ritz_all=rr.theta.copy(), nconv=nconv, converged=nconv >= nev, restarts=self.restarts,
-                         op_applies=getattr(self.op, "applications", 0) - self.applies0, history=self.history)
+                         op_applies=self._applications() - self.applies0, history=self.history)
 
 
 def lanczos_krylov_schur(ops: DeviceOps, op, which: str, opt: KrylovOptions,
-                         deflation: torch.Tensor | None = None) -> EigResult:
+                         deflation: torch.Tensor | None = None, prefetch: bool = False) -> EigResult:
     """Entry point (src/eigen/lanczos_krylov_schur.cpp:617)."""
-    return BlockLanczos(ops, op, which, opt, deflation).run()
+    return BlockLanczos(ops, op, which, opt, deflation, prefetch).run()

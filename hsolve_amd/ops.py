@@ -211,6 +211,15 @@ class ShiftInvertOperator:
         self.last_converged = True
         self._factorizations = factor.stats.factorizations
 
+    def discard(self, y: torch.Tensor) -> None:
+        """Undoes the counts of an application whose result goes unused (launched ahead of a host
+        decision that then ended the run), so that the counters report the applications the
+        algorithm used."""
+        p = y.shape[0]
+        self.applications -= p
+        self.factor.stats.solves -= 1
+        self.factor.stats.rhs_solved -= p
+
     def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
         if self.factor.stats.factorizations != self._factorizations:
             from .errors import InternalError
