@@ -108,7 +108,7 @@ class Engine {
   EngineOptions opt_;
   std::vector<int64_t> h_asm_, h_ia_, h_ib_, h_w_off_;
   // solve phase plan: diagonal 64-block pairs, tile-product stages building G, transpose tiles,
-  // and the pull map of the forward gather (per front row, the children's U rows mapped onto it)
+  // and the pull map of the forward products (per front row, the children's U rows mapped onto it)
   std::vector<int32_t> h_pairs_, h_ttiles_;
   std::vector<GemmTask> h_tasks_;
   std::vector<BwdTask> h_btasks_, h_bheads_;

@@ -1193,7 +1193,7 @@ void Engine::plan_solve() {
     L.bh_count = (int64_t)h_bheads_.size() - L.bh_begin;
     part_slots_ = std::max(part_slots_, slots);
   }
-  // pull map of the forward gather: for front row r (index rowoff + r), the children's U rows
+  // pull map of the forward products: for front row r (index rowoff + r), the children's U rows
   // mapped onto it, children in child-list order
   h_cptr_.assign(rows_total_ + 1, 0);
   for (int s = 0; s < nsn; ++s)

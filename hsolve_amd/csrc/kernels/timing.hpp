@@ -13,7 +13,7 @@ namespace hsg {
 enum TimedPhase : int {
   kPhaseFactor = 0,    // numeric factorization (assembly, fronts, panels, updates): flops
   kPhaseInverse = 1,   // inverse panels G, Gt from L (first solve after a factorization): flops
-  kPhaseForward = 2,   // forward sweep of a solve (gathers + products): bytes
+  kPhaseForward = 2,   // forward sweep of a solve (products with their gathers, reductions): bytes
   kPhaseBackward = 3,  // backward sweep of a solve (transposed products): bytes
   kPhaseCount = 4
 };
