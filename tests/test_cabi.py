@@ -47,3 +47,19 @@ def test_version_and_error_channel():
     assert lib.hsg_version().decode().count(".") == 2
     assert lib.hsg_timing_read(99, None, None, None, None) == 1  # invalid phase -> HSG_INVALID_INPUT
     assert b"phase" in lib.hsg_last_error_message()
+
+
+def test_binding_keeps_the_gil_only_across_launch_only_calls():
+    # long entry points (analysis, factorization, solve, clone, frees) release the GIL; the
+    # launch-only ones keep it (a thread that released it for microseconds could wait a whole
+    # interpreter switch interval to get it back while other slice threads run Python)
+    import ctypes as C
+    from hsolve_amd import _lib
+    assert _lib.RELEASES_GIL <= set(_lib.SIGNATURES)
+    assert {"hsg_analyze", "hsg_factorize", "hsg_solve"} <= _lib.RELEASES_GIL
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libhsg.so not built")
+    b = _lib.Bound(_lib.LIB_PATH)
+    for name in _lib.SIGNATURES:
+        keeps = bool(getattr(b, name)._flags_ & C._FUNCFLAG_PYTHONAPI)
+        assert keeps == (name not in _lib.RELEASES_GIL), name

@@ -139,3 +139,43 @@ def test_concurrent_slices_match_sequential(gpu):
     np.testing.assert_allclose(l3, l1, rtol=1e-9, atol=0)
     out = ev.evaluate(A, None, l3, v3, cfg["target"], ref, ref_complete=True)
     assert out["passed"], out
+
+
+def test_krylov_prefetch_changes_nothing_but_the_timing(gpu):
+    # launching the next block's operator image ahead of the host's Rayleigh-Ritz probe gives the
+    # same Ritz values and vectors bit for bit, and the same counts of applications and solves
+    # (an image launched but unused is not counted)
+    import torch
+    from hsolve_amd.factor import GpuFactorization, Pencil
+    from hsolve_amd.krylov import KrylovOptions, LARGEST_MAGNITUDE, lanczos_krylov_schur
+    from hsolve_amd.ops import DeviceOps, ShiftInvertOperator
+    A, B, g = yee.jittered_box(cells=(10, 6, 8), lengths=(2 * np.pi, np.pi, 1.5 * np.pi), seed=3)
+    pencil = Pencil(A, B)
+    ops = DeviceOps(pencil)
+    out = []
+    for prefetch in (False, True):
+        F = GpuFactorization(pencil)
+        assert F.factorize(1.0, -12.3)
+        op = ShiftInvertOperator(F, ops, 12.3)
+        opt = KrylovOptions(nev=40, ncv=0, max_restarts=50, tol=1e-11, seed=2026, block_size=8)
+        res = lanczos_krylov_schur(ops, op.apply, LARGEST_MAGNITUDE, opt, prefetch=prefetch)
+        torch.cuda.synchronize()
+        out.append((res.theta, res.vectors.cpu().numpy(), op.applications, F.stats.solves, F.stats.rhs_solved,
+                    [h["op_applies"] for h in res.history]))
+        F.close()
+    (t0, v0, a0, s0, r0, h0), (t1, v1, a1, s1, r1, h1) = out
+    assert np.array_equal(t0, t1) and np.array_equal(v0, v1)
+    assert (a0, s0, r0) == (a1, s1, r1) and h0 == h1
+    assert a0 > 0 and h0[-1] == a0  # the history counts the operator's applications
+
+
+def test_stream_pool_reuses_streams_and_hands_out_distinct_ones(gpu):
+    from hsolve_amd import streams
+    first = streams.checkout(3)
+    other = streams.checkout(2)  # held at the same time: none shared
+    assert len({id(s) for s in first + other}) == 5
+    streams.checkin(other)
+    streams.checkin(first)
+    again = streams.checkout(3)
+    assert {id(s) for s in again} <= {id(s) for s in first + other}
+    streams.checkin(again)
