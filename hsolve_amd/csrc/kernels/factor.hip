@@ -20,7 +20,10 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <list>
+#include <mutex>
 #include <stdexcept>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -981,17 +984,129 @@ void hip_check(hipError_t e, const char* what) {
 
 namespace {
 
+// Device blocks of the analysis and of factorization objects are kept for reuse instead of being
+// returned to the driver: repeated solves of a pattern allocate the same sizes, and after a few
+// hundred GB of hipMalloc / hipFree cycles the driver's calls took up to seconds each. A cached
+// block is handed out again only for exactly its size. Freeing synchronizes the device first (as
+// hipFree does), so no queued kernel still uses a block when a new owner receives it. At most
+// kCacheCap idle bytes stay cached per device (the longest-idle blocks go back first); when
+// hipMalloc runs out of memory, the device's idle blocks are returned and the call is retried.
+class BlockCache {
+ public:
+  void* alloc(size_t bytes, const char* what) {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), what);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto it = idle_.begin(); it != idle_.end(); ++it)
+        if (it->dev == dev && it->bytes == bytes) {
+          void* p = it->ptr;
+          idle_bytes_[dev] -= bytes;
+          idle_.erase(it);
+          live_[p] = {dev, bytes};
+          return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      release_idle(dev);
+      e = hipMalloc(&p, bytes);
+    }
+    hip_check(e, what);
+    std::lock_guard<std::mutex> lk(mu_);
+    live_[p] = {dev, bytes};
+    return p;
+  }
+  void free(void* p) {
+    Block b;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = live_.find(p);
+      if (it == live_.end()) return;
+      b = {it->second.dev, it->second.bytes, p};
+      live_.erase(it);
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != b.dev) (void)hipSetDevice(b.dev);
+    (void)hipDeviceSynchronize();
+    std::vector<void*> evict;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      idle_.push_back(b);
+      idle_bytes_[b.dev] += b.bytes;
+      for (auto it = idle_.begin(); it != idle_.end() && idle_bytes_[b.dev] > kCacheCap;) {
+        if (it->dev == b.dev) {
+          idle_bytes_[b.dev] -= it->bytes;
+          evict.push_back(it->ptr);
+          it = idle_.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (void* q : evict) (void)hipFree(q);
+    if (cur != b.dev) (void)hipSetDevice(cur);
+  }
+
+ private:
+  struct Block {
+    int dev = 0;
+    size_t bytes = 0;
+    void* ptr = nullptr;
+  };
+  struct Owner {
+    int dev;
+    size_t bytes;
+  };
+  static constexpr size_t kCacheCap = size_t(128) << 30;
+  void release_idle(int dev) {
+    std::vector<void*> out;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto it = idle_.begin(); it != idle_.end();) {
+        if (it->dev == dev) {
+          out.push_back(it->ptr);
+          it = idle_.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      idle_bytes_[dev] = 0;
+    }
+    (void)hipDeviceSynchronize();
+    for (void* q : out) (void)hipFree(q);
+  }
+  std::mutex mu_;
+  std::unordered_map<void*, Owner> live_;
+  std::list<Block> idle_;  // oldest first
+  std::unordered_map<int, size_t> idle_bytes_;
+};
+
+BlockCache& block_cache() {
+  static BlockCache* c = new BlockCache();  // never destroyed: frees may run during process exit
+  return *c;
+}
+
+void* dalloc(size_t bytes, const char* what) { return block_cache().alloc(std::max<size_t>(1, bytes), what); }
+
+template <class P>
+void dalloc_into(P*& p, size_t bytes, const char* what) {
+  p = static_cast<P*>(dalloc(bytes, what));
+}
+
 template <class V>
 V* upload(const std::vector<V>& h) {
-  V* d = nullptr;
   const size_t bytes = std::max<size_t>(1, h.size()) * sizeof(V);
-  hip_check(hipMalloc(&d, bytes), "hipMalloc (analysis)");
+  V* d = static_cast<V*>(dalloc(bytes, "hipMalloc (analysis)"));
   if (!h.empty()) hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice), "hipMemcpy (analysis)");
   return d;
 }
 
 void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (p) block_cache().free(p);
 }
 
 int blocks_for(int64_t n, int per = 256, int cap = 65536) {
@@ -1328,19 +1443,19 @@ void Engine::alloc_numeric(Numeric& nm, int precision) {
   nm.free_all();
   const size_t ts = precision == 0 ? sizeof(double) : sizeof(float);
   const int64_t n = sym_.n;
-  hip_check(hipMalloc(&nm.d_L_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (factor panels)");
-  hip_check(hipMalloc(&nm.d_CB_, std::max<int64_t>(1, cb_total_) * ts), "hipMalloc (contribution blocks)");
-  hip_check(hipMalloc(&nm.d_W_, std::max<int64_t>(1, w_total_) * ts), "hipMalloc (panel workspace)");
-  hip_check(hipMalloc(&nm.d_dval_, n * ts), "hipMalloc (D)");
-  hip_check(hipMalloc(&nm.d_doff_, n * ts), "hipMalloc (D)");
-  hip_check(hipMalloc(&nm.d_ptype_, n), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&nm.d_piv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&nm.d_ipiv_, n * sizeof(int32_t)), "hipMalloc (pivots)");
-  hip_check(hipMalloc(&nm.d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t)),
-            "hipMalloc (pivots)");
-  hip_check(hipMalloc(&nm.d_cnt_, kCntSlots * sizeof(unsigned long long)), "hipMalloc (counters)");
-  hip_check(hipMalloc(&nm.d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts),
-            "hipMalloc (panel inverses)");
+  dalloc_into(nm.d_L_, std::max<int64_t>(1, panel_total_) * ts, "hipMalloc (factor panels)");
+  dalloc_into(nm.d_CB_, std::max<int64_t>(1, cb_total_) * ts, "hipMalloc (contribution blocks)");
+  dalloc_into(nm.d_W_, std::max<int64_t>(1, w_total_) * ts, "hipMalloc (panel workspace)");
+  dalloc_into(nm.d_dval_, n * ts, "hipMalloc (D)");
+  dalloc_into(nm.d_doff_, n * ts, "hipMalloc (D)");
+  dalloc_into(nm.d_ptype_, n, "hipMalloc (pivots)");
+  dalloc_into(nm.d_piv_, n * sizeof(int32_t), "hipMalloc (pivots)");
+  dalloc_into(nm.d_ipiv_, n * sizeof(int32_t), "hipMalloc (pivots)");
+  dalloc_into(nm.d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t),
+              "hipMalloc (pivots)");
+  dalloc_into(nm.d_cnt_, kCntSlots * sizeof(unsigned long long), "hipMalloc (counters)");
+  dalloc_into(nm.d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts,
+              "hipMalloc (panel inverses)");
   nm.precision_ = precision;
 }
 
@@ -1434,14 +1549,15 @@ InertiaResult Engine::factorize(Numeric& nm, double alpha, double beta, const do
 template <class T>
 void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
   const size_t ts = sizeof(T);
-  if (!nm.d_G_) hip_check(hipMalloc(&nm.d_G_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
-  if (!nm.d_Gt_) hip_check(hipMalloc(&nm.d_Gt_, std::max<int64_t>(1, panel_total_) * ts), "hipMalloc (inverse panels)");
-  if (!nm.d_T_) hip_check(hipMalloc(&nm.d_T_, std::max<int64_t>(1, t_total_) * ts), "hipMalloc (inverse scratch)");
-  if (!nm.d_pos_) hip_check(hipMalloc(&nm.d_pos_, (size_t)sym_.n * sizeof(int32_t)), "hipMalloc (pivot positions)");
+  if (!nm.d_G_) dalloc_into(nm.d_G_, std::max<int64_t>(1, panel_total_) * ts, "hipMalloc (inverse panels)");
+  if (!nm.d_Gt_) dalloc_into(nm.d_Gt_, std::max<int64_t>(1, panel_total_) * ts, "hipMalloc (inverse panels)");
+  if (!nm.d_T_) dalloc_into(nm.d_T_, std::max<int64_t>(1, t_total_) * ts, "hipMalloc (inverse scratch)");
+  if (!nm.d_pos_) dalloc_into(nm.d_pos_, (size_t)sym_.n * sizeof(int32_t), "hipMalloc (pivot positions)");
   if (!nm.d_zpull_)
-    hip_check(hipMalloc(&nm.d_zpull_, (size_t)sym_.n * 2 * sizeof(int64_t)), "hipMalloc (pivot positions)");
+    dalloc_into(nm.d_zpull_, (size_t)sym_.n * 2 * sizeof(int64_t), "hipMalloc (pivot positions)");
   if (!nm.d_rpos_)
-    hip_check(hipMalloc(&nm.d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t)), "hipMalloc (pivot positions)");
+    dalloc_into(nm.d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t),
+                "hipMalloc (pivot positions)");
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
@@ -1483,12 +1599,12 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
     for (void* q : {nm.d_Xp_, nm.d_Y_, nm.d_U_, nm.d_part_}) dfree(q);
     nm.d_Xp_ = nm.d_Y_ = nm.d_U_ = nm.d_part_ = nullptr;
     const int64_t nrc = (nrhs + kMaxRhsChunk - 1) / kMaxRhsChunk;
-    hip_check(hipMalloc(&nm.d_Xp_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&nm.d_Y_, (size_t)sym_.n * nrhs * sizeof(T)), "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&nm.d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T)),
-              "hipMalloc (solve workspace)");
-    hip_check(hipMalloc(&nm.d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T)),
-              "hipMalloc (solve workspace)");
+    dalloc_into(nm.d_Xp_, (size_t)sym_.n * nrhs * sizeof(T), "hipMalloc (solve workspace)");
+    dalloc_into(nm.d_Y_, (size_t)sym_.n * nrhs * sizeof(T), "hipMalloc (solve workspace)");
+    dalloc_into(nm.d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T),
+                "hipMalloc (solve workspace)");
+    dalloc_into(nm.d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T),
+                "hipMalloc (solve workspace)");
     nm.ws_cols_ = nrhs;
   }
   T* part = (T*)nm.d_part_;
