@@ -47,7 +47,6 @@ struct Numeric {
   int32_t *d_pos_ = nullptr, *d_rpos_ = nullptr;  // pivot position of each original row / front row
   int64_t* d_zpull_ = nullptr;  // per pivot position: the range of its row's children update rows in cidx
   void* d_part_ = nullptr;       // partial sums of split product tiles
-  void* d_pinv_ = nullptr;  // panel inverses of the large fronts of one level (panel solve)
   bool g_ready_ = false;
 };
 

@@ -502,11 +502,9 @@ constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
 template <class T>
 __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                            int p, int nb, int32_t* __restrict__ bperm, double null_tol,
-                                                           int detect, T* __restrict__ Linv) {
-  __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D);
-                                // after the write-back, the inverse of L11 (Xi)
+                                                           int detect) {
+  __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D)
   __shared__ T Lm[64 * kBkLd];  // multipliers of the pivot columns (strictly lower L11)
-  T* const Xi = F;
   __shared__ int perm[64];
   __shared__ int8_t ptl[64];
   __shared__ int8_t pairc[64];  // 1: column j starts a 2x2 step, F(j + 1, j) belongs to D
@@ -740,33 +738,6 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     bperm[(int64_t)f * nb + lane] = perm[lane];
     ptl[lane] = my_pt;
   }
-  // ---- inverse of the unit lower L11 (2x2 / null pair couplings are not part of L), outer-product
-  //      form: X = I, then for t = 0.. every row i > t does X(i, 0:t) -= L(i, t) X(t, 0:t); wave w
-  //      takes the columns j = w (mod 4), row t is read into registers first; every lane stores all
-  //      of its columns (rows of X are written only by the wave that owns the column). X takes F's
-  //      storage once every wave has written its part of the block back.
-  __syncthreads();
-  for (int j = wv; j < w; j += kBkWaves) Xi[lane + j * kBkLd] = lane == j ? (T)1 : (T)0;
-  __syncthreads();
-  for (int t = 0; t + 1 < w; ++t) {
-    T rt[kBkCols], old[kBkCols];
-    T* xrow = Xi + lane;
-#pragma unroll
-    for (int q = 0; q < kBkCols; ++q) {
-      rt[q] = Xi[t + (wv + kBkWaves * q) * kBkLd];
-      old[q] = xrow[(wv + kBkWaves * q) * kBkLd];
-    }
-    const bool pair = lane == t + 1 && (ptl[t] == kPiv2First || ptl[t] == kPivNull);
-    const T ft = Lm[lane + t * kBkLd];
-    const T lit = (lane > t && lane < w && !pair) ? ft : (T)0;
-    // row t of X vanishes beyond column t, so columns j > t keep their values
-#pragma unroll
-    for (int q = 0; q < kBkCols; ++q) xrow[(wv + kBkWaves * q) * kBkLd] = old[q] - lit * rt[q];
-    __syncthreads();
-  }
-  T* out = Linv + (int64_t)f * nb * nb;
-  for (int j = wv; j < w; j += kBkWaves)
-    if (lane < w) out[(int64_t)j * w + lane] = Xi[lane + j * kBkLd];
   if (lead && lane == 0) {
     unsigned long long* cnt = N.cnt;
     if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
@@ -780,13 +751,12 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
 constexpr int kPanelRows = 64;  // rows per panel-solve workgroup
 
 // Panel solve below the diagonal block: for rows r in [c0 + w, m) of the panel columns
-// [c0, c0 + w): apply the block's column permutation, W = A21 L11^-T (stored in the W workspace),
-// L21 = W D^-1 (stored in the panel). Workgroups beyond the row chunks permute rows
+// [c0, c0 + w): apply the block's column permutation, W = A21 L11^-T by forward substitution with
+// the block's L11 (W is stored in the W workspace), L21 = W D^-1 (stored in the panel). Workgroups beyond the row chunks permute rows
 // [c0, c0 + w) of the earlier panel columns [0, c0).
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                         int p, int nb, const int32_t* __restrict__ bperm,
-                                                         const T* __restrict__ Linv) {
+                                                         int p, int nb, const int32_t* __restrict__ bperm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int f = blockIdx.y;
   const int s = list[f];
@@ -816,14 +786,18 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
   }
   const int r0 = c1 + blockIdx.x * kPanelRows;
   const int rn = min(kPanelRows, m - r0);
-  T* Li = reinterpret_cast<T*>(smem_raw);   // w x w inverse of L11, column-major
-  T* X = Li + nb * nb;                       // rn x w, ld kPanelRows
+  T* Lb = reinterpret_cast<T*>(smem_raw);   // strictly lower L11 of the block, column-major, ld nb
+  T* X = Lb + nb * nb;                       // rn x w, ld kPanelRows
   __shared__ T dinv[3 * 64];
   __shared__ int8_t ptl[64];
-  const T* lsrc = Linv + (int64_t)f * nb * nb;
-  for (int idx = tid; idx < w * w; idx += blockDim.x) Li[idx] = lsrc[idx];
   for (int i = tid; i < w; i += blockDim.x) ptl[i] = N.ptype[first + c0 + i];
   __syncthreads();
+  // the (j + 1, j) slot of a 2x2 or null pivot pair holds D's coupling, not L
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int i = idx % w, j = idx / w;
+    const bool coupling = i == j + 1 && (ptl[j] == kPiv2First || ptl[j] == kPivNull);
+    Lb[i + j * nb] = (i > j && !coupling) ? P[(int64_t)(c0 + j) * m + c0 + i] : (T)0;
+  }
   if (tid < w) {
     const int j = tid;
     const int8_t t = ptl[j];
@@ -847,22 +821,39 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
     X[r + j * kPanelRows] = P[(int64_t)(c0 + bp[j]) * m + r0 + r];
   }
   __syncthreads();
-  // W = A21 inv(L11)^T: W(r, j) = sum_{t <= j} A21(r, t) inv(L11)(j, t); 4 groups of columns
-  T Wr[16];
-  const int r = tid % kPanelRows, jg = tid / kPanelRows;
+  // W = A21 L11^-T by forward substitution along each row: W(r, j) = X(r, j) - sum_{t < j}
+  // L11(j, t) W(r, t). Four adjacent lanes per row, lane `part` holding W(r, t) for t = part
+  // (mod 4) in registers; their partial sums are added by lane exchanges (the same total in all
+  // four lanes), so the only dependent chain is the row's 64 steps, without barriers.
+  {
+    static_assert(kPanelRows * 4 == kThreads, "panel solve layout: four lanes per row");
+    const int rr = tid >> 2, part = tid & 3;
+    const bool rin = rr < rn;
+    T wv[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int j = jg + 4 * q;
-    T acc = (T)0;
-    if (j < w && r < rn)
-      for (int t = 0; t <= j; ++t) acc += X[r + t * kPanelRows] * Li[j + t * w];
-    Wr[q] = acc;
-  }
-  __syncthreads();
+    for (int u = 0; u < 16; ++u) wv[u] = (T)0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int j = jg + 4 * q;
-    if (j < w && r < rn) X[r + j * kPanelRows] = Wr[q];
+    for (int j = 0; j < 64; ++j) {
+      if (j < w) {
+        T acc = (T)0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          if (4 * u < j) {
+            const int t = part + 4 * u;
+            if (t < j) acc += Lb[j + t * nb] * wv[u];
+          }
+        }
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        const T wj = (rin ? X[rr + j * kPanelRows] : (T)0) - acc;
+        if (part == (j & 3)) wv[j >> 2] = wj;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int t = part + 4 * u;
+      if (rin && t < w) X[rr + t * kPanelRows] = wv[u];
+    }
   }
   __syncthreads();
   T* Wp = N.W + S.w_off[s];
@@ -1421,10 +1412,10 @@ Numeric::~Numeric() { free_all(); }
 
 void Numeric::free_all() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
-                  (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_pinv_, d_G_, d_Gt_, d_T_, d_part_,
+                  (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_G_, d_Gt_, d_T_, d_part_,
                   (void*)d_pos_, (void*)d_rpos_, (void*)d_zpull_})
     dfree(p);
-  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Xp_ = d_Y_ = d_U_ = d_pinv_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
+  d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Xp_ = d_Y_ = d_U_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
   d_pos_ = d_rpos_ = nullptr;
   d_zpull_ = nullptr;
   g_ready_ = false;
@@ -1454,8 +1445,6 @@ void Engine::alloc_numeric(Numeric& nm, int precision) {
   dalloc_into(nm.d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t),
               "hipMalloc (pivots)");
   dalloc_into(nm.d_cnt_, kCntSlots * sizeof(unsigned long long), "hipMalloc (counters)");
-  dalloc_into(nm.d_pinv_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * opt_.panel_width * ts,
-              "hipMalloc (panel inverses)");
   nm.precision_ = precision;
 }
 
@@ -1501,9 +1490,9 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
         HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb, nm.d_bperm_,
-                           null_tol, detect, (T*)nm.d_pinv_);
+                           null_tol, detect);
         HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
-                           nb, (const int32_t*)nm.d_bperm_, (const T*)nm.d_pinv_);
+                           nb, (const int32_t*)nm.d_bperm_);
         const int rem = L.large_maxm - (p + 1) * nb;
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
