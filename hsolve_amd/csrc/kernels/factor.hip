@@ -486,15 +486,15 @@ __device__ __forceinline__ float readlane_of(float v, int src) {
 // front: lane i of every wave stands for row i of the block (LDS, padded leading dimension).
 // Every wave reads the pivot column into a register and runs the same pivot search on it (DPP
 // wave reductions; argmax = lowest lane holding the maximum, by ballot), so all reach the same
-// decision without a barrier. Wave w then updates the columns j = w (mod 4) of the trailing
+// decision without a barrier. Wave w then updates the columns [16 w, 16 w + 16) of the trailing
 // triangle with the step's multipliers formed from the pivot column (broadcast LDS reads), and
 // every lane stores all of its columns (the new value where it changes, the old one elsewhere),
-// so the update has no branches. The pivot columns of F keep their unscaled values; their
-// multipliers go to a separate array Lm, written by wave 0 in the same step (nobody reads Lm during
-// a step, and a stored-back old value of F is never stale). Wave 0 alone swaps rows and
-// columns (Lm for the columns already pivoted, F for the rest), keeps the counts and the pivot
-// data. One barrier per step (four when rows are swapped). Also forms the inverse of the block's
-// unit lower L11 (per-front scratch, read by the panel solve), split over the waves the same way.
+// so the update has no per-lane branches; a wave whose columns all lie left of the pivot skips
+// it. The pivot columns of F keep their unscaled values; their multipliers go to a separate
+// array Lm, written by wave 0 in the same step (nobody reads Lm during a step, and a stored-back
+// old value of F is never stale). Wave 0 alone swaps rows and columns (Lm for the columns
+// already pivoted, F for the rest), keeps the counts and the pivot data. One barrier per step
+// (four when rows are swapped).
 constexpr int kBkLd = 65;
 constexpr int kBkWaves = 4;
 constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
@@ -506,7 +506,6 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D)
   __shared__ T Lm[64 * kBkLd];  // multipliers of the pivot columns (strictly lower L11)
   __shared__ int perm[64];
-  __shared__ int8_t ptl[64];
   __shared__ int8_t pairc[64];  // 1: column j starts a 2x2 step, F(j + 1, j) belongs to D
   const int f = blockIdx.x;
   const int s = list[f];
@@ -518,6 +517,7 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   const int lane = threadIdx.x % 64;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const bool lead = wv == 0;
+  const int jb = wv * kBkCols;  // this wave updates the block columns [jb, jb + kBkCols)
   {
     // branch-free loads (row index clamped into the block) so that they are issued together
     const T* src = P + (int64_t)c0 * m + c0 + min(lane, w - 1);
@@ -640,23 +640,26 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       }
       const bool below = lane > k && lane < w;
       const T wi = below ? c1 : (T)0;
-      // this wave's multipliers l_j = F(j, k) / d, j = wv (mod 4) (broadcast LDS reads: column k of
-      // F is not written during the step)
-      T lv[kBkCols];
+      // rank-1 update of this wave's columns jb + q, skipped once they all lie left of the pivot
+      if (jb + kBkCols - 1 > k) {
+        // multipliers l_j = F(j, k) / d (broadcast LDS reads: column k of F is not written during
+        // the step)
+        T lv[kBkCols];
 #pragma unroll
-      for (int q = 0; q < kBkCols; ++q) {
-        const int j = wv + kBkWaves * q;
-        const T fj = F[j + k * kBkLd];
-        lv[q] = (j > k && j < w) ? fj * inv : (T)0;
+        for (int q = 0; q < kBkCols; ++q) {
+          const int j = jb + q;
+          const T fj = F[j + k * kBkLd];
+          lv[q] = (j > k && j < w) ? fj * inv : (T)0;
+        }
+        // row `lane`: lv vanishes for j <= k and wi outside the rows below the pivot, so the other
+        // entries keep their values; where j > lane the entry is in the upper triangle, which
+        // nothing reads, so no per-lane mask is needed
+        T old[kBkCols];
+#pragma unroll
+        for (int q = 0; q < kBkCols; ++q) old[q] = row[(jb + q) * kBkLd];
+#pragma unroll
+        for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - wi * lv[q];
       }
-      // rank-1 update: row `lane`, this wave's columns. lv vanishes for j <= k and wi outside the
-      // rows below the pivot, so the other entries keep their values; where j > lane the entry is
-      // in the upper triangle, which nothing reads, so no per-lane mask is needed
-      T old[kBkCols];
-#pragma unroll
-      for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
-#pragma unroll
-      for (int q = 0; q < kBkCols; ++q) row[(wv + kBkWaves * q) * kBkLd] = old[q] - wi * lv[q];
       if (own && below) Lm[lane + k * kBkLd] = wi * inv;
       __syncthreads();
       k += 1;
@@ -697,24 +700,25 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       const bool below = lane > k + 1 && lane < w;
       const T w1 = below ? c1 : (T)0;
       const T w2 = below ? c2 : (T)0;
-      // this wave's multipliers of the 2x2 step, j = wv (mod 4)
-      T l1v[kBkCols], l2v[kBkCols];
+      // rank-2 update of this wave's columns, unmasked as in the 1x1 step (l1v, l2v vanish for
+      // j <= k + 1 and when dead), skipped once they all lie left of the pivot pair
+      if (jb + kBkCols - 1 > k + 1) {
+        T l1v[kBkCols], l2v[kBkCols];
 #pragma unroll
-      for (int q = 0; q < kBkCols; ++q) {
-        const int j = wv + kBkWaves * q;
-        const bool in = j > k + 1 && j < w;
-        const T f1 = F[j + k * kBkLd], f2 = F[j + (k + 1) * kBkLd];
-        const T a1 = in ? f1 : (T)0, a2 = in ? f2 : (T)0;
-        l1v[q] = a1 * i11 + a2 * i12;
-        l2v[q] = a1 * i12 + a2 * i22;
+        for (int q = 0; q < kBkCols; ++q) {
+          const int j = jb + q;
+          const bool in = j > k + 1 && j < w;
+          const T f1 = F[j + k * kBkLd], f2 = F[j + (k + 1) * kBkLd];
+          const T a1 = in ? f1 : (T)0, a2 = in ? f2 : (T)0;
+          l1v[q] = a1 * i11 + a2 * i12;
+          l2v[q] = a1 * i12 + a2 * i22;
+        }
+        T old[kBkCols];
+#pragma unroll
+        for (int q = 0; q < kBkCols; ++q) old[q] = row[(jb + q) * kBkLd];
+#pragma unroll
+        for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
       }
-      // rank-2 update, unmasked as in the 1x1 step (l1v, l2v vanish for j <= k + 1 and when dead)
-      T old[kBkCols];
-#pragma unroll
-      for (int q = 0; q < kBkCols; ++q) old[q] = row[(wv + kBkWaves * q) * kBkLd];
-#pragma unroll
-      for (int q = 0; q < kBkCols; ++q)
-        row[(wv + kBkWaves * q) * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
       if (own && below) {
         Lm[lane + k * kBkLd] = w1 * i11 + w2 * i12;
         Lm[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
@@ -736,7 +740,6 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     N.piv[first + c0 + lane] = c0 + perm[lane];
     N.ipiv[first + c0 + perm[lane]] = c0 + lane;
     bperm[(int64_t)f * nb + lane] = perm[lane];
-    ptl[lane] = my_pt;
   }
   if (lead && lane == 0) {
     unsigned long long* cnt = N.cnt;
