@@ -752,6 +752,33 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
 }
 
 constexpr int kPanelRows = 64;  // rows per panel-solve workgroup
+constexpr int kLqLd = 18;       // padded stride of the panel solve's L11 layout (conflict-free quads)
+
+// v of lane S of each quad of lanes, in every lane of the quad (DPP quad_perm [S, S, S, S]).
+template <int S>
+__device__ __forceinline__ int quad_bcast_bits(int v) {
+  return __builtin_amdgcn_mov_dpp(v, S * 0x55, 0xf, 0xf, false);
+}
+template <int S>
+__device__ __forceinline__ double quad_bcast_c(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = quad_bcast_bits<S>((int)bits), hi = quad_bcast_bits<S>((int)(bits >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int S>
+__device__ __forceinline__ float quad_bcast_c(float v) {
+  return __int_as_float(quad_bcast_bits<S>(__float_as_int(v)));
+}
+// s is a compile-time constant wherever the callers' loops are unrolled
+template <class T>
+__device__ __forceinline__ T quad_bcast(T v, int s) {
+  switch (s) {
+    case 0: return quad_bcast_c<0>(v);
+    case 1: return quad_bcast_c<1>(v);
+    case 2: return quad_bcast_c<2>(v);
+    default: return quad_bcast_c<3>(v);
+  }
+}
 
 // Panel solve below the diagonal block: for rows r in [c0 + w, m) of the panel columns
 // [c0, c0 + w): apply the block's column permutation, W = A21 L11^-T by forward substitution with
@@ -789,21 +816,19 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
   }
   const int r0 = c1 + blockIdx.x * kPanelRows;
   const int rn = min(kPanelRows, m - r0);
-  T* Lb = reinterpret_cast<T*>(smem_raw);   // strictly lower L11 of the block, column-major, ld nb
-  T* X = Lb + nb * nb;                       // rn x w, ld kPanelRows
+  // L11 by column and lane of a quad: Lq[(4 j + g) * kLqLd + u] = L11(g + 4 u, j), zero outside
+  // the strictly lower triangle and at the (j + 1, j) slot of a 2x2 or null pivot pair, which
+  // holds D's coupling, not L
+  T* Lq = reinterpret_cast<T*>(smem_raw);
+  T* X = Lq + 64 * 4 * kLqLd;  // 64 x 64, ld kPanelRows; zero outside rn x w
   __shared__ T dinv[3 * 64];
   __shared__ int8_t ptl[64];
-  for (int i = tid; i < w; i += blockDim.x) ptl[i] = N.ptype[first + c0 + i];
-  __syncthreads();
-  // the (j + 1, j) slot of a 2x2 or null pivot pair holds D's coupling, not L
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int i = idx % w, j = idx / w;
-    const bool coupling = i == j + 1 && (ptl[j] == kPiv2First || ptl[j] == kPivNull);
-    Lb[i + j * nb] = (i > j && !coupling) ? P[(int64_t)(c0 + j) * m + c0 + i] : (T)0;
-  }
+  __shared__ int32_t sbp[64];
   if (tid < w) {
     const int j = tid;
-    const int8_t t = ptl[j];
+    const int8_t t = N.ptype[first + c0 + j];
+    ptl[j] = t;
+    sbp[j] = bp[j];
     const T d = N.dval[first + c0 + j];
     if (t == kPiv1) {
       dinv[3 * j] = (T)1 / d;
@@ -818,67 +843,76 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
       dinv[3 * j] = dinv[3 * j + 1] = dinv[3 * j + 2] = (T)0;
     }
   }
-  // gather the permuted columns of A21
-  for (int idx = tid; idx < rn * w; idx += blockDim.x) {
-    const int r = idx % rn, j = idx / rn;
-    X[r + j * kPanelRows] = P[(int64_t)(c0 + bp[j]) * m + r0 + r];
+  __syncthreads();
+  // stage L11 and the permuted columns of A21: row li, columns lj + 4 q. All 32 loads are issued
+  // before the first is used (addresses clamped into the block, values masked afterwards).
+  static_assert(kThreads == 256 && kPanelRows == 64, "panel solve layout: 64 rows, four lanes per row");
+  const int li = tid & 63, lj = tid >> 6;
+  {
+    T lv[16], av[16];
+    const int lic = min(li, w - 1), ric = min(li, rn - 1);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int jc = min(lj + 4 * q, w - 1);
+      lv[q] = P[(int64_t)(c0 + jc) * m + c0 + lic];
+      av[q] = P[(int64_t)(c0 + sbp[jc]) * m + r0 + ric];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = lj + 4 * q;
+      const bool jin = j < w;
+      const bool coupling = jin && li == j + 1 && (ptl[j] == kPiv2First || ptl[j] == kPivNull);
+      Lq[(4 * j + (li & 3)) * kLqLd + (li >> 2)] = (jin && li > j && li < w && !coupling) ? lv[q] : (T)0;
+      X[li + j * kPanelRows] = (jin && li < rn) ? av[q] : (T)0;
+    }
   }
   __syncthreads();
-  // W = A21 L11^-T by forward substitution along each row: W(r, j) = X(r, j) - sum_{t < j}
-  // L11(j, t) W(r, t). Four adjacent lanes per row, lane `part` holding W(r, t) for t = part
-  // (mod 4) in registers; their partial sums are added by lane exchanges (the same total in all
-  // four lanes), so the only dependent chain is the row's 64 steps, without barriers.
+  // W = A21 L11^-T by forward substitution along each row, four lanes per row: lane g of the
+  // row's quad holds W(r, g + 4 u) in registers. At step j the quad takes W(r, j) from the lane
+  // that owns it (DPP broadcast within the quad) and every lane subtracts L11(i, j) W(r, j) from
+  // its entries i; those updates are independent, so the dependent chain is one broadcast and
+  // one FMA per column. Entries i <= j meet zeros of Lq and keep their values.
   {
-    static_assert(kPanelRows * 4 == kThreads, "panel solve layout: four lanes per row");
-    const int rr = tid >> 2, part = tid & 3;
-    const bool rin = rr < rn;
-    T wv[16];
+    const int rr = tid >> 2, g = tid & 3;
+    T xv[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) wv[u] = (T)0;
+    for (int u = 0; u < 16; ++u) xv[u] = X[rr + (g + 4 * u) * kPanelRows];
+    const T* Lg = Lq + g * kLqLd;
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
       if (j < w) {
-        T acc = (T)0;
+        const T wj = quad_bcast(xv[j >> 2], j & 3);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          if (4 * u < j) {
-            const int t = part + 4 * u;
-            if (t < j) acc += Lb[j + t * nb] * wv[u];
-          }
-        }
-        acc += __shfl_xor(acc, 1);
-        acc += __shfl_xor(acc, 2);
-        const T wj = (rin ? X[rr + j * kPanelRows] : (T)0) - acc;
-        if (part == (j & 3)) wv[j >> 2] = wj;
+        for (int u = 0; u < 16; ++u)
+          if (4 * u + 3 > j) xv[u] -= Lg[4 * j * kLqLd + u] * wj;
       }
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int t = part + 4 * u;
-      if (rin && t < w) X[rr + t * kPanelRows] = wv[u];
-    }
+    for (int u = 0; u < 16; ++u) X[rr + (g + 4 * u) * kPanelRows] = xv[u];
   }
   __syncthreads();
   T* Wp = N.W + S.w_off[s];
-  for (int idx = tid; idx < rn * w; idx += blockDim.x) {
-    const int rr = idx % rn, j = idx / rn;
-    Wp[(int64_t)(c0 + j) * m + r0 + rr] = X[rr + j * kPanelRows];
-  }
-  // L21 = W D^-1
-  for (int idx = tid; idx < rn * w; idx += blockDim.x) {
-    const int rr = idx % rn, j = idx / rn;
-    const int8_t t = ptl[j];
-    T v;
-    if (t == kPiv1) {
-      v = X[rr + j * kPanelRows] * dinv[3 * j];
-    } else if (t == kPiv2First) {
-      v = X[rr + j * kPanelRows] * dinv[3 * j] + X[rr + (j + 1) * kPanelRows] * dinv[3 * j + 1];
-    } else if (t == kPiv2Second) {
-      v = X[rr + (j - 1) * kPanelRows] * dinv[3 * (j - 1) + 1] + X[rr + j * kPanelRows] * dinv[3 * (j - 1) + 2];
-    } else {
-      v = (T)0;
+  if (li < rn) {
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+      const int j = lj + 4 * q;
+      if (j >= w) break;
+      const T x = X[li + j * kPanelRows];
+      Wp[(int64_t)(c0 + j) * m + r0 + li] = x;
+      // L21 = W D^-1
+      const int8_t t = ptl[j];
+      T v;
+      if (t == kPiv1) {
+        v = x * dinv[3 * j];
+      } else if (t == kPiv2First) {
+        v = x * dinv[3 * j] + X[li + (j + 1) * kPanelRows] * dinv[3 * j + 1];
+      } else if (t == kPiv2Second) {
+        v = X[li + (j - 1) * kPanelRows] * dinv[3 * (j - 1) + 1] + x * dinv[3 * (j - 1) + 2];
+      } else {
+        v = (T)0;
+      }
+      P[(int64_t)(c0 + j) * m + r0 + li] = v;
     }
-    P[(int64_t)(c0 + j) * m + r0 + rr] = v;
   }
 }
 
@@ -1487,7 +1521,7 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
                          dim3(kThreads), 0, st, S, N, L.d_large);
       const int npan = (L.large_max_ncol + nb - 1) / nb;
       const size_t shm_bk = (size_t)nb * nb * sizeof(T) + 4 * (size_t)nb * sizeof(T) + nb * sizeof(int) + nb + 16;
-      const size_t shm_pa = ((size_t)nb * nb + (size_t)kPanelRows * nb) * sizeof(T);
+      const size_t shm_pa = ((size_t)64 * 4 * kLqLd + (size_t)kPanelRows * 64) * sizeof(T);
       const int nrc = (L.large_maxm + kPanelRows - 1) / kPanelRows;
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
