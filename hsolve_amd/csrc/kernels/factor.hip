@@ -392,9 +392,40 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
 // ------------------------------------------------------------------------------ large fronts
 
 constexpr int kEaCols = 16;  // parent columns per extend-add workgroup
+constexpr int kEaBatch = 8;  // child entries per thread whose loads are in flight together
 
+// First index i in [0, n) with rel[i] >= key (n if none), rel strictly increasing; called by a
+// whole wave, returned to every lane. Each round samples 64 evenly spaced entries and keeps the
+// bracket between the last sample below the key and the first one at or above it, so a search
+// over n entries takes ceil(log64 n) dependent loads instead of log2 n.
+__device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ rel, int n, int key, int lane) {
+  int lo = 0, len = n;  // the answer lies in [lo, lo + len]
+  while (len > 64) {
+    const int step = (len + 63) / 64;
+    const int off = lane * step;
+    const bool less = off < len && rel[lo + off] < key;
+    const int c = __popcll(__ballot(less));
+    if (c == 0) return lo;
+    const int nlo = lo + (c - 1) * step + 1;
+    const int nhi = min(lo + c * step, lo + len);
+    lo = nlo;
+    len = nhi - nlo;
+  }
+  const bool less = lane < len && rel[lo + lane] < key;
+  return lo + __popcll(__ballot(less));
+}
+
+// Extend-add into the large fronts of one level: workgroup (x, y) adds, child by child, the
+// entries of the children's update matrices that land in parent columns [16 x, 16 x + 16) of front
+// list[y] (panel part or the parent's own update matrix). The child columns that map there are a
+// contiguous range (relmap is increasing), found by two waves at once; its (column, row) pairs of
+// the lower triangle are then spread over the workgroup, each thread issuing the loads of
+// kEaBatch entries (row map, child value, target) before using any. The entries of one child map
+// to distinct parent entries, and a barrier separates the children, so the sums are
+// deterministic and need no atomics.
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<T> N, const int32_t* __restrict__ list) {
+  __shared__ int jr[2][2];
   const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s];
   const int cb = m - ncol;
@@ -403,36 +434,50 @@ __global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<
   const int b = min(m, a + kEaCols);
   T* P = N.L + S.panel_off[s];
   T* C = N.CB + S.cb_off[s];
-  for (int q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q0 = S.child_ptr[s], q1 = S.child_ptr[s + 1];
+  for (int q = q0; q < q1; ++q) {
     const int ch = S.child_list[q];
     const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
     const int32_t* rel = S.relmap + S.sn_rowoff[ch] + S.sn_ncol[ch];
     const T* cbp = N.CB + S.cb_off[ch];
-    // child columns mapping into [a, b): contiguous because rel is increasing
-    int lo = 0, hi = cbc;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (rel[mid] < a) lo = mid + 1;
-      else hi = mid;
+    // jr[q & 1] is rewritten two children later, after every thread has passed the barrier below
+    // for the child in between
+    if (wv < 2) {
+      const int r = wave_lower_bound(rel, cbc, wv == 0 ? a : b, lane);
+      if (lane == 0) jr[q & 1][wv] = r;
     }
-    const int j0 = lo;
-    hi = cbc;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (rel[mid] < b) lo = mid + 1;
-      else hi = mid;
-    }
-    const int j1 = lo;
-    for (int j = j0; j < j1; ++j) {
-      const int pc = rel[j];
-      const T* col = cbp + (int64_t)j * cbc;
-      if (pc < ncol) {
-        T* dst = P + (int64_t)pc * m;
-        for (int i = j + threadIdx.x; i < cbc; i += blockDim.x) dst[rel[i]] += col[i];
-      } else {
-        T* dst = C + (int64_t)(pc - ncol) * cb - ncol;
-        for (int i = j + threadIdx.x; i < cbc; i += blockDim.x) dst[rel[i]] += col[i];
+    __syncthreads();
+    const int j0 = jr[q & 1][0], j1 = jr[q & 1][1];
+    if (j0 == j1) continue;
+    const int R = cbc - j0, nc = j1 - j0;
+    const int tot = R * nc;  // pairs (column j0 + t / R, row j0 + t % R); rows above the diagonal skipped
+    for (int t0 = 0; t0 < tot; t0 += kThreads * kEaBatch) {
+      int ri[kEaBatch], pc[kEaBatch];
+      T cv[kEaBatch];
+      bool ok[kEaBatch];
+#pragma unroll
+      for (int u = 0; u < kEaBatch; ++u) {
+        const int t = t0 + tid + kThreads * u;
+        const int jj = t / R;
+        const int i = j0 + (t - jj * R), j = j0 + jj;
+        ok[u] = t < tot && i >= j;
+        const int ic = ok[u] ? i : j0, jc = ok[u] ? j : j0;  // the diagonal entry j0 stands in
+        ri[u] = rel[ic];
+        pc[u] = rel[jc];
+        cv[u] = cbp[(int64_t)jc * cbc + ic];
       }
+      T* dst[kEaBatch];
+      T dv[kEaBatch];
+#pragma unroll
+      for (int u = 0; u < kEaBatch; ++u) {
+        dst[u] = pc[u] < ncol ? P + (int64_t)pc[u] * m + ri[u] : C + (int64_t)(pc[u] - ncol) * cb + (ri[u] - ncol);
+        dv[u] = *dst[u];
+      }
+#pragma unroll
+      for (int u = 0; u < kEaBatch; ++u)
+        if (ok[u]) *dst[u] = dv[u] + cv[u];
     }
     __syncthreads();
   }
