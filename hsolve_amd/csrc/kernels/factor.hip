@@ -1010,15 +1010,35 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
   const int rbase = row0 + ti * kTile, cbase = col0 + tj * kTile;
+  const int clim = mode == 0 ? ncol : m;
+  // staging: thread (lr, lt) carries row / column lr of the tile for k offsets lt + 4 v; the next
+  // K step's values are loaded into registers before the current step's products (addresses
+  // clamped into the front, values masked when stored)
+  const int lr = tid & 63, lt = tid >> 6;
+  const int gr = rbase + lr, gc = cbase + lr;
+  const bool rin = gr < m, cin = gc < clim;
+  const T* Ar = A + min(gr, m - 1);
+  const T* Br = B + min(gc, m - 1);
+  T pa[kKStep / 4], pb[kKStep / 4];
+  auto stage = [&](int kk) {
+#pragma unroll
+    for (int v = 0; v < kKStep / 4; ++v) {
+      const int64_t gk = k0 + min(kk + lt + 4 * v, kn - 1);
+      pa[v] = Ar[gk * m];
+      pb[v] = Br[gk * m];
+    }
+  };
+  stage(0);
   for (int kk = 0; kk < kn; kk += kKStep) {
-    for (int idx = tid; idx < kKStep * kTile; idx += kThreads) {
-      const int r = idx % kTile, t = idx / kTile;
-      const int gr = rbase + r, gc = cbase + r, gk = k0 + kk + t;
+#pragma unroll
+    for (int v = 0; v < kKStep / 4; ++v) {
+      const int t = lt + 4 * v;
       const bool kin = kk + t < kn;
-      As[t][r] = (kin && gr < m) ? A[(int64_t)gk * m + gr] : (T)0;
-      Bs[t][r] = (kin && gc < (mode == 0 ? ncol : m)) ? B[(int64_t)gk * m + gc] : (T)0;
+      As[t][lr] = (kin && rin) ? pa[v] : (T)0;
+      Bs[t][lr] = (kin && cin) ? pb[v] : (T)0;
     }
     __syncthreads();
+    if (kk + kKStep < kn) stage(kk + kKStep);
 #pragma unroll
     for (int t = 0; t < kKStep; ++t) {
       T av[4], bv[4];
@@ -1033,20 +1053,25 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
     }
     __syncthreads();
   }
+  // C -= acc on the lower triangle: all 16 targets are read before the first is written
   const int cb = m - ncol;
+  T* const Cbase = mode == 0 ? N.L + S.panel_off[s] : N.CB + S.cb_off[s];
+  T* dst[4][4];
+  T cv[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int gr = rbase + tx + 16 * a, gc = cbase + ty + 16 * b;
-      if (gr >= m || gr < gc) continue;
-      if (mode == 0) {
-        if (gc >= ncol) continue;
-        N.L[S.panel_off[s] + (int64_t)gc * m + gr] -= acc[a][b];
-      } else {
-        N.CB[S.cb_off[s] + (int64_t)(gc - ncol) * cb + (gr - ncol)] -= acc[a][b];
-      }
+      const int r = rbase + tx + 16 * a, c = cbase + ty + 16 * b;
+      const bool ok = r < m && r >= c && c < clim;
+      dst[a][b] = ok ? Cbase + (mode == 0 ? (int64_t)c * m + r : (int64_t)(c - ncol) * cb + (r - ncol)) : nullptr;
+      cv[a][b] = ok ? *dst[a][b] : (T)0;
     }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (dst[a][b]) *dst[a][b] = cv[a][b] - acc[a][b];
 }
 
 // ------------------------------------------------------------------------------ host engine
