@@ -527,11 +527,11 @@ __device__ __forceinline__ float readlane_of(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
-// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, four 64-lane waves per
+// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, eight 64-lane waves per
 // front: lane i of every wave stands for row i of the block (LDS, padded leading dimension).
 // Every wave reads the pivot column into a register and runs the same pivot search on it (DPP
 // wave reductions; argmax = lowest lane holding the maximum, by ballot), so all reach the same
-// decision without a barrier. Wave w then updates the columns [16 w, 16 w + 16) of the trailing
+// decision without a barrier. Wave w then updates the columns [8 w, 8 w + 8) of the trailing
 // triangle with the step's multipliers formed from the pivot column (broadcast LDS reads), and
 // every lane stores all of its columns (the new value where it changes, the old one elsewhere),
 // so the update has no per-lane branches; a wave whose columns all lie left of the pivot skips
@@ -539,9 +539,10 @@ __device__ __forceinline__ float readlane_of(float v, int src) {
 // array Lm, written by wave 0 in the same step (nobody reads Lm during a step, and a stored-back
 // old value of F is never stale). Wave 0 alone swaps rows and columns (Lm for the columns
 // already pivoted, F for the rest), keeps the counts and the pivot data. One barrier per step
-// (four when rows are swapped).
+// (four when rows are swapped). Eight waves measured faster than four (shorter update per wave,
+// and waves drop out sooner) and than sixteen (the redundant searches crowd the SIMDs).
 constexpr int kBkLd = 65;
-constexpr int kBkWaves = 4;
+constexpr int kBkWaves = 8;
 constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
 
 template <class T>
