@@ -37,9 +37,19 @@ constexpr int kThreads = 256;
 
 // ------------------------------------------------------------------------------ small helpers
 
+// Zeroes n elements of p (a device allocation, so 16-byte aligned): 16-byte streaming stores, the
+// remainder element-wise.
+typedef unsigned int hsg_u32x4 __attribute__((ext_vector_type(4)));
 template <class T>
-__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+__global__ void k_zero(T* __restrict__ p, int64_t n) {
+  constexpr int kV = 16 / sizeof(T);
+  const int64_t nv = n / kV;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  hsg_u32x4* q = reinterpret_cast<hsg_u32x4*>(p);
+  const hsg_u32x4 z = {0u, 0u, 0u, 0u};
+  for (int64_t i = first; i < nv; i += stride) __builtin_nontemporal_store(z, q + i);
+  for (int64_t i = nv * kV + first; i < n; i += stride) p[i] = (T)0;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int n) {
@@ -1573,9 +1583,9 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
   const int detect = null_tol > 0 ? 1 : 0;
   const int64_t tok = timing_begin(kPhaseFactor, st);
   int64_t nl = 0;
-  HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(panel_total_)), dim3(256), 0, st, N.L, panel_total_, (T)0);
+  HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(panel_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.L, panel_total_);
   if (cb_total_ > 0)
-    HSG_LAUNCH_COUNTED(k_fill<T>, dim3(blocks_for(cb_total_)), dim3(256), 0, st, N.CB, cb_total_, (T)0);
+    HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(cb_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.CB, cb_total_);
   HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, nm.d_cnt_, (int)kCntSlots);
   if (nnz_ > 0)
     HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
