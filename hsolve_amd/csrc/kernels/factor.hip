@@ -696,27 +696,20 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       }
       const bool below = lane > k && lane < w;
       const T wi = below ? c1 : (T)0;
+      const T li = wi * inv;  // multiplier of row `lane`, zero outside the rows below the pivot
       // rank-1 update of this wave's columns jb + q, skipped once they all lie left of the pivot
       if (jb + kBkCols - 1 > k) {
-        // multipliers l_j = F(j, k) / d (broadcast LDS reads: column k of F is not written during
-        // the step)
-        T lv[kBkCols];
-#pragma unroll
-        for (int q = 0; q < kBkCols; ++q) {
-          const int j = jb + q;
-          const T fj = F[j + k * kBkLd];
-          lv[q] = (j > k && j < w) ? fj * inv : (T)0;
-        }
-        // row `lane`: lv vanishes for j <= k and wi outside the rows below the pivot, so the other
-        // entries keep their values; where j > lane the entry is in the upper triangle, which
-        // nothing reads, so no per-lane mask is needed
+        // row `lane`: F(lane, j) -= l_lane F(j, k), with F(j, k) read from lane j of the pivot
+        // column in registers (a uniform lane index). wi vanishes for j <= k and li outside the
+        // rows below the pivot, so the other entries keep their values; where j > lane the entry
+        // is in the upper triangle, which nothing reads, so no per-lane mask is needed
         T old[kBkCols];
 #pragma unroll
         for (int q = 0; q < kBkCols; ++q) old[q] = row[(jb + q) * kBkLd];
 #pragma unroll
-        for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - wi * lv[q];
+        for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - li * readlane_of(wi, jb + q);
       }
-      if (own && below) Lm[lane + k * kBkLd] = wi * inv;
+      if (own && below) Lm[lane + k * kBkLd] = li;
       __syncthreads();
       k += 1;
     } else {
