@@ -7,11 +7,14 @@
 //     load the panel (original entries were scattered there beforehand), extend-add the children's
 //     contribution blocks, factor the pivot columns with Bunch-Kaufman 1x1/2x2 pivots restricted
 //     to the front's fully-summed block, write L, D and the contribution block back;
-//   * large fronts: extend-add by column ranges (each parent column owned by one workgroup, children
-//     in order: deterministic), then per panel of nb pivot columns a one-workgroup Bunch-Kaufman
-//     factorization of the nb x nb diagonal block, a multi-workgroup panel solve that forms
-//     W = A21 L11^-T and L21 = W D^-1, and a trailing GEMM update of the remaining pivot columns;
-//     finally one GEMM contribution-block update CB -= W_cb L_cb^T over all pivot columns.
+//   * large fronts: extend-add of the children into the pivot columns, by column ranges (each
+//     parent column owned by one workgroup, children in order: deterministic), then per panel of
+//     nb pivot columns a one-workgroup Bunch-Kaufman factorization of the nb x nb diagonal block,
+//     a multi-workgroup panel solve that forms W = A21 L11^-T and L21 = W D^-1, and a trailing
+//     GEMM update of the remaining pivot columns; finally one GEMM stores the contribution block
+//     CB = -W_cb L_cb^T over all pivot columns, and the children's entries of the CB columns are
+//     added to it (the contribution blocks need no zero fill: every entry of their lower
+//     triangles is stored before it is read, and nothing reads above the diagonal).
 // Inertia: signs of D (2x2 blocks by their eigenvalues), integer counters. Null pivots: a pivot
 // block with an eigenvalue of magnitude <= null_tol * max|K_ij| is flagged (the reference's
 // CNTL(3) rule, include/hsolve/factor.hpp:79-115), counted as zero and removed from the
@@ -426,70 +429,80 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ rel,
 }
 
 // Extend-add into the large fronts of one level: workgroup (x, y) adds, child by child, the
-// entries of the children's update matrices that land in parent columns [16 x, 16 x + 16) of front
-// list[y] (panel part or the parent's own update matrix). The child columns that map there are a
-// contiguous range (relmap is increasing), found by two waves at once; its (column, row) pairs of
-// the lower triangle are then spread over the workgroup, each thread issuing the loads of
-// kEaBatch entries (row map, child value, target) before using any. The entries of one child map
-// to distinct parent entries, and a barrier separates the children, so the sums are
-// deterministic and need no atomics.
+// entries of the children's update matrices that land in 16 parent columns of front list[y]:
+// part 0 covers the panel columns [16 x, 16 x + 16) (before the front is factorized), part 1 the
+// columns ncol + [16 x, 16 x + 16) of the front's own update matrix, after the front's update
+// product has been stored there. The child columns that map there are a contiguous range
+// (relmap is increasing); the four waves find the ranges of two children at a time, one end
+// each. A child's (column, row) pairs of the lower triangle are then spread over the workgroup,
+// each thread issuing the loads of kEaBatch entries (row map, child value, target) before using
+// any. The entries of one child map to distinct parent entries, and a barrier separates the
+// children, so the sums are deterministic and need no atomics.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<T> N, const int32_t* __restrict__ list) {
+__global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
+                                                              int part) {
   __shared__ int jr[2][2];
   const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s];
   const int cb = m - ncol;
-  const int a = blockIdx.x * kEaCols;
-  if (a >= m) return;
-  const int b = min(m, a + kEaCols);
+  const int lim = part == 0 ? ncol : m;
+  const int a = (part == 0 ? 0 : ncol) + blockIdx.x * kEaCols;
+  if (a >= lim) return;
+  const int b = min(lim, a + kEaCols);
   T* P = N.L + S.panel_off[s];
   T* C = N.CB + S.cb_off[s];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  static_assert(kThreads == 256, "extend-add: four waves search two children's ranges at once");
   const int q0 = S.child_ptr[s], q1 = S.child_ptr[s + 1];
-  for (int q = q0; q < q1; ++q) {
-    const int ch = S.child_list[q];
-    const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
-    const int32_t* rel = S.relmap + S.sn_rowoff[ch] + S.sn_ncol[ch];
-    const T* cbp = N.CB + S.cb_off[ch];
-    // jr[q & 1] is rewritten two children later, after every thread has passed the barrier below
-    // for the child in between
-    if (wv < 2) {
-      const int r = wave_lower_bound(rel, cbc, wv == 0 ? a : b, lane);
-      if (lane == 0) jr[q & 1][wv] = r;
+  for (int g0 = q0; g0 < q1; g0 += 2) {
+    const int ng = min(2, q1 - g0);
+    // wave w finds, for child g0 + w / 2, the first column mapping at or beyond a (w even) or b
+    if ((wv >> 1) < ng) {
+      const int ch = S.child_list[g0 + (wv >> 1)];
+      const int32_t* rel = S.relmap + S.sn_rowoff[ch] + S.sn_ncol[ch];
+      const int r = wave_lower_bound(rel, S.sn_m[ch] - S.sn_ncol[ch], (wv & 1) ? b : a, lane);
+      if (lane == 0) jr[wv >> 1][wv & 1] = r;
     }
     __syncthreads();
-    const int j0 = jr[q & 1][0], j1 = jr[q & 1][1];
-    if (j0 == j1) continue;
-    const int R = cbc - j0, nc = j1 - j0;
-    const int tot = R * nc;  // pairs (column j0 + t / R, row j0 + t % R); rows above the diagonal skipped
-    for (int t0 = 0; t0 < tot; t0 += kThreads * kEaBatch) {
-      int ri[kEaBatch], pc[kEaBatch];
-      T cv[kEaBatch];
-      bool ok[kEaBatch];
-#pragma unroll
-      for (int u = 0; u < kEaBatch; ++u) {
-        const int t = t0 + tid + kThreads * u;
-        const int jj = t / R;
-        const int i = j0 + (t - jj * R), j = j0 + jj;
-        ok[u] = t < tot && i >= j;
-        const int ic = ok[u] ? i : j0, jc = ok[u] ? j : j0;  // the diagonal entry j0 stands in
-        ri[u] = rel[ic];
-        pc[u] = rel[jc];
-        cv[u] = cbp[(int64_t)jc * cbc + ic];
+    for (int c = 0; c < ng; ++c) {
+      const int ch = S.child_list[g0 + c];
+      const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
+      const int32_t* rel = S.relmap + S.sn_rowoff[ch] + S.sn_ncol[ch];
+      const T* cbp = N.CB + S.cb_off[ch];
+      const int j0 = jr[c][0], j1 = jr[c][1];
+      if (j0 == j1) continue;
+      const int R = cbc - j0, nc = j1 - j0;
+      const int tot = R * nc;  // pairs (column j0 + t / R, row j0 + t % R); rows above the diagonal skipped
+      for (int t0 = 0; t0 < tot; t0 += kThreads * kEaBatch) {
+        int ri[kEaBatch], pc[kEaBatch];
+        T cv[kEaBatch];
+        bool ok[kEaBatch];
+  #pragma unroll
+        for (int u = 0; u < kEaBatch; ++u) {
+          const int t = t0 + tid + kThreads * u;
+          const int jj = t / R;
+          const int i = j0 + (t - jj * R), j = j0 + jj;
+          ok[u] = t < tot && i >= j;
+          const int ic = ok[u] ? i : j0, jc = ok[u] ? j : j0;  // the diagonal entry j0 stands in
+          ri[u] = rel[ic];
+          pc[u] = rel[jc];
+          cv[u] = cbp[(int64_t)jc * cbc + ic];
+        }
+        T* dst[kEaBatch];
+        T dv[kEaBatch];
+  #pragma unroll
+        for (int u = 0; u < kEaBatch; ++u) {
+          dst[u] = pc[u] < ncol ? P + (int64_t)pc[u] * m + ri[u] : C + (int64_t)(pc[u] - ncol) * cb + (ri[u] - ncol);
+          dv[u] = *dst[u];
+        }
+  #pragma unroll
+        for (int u = 0; u < kEaBatch; ++u)
+          if (ok[u]) *dst[u] = dv[u] + cv[u];
       }
-      T* dst[kEaBatch];
-      T dv[kEaBatch];
-#pragma unroll
-      for (int u = 0; u < kEaBatch; ++u) {
-        dst[u] = pc[u] < ncol ? P + (int64_t)pc[u] * m + ri[u] : C + (int64_t)(pc[u] - ncol) * cb + (ri[u] - ncol);
-        dv[u] = *dst[u];
-      }
-#pragma unroll
-      for (int u = 0; u < kEaBatch; ++u)
-        if (ok[u]) *dst[u] = dv[u] + cv[u];
+      __syncthreads();  // the next child's entries may land on the same parent entries
     }
-    __syncthreads();
+    __syncthreads();  // every thread has read jr before the next pair's searches rewrite it
   }
 }
 
@@ -1057,7 +1070,9 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
     }
     __syncthreads();
   }
-  // C -= acc on the lower triangle: all 16 targets are read before the first is written
+  // C -= acc on the lower triangle: all 16 targets are read before the first is written. The
+  // contribution block (mode 1) receives the children's entries only afterwards, so there the
+  // product is stored as it is: C = -acc, with nothing read and no zero fill beforehand.
   const int cb = m - ncol;
   T* const Cbase = mode == 0 ? N.L + S.panel_off[s] : N.CB + S.cb_off[s];
   T* dst[4][4];
@@ -1069,7 +1084,7 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
       const int r = rbase + tx + 16 * a, c = cbase + ty + 16 * b;
       const bool ok = r < m && r >= c && c < clim;
       dst[a][b] = ok ? Cbase + (mode == 0 ? (int64_t)c * m + r : (int64_t)(c - ncol) * cb + (r - ncol)) : nullptr;
-      cv[a][b] = ok ? *dst[a][b] : (T)0;
+      cv[a][b] = (ok && mode == 0) ? *dst[a][b] : (T)0;
     }
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -1577,8 +1592,6 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
   const int64_t tok = timing_begin(kPhaseFactor, st);
   int64_t nl = 0;
   HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(panel_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.L, panel_total_);
-  if (cb_total_ > 0)
-    HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(cb_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.CB, cb_total_);
   HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, nm.d_cnt_, (int)kCntSlots);
   if (nnz_ > 0)
     HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
@@ -1591,8 +1604,8 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
                          detect);
     }
     if (L.n_large) {
-      HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_maxm + kEaCols - 1) / kEaCols, L.n_large),
-                         dim3(kThreads), 0, st, S, N, L.d_large);
+      HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_ncol + kEaCols - 1) / kEaCols, L.n_large),
+                         dim3(kThreads), 0, st, S, N, L.d_large, 0);
       const int npan = (L.large_max_ncol + nb - 1) / nb;
       const size_t shm_bk = (size_t)nb * nb * sizeof(T) + 4 * (size_t)nb * sizeof(T) + nb * sizeof(int) + nb + 16;
       const size_t shm_pa = ((size_t)64 * 4 * kLqLd + (size_t)kPanelRows * 64) * sizeof(T);
@@ -1615,6 +1628,8 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
       if (L.large_max_cb > 0) {
         const int t = (L.large_max_cb + kTile - 1) / kTile;
         HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * t, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0, nb);
+        HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_cb + kEaCols - 1) / kEaCols, L.n_large),
+                           dim3(kThreads), 0, st, S, N, L.d_large, 1);
       }
     }
   }
