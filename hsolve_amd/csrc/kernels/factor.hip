@@ -404,8 +404,8 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
 
 // ------------------------------------------------------------------------------ large fronts
 
-constexpr int kEaCols = 16;  // parent columns per extend-add workgroup
-constexpr int kEaBatch = 8;  // child entries per thread whose loads are in flight together
+constexpr int kEaCols = 8;   // parent columns per extend-add workgroup
+constexpr int kEaBatch = 16; // child entries per thread whose loads are in flight together
 
 // First index i in [0, n) with rel[i] >= key (n if none), rel strictly increasing; called by a
 // whole wave, returned to every lane. Each round samples 64 evenly spaced entries and keeps the
