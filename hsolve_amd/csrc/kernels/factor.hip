@@ -78,10 +78,17 @@ __global__ void k_assemble_original(int64_t nnz, const int64_t* __restrict__ dst
     L[d] = (T)v;
     local_max = fmax(local_max, fabs(v));
   }
-  // wave reduction then one atomic per wave
+  // wave reduction, then the workgroup's waves through LDS, then one atomic per workgroup (atomics
+  // on the one counter serialize: one per wave of an entry-per-thread grid cost ~0.25 ms)
+  __shared__ double wmax[kThreads / 64];
   for (int off = 32; off > 0; off >>= 1) local_max = fmax(local_max, __shfl_xor(local_max, off));
-  if ((threadIdx.x & 63) == 0)
-    atomicMax(&cnt[kCntKnorm], (unsigned long long)__double_as_longlong(local_max));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = local_max;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mx = fmax(mx, wmax[w]);
+    atomicMax(&cnt[kCntKnorm], (unsigned long long)__double_as_longlong(mx));
+  }
 }
 
 // ------------------------------------------------------------------------ Bunch-Kaufman in LDS
@@ -1594,7 +1601,7 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
   HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(panel_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.L, panel_total_);
   HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, nm.d_cnt_, (int)kCntSlots);
   if (nnz_ > 0)
-    HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 8192)), dim3(256), 0, st, nnz_, d_asm_, d_ia_,
+    HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 1024)), dim3(kThreads), 0, st, nnz_, d_asm_, d_ia_,
                        d_ib_, av, bv, alpha, beta, N.L, nm.d_cnt_);
   for (const Level& L : levels_) {
     if (L.n_small) {
