@@ -374,9 +374,21 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
   int* perm = reinterpret_cast<int*>(dof + m);
   int8_t* pt = reinterpret_cast<int8_t*>(perm + m);
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < m * m; idx += blockDim.x) {
-    const int c = idx / m;
-    F[idx] = c < ncol ? N.L[poff + idx] : (T)0;
+  // the front's panel, then each child's update matrix added in: kBatch global loads per thread
+  // are issued before the first is used (the loops had waited on every load in turn)
+  constexpr int kBatch = 8;
+  for (int base = 0; base < m * m; base += kThreads * kBatch) {
+    T v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int idx = base + tid + kThreads * u;
+      v[u] = (idx < m * m && idx / m < ncol) ? N.L[poff + idx] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int idx = base + tid + kThreads * u;
+      if (idx < m * m) F[idx] = v[u];
+    }
   }
   __syncthreads();
   for (int q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
@@ -384,9 +396,23 @@ __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N
     const int cbc = S.sn_m[ch] - S.sn_ncol[ch];
     const int32_t* rel = S.relmap + S.sn_rowoff[ch] + S.sn_ncol[ch];
     const T* cbp = N.CB + S.cb_off[ch];
-    for (int idx = tid; idx < cbc * cbc; idx += blockDim.x) {
-      const int i = idx % cbc, j = idx / cbc;
-      if (i >= j) F[rel[i] + rel[j] * m] += cbp[idx];
+    for (int base = 0; base < cbc * cbc; base += kThreads * kBatch) {
+      T v[kBatch];
+      int ri[kBatch], rj[kBatch];
+      bool ok[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int idx = base + tid + kThreads * u;
+        const int i = idx % cbc, j = idx / cbc;
+        ok[u] = idx < cbc * cbc && i >= j;
+        v[u] = ok[u] ? cbp[idx] : (T)0;
+        ri[u] = ok[u] ? rel[i] : 0;
+        rj[u] = ok[u] ? rel[j] : 0;
+      }
+      // one child's entries land on distinct entries of the front
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u)
+        if (ok[u]) F[ri[u] + rj[u] * m] += v[u];
     }
     __syncthreads();
   }
