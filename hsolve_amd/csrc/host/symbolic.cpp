@@ -272,43 +272,87 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
     for (int32_t s = 0; s < S.nsn; ++s)
       if (S.sn_parent[s] >= 0) S.child_list[fill[S.sn_parent[s]]++] = s;
   }
-  // row structures bottom-up
+  // levels (height): children first
+  S.level.assign(S.nsn, 0);
+  for (int32_t s = 0; s < S.nsn; ++s)
+    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q)
+      S.level[s] = std::max(S.level[s], S.level[S.child_list[q]] + 1);
+  S.nlevels = 0;
+  for (int32_t s = 0; s < S.nsn; ++s) S.nlevels = std::max(S.nlevels, S.level[s] + 1);
+  S.level_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  for (int32_t s = 0; s < S.nsn; ++s) ++S.level_ptr[S.level[s] + 1];
+  for (int32_t l = 0; l < S.nlevels; ++l) S.level_ptr[l + 1] += S.level_ptr[l];
+  S.level_sn.resize(S.nsn);
+  {
+    std::vector<int32_t> fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
+    for (int32_t s = 0; s < S.nsn; ++s) S.level_sn[fill[S.level[s]]++] = s;
+  }
+  // row structures, level by level (a front's children lie on lower levels): the fronts of a level
+  // with many of them on up to 16 threads, each thread with its own marks; the lists are then
+  // concatenated in front order, so the result does not depend on the thread count
   S.sn_rowoff.assign(static_cast<size_t>(S.nsn) + 1, 0);
   S.sn_m.assign(S.nsn, 0);
-  std::vector<int32_t> mark(n, -1);
-  std::vector<int32_t> tmp;
-  S.rows.clear();
-  for (int32_t s = 0; s < S.nsn; ++s) {
-    const int32_t f = S.sn_first[s], l = f + S.sn_ncol[s];
-    tmp.clear();
-    for (int32_t j = f; j < l; ++j) {
-      mark[j] = s;
-      tmp.push_back(j);
-    }
-    for (int32_t j = f; j < l; ++j)
-      for (int64_t p = cp[j]; p < cp[j + 1]; ++p) {
-        const int32_t i = ri[p];
-        if (mark[i] != s) {
-          mark[i] = s;
-          tmp.push_back(i);
+  {
+    std::vector<std::vector<int32_t>> rl(S.nsn);
+    const int nthr = n < 8192 ? 1 : std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<int32_t>> marks(nthr);
+    std::vector<char> bad(nthr, 0);
+    auto fronts = [&](int t, int32_t k0, int32_t k1) {
+      std::vector<int32_t>& mark = marks[t];
+      if (mark.empty()) mark.assign(n, -1);
+      for (int32_t k = k0; k < k1; ++k) {
+        const int32_t s = S.level_sn[k];
+        const int32_t f = S.sn_first[s], l = f + S.sn_ncol[s];
+        std::vector<int32_t>& tmp = rl[s];
+        for (int32_t j = f; j < l; ++j) {
+          mark[j] = s;
+          tmp.push_back(j);
         }
-      }
-    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
-      const int32_t c = S.child_list[q];
-      for (int64_t p = S.sn_rowoff[c] + S.sn_ncol[c]; p < S.sn_rowoff[c + 1]; ++p) {
-        const int32_t i = S.rows[p];
-        if (mark[i] != s) {
-          mark[i] = s;
-          tmp.push_back(i);
+        for (int32_t j = f; j < l; ++j)
+          for (int64_t p = cp[j]; p < cp[j + 1]; ++p) {
+            const int32_t i = ri[p];
+            if (mark[i] != s) {
+              mark[i] = s;
+              tmp.push_back(i);
+            }
+          }
+        for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
+          const std::vector<int32_t>& cr = rl[S.child_list[q]];
+          for (size_t p = static_cast<size_t>(S.sn_ncol[S.child_list[q]]); p < cr.size(); ++p) {
+            const int32_t i = cr[p];
+            if (mark[i] != s) {
+              mark[i] = s;
+              tmp.push_back(i);
+            }
+          }
         }
+        std::sort(tmp.begin() + (l - f), tmp.end());
+        for (size_t q = static_cast<size_t>(l - f); q < tmp.size(); ++q)
+          if (tmp[q] < l) bad[t] = 1;
       }
+    };
+    for (int32_t lv = 0; lv < S.nlevels; ++lv) {
+      const int32_t k0 = static_cast<int32_t>(S.level_ptr[lv]), k1 = static_cast<int32_t>(S.level_ptr[lv + 1]);
+      const int nt = (k1 - k0) >= 512 ? nthr : 1;
+      if (nt <= 1) {
+        fronts(0, k0, k1);
+        continue;
+      }
+      std::vector<std::thread> pool;
+      for (int t = 1; t < nt; ++t)
+        pool.emplace_back(fronts, t, k0 + static_cast<int32_t>(int64_t{k1 - k0} * t / nt),
+                          k0 + static_cast<int32_t>(int64_t{k1 - k0} * (t + 1) / nt));
+      fronts(0, k0, k0 + static_cast<int32_t>(int64_t{k1 - k0} / nt));
+      for (auto& th : pool) th.join();
     }
-    std::sort(tmp.begin() + (l - f), tmp.end());
-    for (size_t k = static_cast<size_t>(l - f); k < tmp.size(); ++k)
-      if (tmp[k] < l) throw std::runtime_error("analyze_pattern: row above the supernode's columns");
-    S.rows.insert(S.rows.end(), tmp.begin(), tmp.end());
-    S.sn_rowoff[s + 1] = static_cast<int64_t>(S.rows.size());
-    S.sn_m[s] = static_cast<int32_t>(tmp.size());
+    for (char b : bad)
+      if (b) throw std::runtime_error("analyze_pattern: row above the supernode's columns");
+    for (int32_t s = 0; s < S.nsn; ++s) {
+      S.sn_rowoff[s + 1] = S.sn_rowoff[s] + static_cast<int64_t>(rl[s].size());
+      S.sn_m[s] = static_cast<int32_t>(rl[s].size());
+    }
+    S.rows.resize(static_cast<size_t>(S.sn_rowoff[S.nsn]));
+    for (int32_t s = 0; s < S.nsn; ++s) std::copy(rl[s].begin(), rl[s].end(), S.rows.begin() + S.sn_rowoff[s]);
   }
   // extend-add maps: local index of each child CB row in the parent's front
   S.relmap.assign(S.rows.size(), -1);
@@ -340,21 +384,6 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
     S.flops += pp * pp * pp / 3.0 + pp * pp * c + pp * c * c;
     S.max_front = std::max<int32_t>(S.max_front, static_cast<int32_t>(m));
     S.max_ncol = std::max<int32_t>(S.max_ncol, static_cast<int32_t>(nc));
-  }
-  // levels (height): children first
-  S.level.assign(S.nsn, 0);
-  for (int32_t s = 0; s < S.nsn; ++s)
-    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q)
-      S.level[s] = std::max(S.level[s], S.level[S.child_list[q]] + 1);
-  S.nlevels = 0;
-  for (int32_t s = 0; s < S.nsn; ++s) S.nlevels = std::max(S.nlevels, S.level[s] + 1);
-  S.level_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  for (int32_t s = 0; s < S.nsn; ++s) ++S.level_ptr[S.level[s] + 1];
-  for (int32_t l = 0; l < S.nlevels; ++l) S.level_ptr[l + 1] += S.level_ptr[l];
-  S.level_sn.resize(S.nsn);
-  {
-    std::vector<int32_t> fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
-    for (int32_t s = 0; s < S.nsn; ++s) S.level_sn[fill[S.level[s]]++] = s;
   }
   return S;
 }
