@@ -43,10 +43,11 @@ SIGNATURES = {
     "hsg_analysis_get_order": (C.c_int, [_P, _P]),
     "hsg_analysis_get_fronts": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "hsg_factorize": (C.c_int, [_P, _D, _D, _P, _P, C.c_int32, _D, _P, C.POINTER(Inertia), C.POINTER(C.c_int32)]),
-    "hsg_solve": (C.c_int, [_P, _P, _I64, _I64, _P]),
+    "hsg_factor_solve": (C.c_int, [_P, _P, _I64, _I64, _P]),
     "hsg_release_factors": (C.c_int, [_P]),
     "hsg_factor_destroy": (None, [_P]),
     "hsg_factor_clone": (C.c_int, [_P, C.POINTER(_P)]),
+    "hsg_release_cached_memory": (C.c_int, [_P]),
     "hsg_csr_spmm": (C.c_int, [_I64, _P, _P, _P, _P, _I64, _P, _I64, _I64, _D, _D, _P]),
     "hsg_diag_spmm": (C.c_int, [_I64, _P, _P, _I64, _P, _I64, _I64, _D, _D, _P]),
     "hsg_shifted_residual": (C.c_int, [_I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _I64, _P, _I64, _P, _I64, _I64,
@@ -68,8 +69,8 @@ for _name in ("csr_spmm", "diag_spmm", "shifted_residual", "gram", "block_update
 # point only enqueues one or two kernels on a stream and returns within microseconds: those keep the
 # GIL. Concurrent spectrum slices drive the device from several Python threads, and a thread that
 # gave up the GIL for a short call could wait a full interpreter switch interval to get it back.
-RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_solve", "hsg_factor_clone", "hsg_release_factors",
-                          "hsg_factor_destroy"})
+RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_factor_solve", "hsg_factor_clone", "hsg_release_factors",
+                          "hsg_factor_destroy", "hsg_release_cached_memory"})
 
 
 class Bound:

@@ -1,18 +1,34 @@
 """hsolve.solve() drop-in (python/hsolve/__init__.py:135-171, :241-419).
 
-Same signature, keyword handling and config merging as the reference; matrices are scipy.sparse,
-dense numpy arrays or paths to .npz / .npy / .mtx files. The solve runs in this package's GPU
-engine (hsolve_amd.solver) instead of the reference's C++ extension; results come back as numpy
-arrays (eigenvectors n x m in Fortran order) with the report as a dict of report.json's keys.
+The signature, the keyword rules, the result type and the error classes are the reference's
+contract; the code below is this package's own. Matrices are scipy.sparse, dense numpy arrays or
+paths to .npz / .npy / .mtx files. The solve runs in this package's GPU engine
+(hsolve_amd.solver); results come back as numpy arrays (eigenvectors n x m in Fortran order) with
+the report as a dict of report.json's keys.
+
+Keyword rules (what the caller's keywords do to the configuration, in this order):
+  * config: None, a dict (deep-copied; numpy scalars become plain numbers) or a path to a JSON
+    file whose top level is an object. Unreadable file: HsolveIOError; bad JSON: InvalidInputError.
+  * mode given explicitly: must be one of MODES and sets target.mode. Otherwise lower or upper
+    given selects "interval" (an error if sigma is given too), sigma alone selects "nearest", and
+    without any of them config.json's target.mode stands.
+  * sigma / lower / upper (when not None) and k (when passed) set the target fields; tol and
+    threads (when passed) set the top-level fields. Real-valued keywords take ints, floats, numpy
+    real scalars and 0-d arrays, never bool or str (TypeError); integer keywords take whatever
+    operator.index accepts, never bool (TypeError).
+A keyword counts as passed when it appears in the call, whatever its value; the defaults shown in
+the signature are the ones config.json itself falls back to.
 """
 from __future__ import annotations
 
+import functools
 import json
 import numbers
 import operator
 import os
 import threading
 import warnings
+from collections.abc import Mapping
 from dataclasses import dataclass
 from typing import Any
 
@@ -45,95 +61,103 @@ class SolveResult:
                 f"success={self.success})")
 
 
-class _Default:
-    __slots__ = ("value",)
-
-    def __init__(self, value):
-        self.value = value
-
-    def __repr__(self):
-        return repr(self.value)
-
-
-_MODES = ("nearest", "interval", "smallest", "largest")
-_STORAGES = ("auto", "full", "upper", "lower")
+MODES = ("nearest", "interval", "smallest", "largest")
+STORAGES = ("auto", "full", "upper", "lower")
 _SOLVE_LOCK = threading.Lock()  # one solve at a time per process (module.cpp:14-16)
 
 
-def _real(x, name):
-    if isinstance(x, np.ndarray) and x.ndim == 0 and x.dtype.kind in "iuf":
-        return float(x)
-    if isinstance(x, (bool, np.bool_)) or not isinstance(x, numbers.Real):
-        raise TypeError(f"{name} must be a real number, got {type(x).__name__}")
-    return float(x)
+def as_real(value: Any, what: str) -> float:
+    """value as a float when it is a real number (bool and str excluded); TypeError otherwise."""
+    if isinstance(value, (bool, np.bool_)):
+        ok = False
+    elif isinstance(value, np.ndarray):
+        ok = value.ndim == 0 and value.dtype.kind in "iuf"
+    else:
+        ok = isinstance(value, numbers.Real)
+    if not ok:
+        raise TypeError(f"{what} must be a real number, got {type(value).__name__}")
+    return float(value)
 
 
-def _integer(x, name):
-    if isinstance(x, (bool, np.bool_)):
-        raise TypeError(f"{name} must be an integer, got bool")
-    try:
-        return operator.index(x)
-    except TypeError as e:
-        raise TypeError(f"{name} must be an integer, got {type(x).__name__}") from e
+def as_int(value: Any, what: str) -> int:
+    """value as an int when it supports operator.index (bool excluded); TypeError otherwise."""
+    if not isinstance(value, (bool, np.bool_)):
+        try:
+            return operator.index(value)
+        except TypeError:
+            pass
+    raise TypeError(f"{what} must be an integer, got {type(value).__name__}")
 
 
-def _read_config(config):
+def _plain(o: Any) -> Any:
+    """JSON encoder hook: numpy scalars (and 0-d arrays) become their Python values."""
+    if hasattr(o, "item") and np.ndim(o) == 0:
+        return o.item()
+    return str(o)
+
+
+def load_config(config: Any) -> dict:
+    """The caller's config argument as a fresh dict of plain JSON values."""
     if config is None:
         return {}
-    if isinstance(config, dict):
-        return json.loads(json.dumps(config, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
-    if isinstance(config, (str, bytes, os.PathLike)):
-        path = os.fsdecode(config)
-        try:
-            with open(path, encoding="utf-8") as f:
-                text = f.read()
-        except OSError as e:
-            raise HsolveIOError(f"cannot read config file {path}: {e}") from e
-        try:
-            doc = json.loads(text)
-        except json.JSONDecodeError as e:
-            raise InvalidInputError(f"config file {path}: invalid JSON: {e}") from None
-        if not isinstance(doc, dict):
-            raise InvalidInputError(f"config file {path}: top level must be a JSON object")
-        return doc
-    raise TypeError(f"config must be a dict, a path to config.json or None, got {type(config).__name__}")
+    if isinstance(config, Mapping):
+        return json.loads(json.dumps(dict(config), default=_plain))
+    if not isinstance(config, (str, bytes, os.PathLike)):
+        raise TypeError(f"config must be a dict, a path to config.json or None, got {type(config).__name__}")
+    path = os.fsdecode(config)
+    try:
+        with open(path, encoding="utf-8") as fh:
+            text = fh.read()
+    except OSError as e:
+        raise HsolveIOError(f"cannot read config file {path}: {e}") from e
+    try:
+        doc = json.loads(text)
+    except json.JSONDecodeError as e:
+        raise InvalidInputError(f"config file {path}: invalid JSON: {e}") from None
+    if not isinstance(doc, dict):
+        raise InvalidInputError(f"config file {path}: top level must be a JSON object")
+    return doc
 
 
-def merge_config(config, sigma, k, mode, lower, upper, tol, threads) -> dict:
-    """_merge_config (python/hsolve/__init__.py:241-278)."""
-    cfg = _read_config(config)
+def _target_mode(given: frozenset, mode: Any, sigma, lower, upper) -> str | None:
+    """target.mode implied by the keywords (None: keep config.json's)."""
+    if "mode" in given:
+        if not isinstance(mode, str):
+            raise TypeError(f"mode must be a string, got {type(mode).__name__}")
+        if mode not in MODES:
+            raise InvalidInputError(f"mode = {mode!r} is not one of {MODES}")
+        return mode
+    if lower is not None or upper is not None:
+        if sigma is not None:
+            raise InvalidInputError("give sigma (nearest) or lower/upper (interval), not both, or pass mode=")
+        return "interval"
+    return "nearest" if sigma is not None else None
+
+
+def merge_config(config, given: frozenset = frozenset(), *, sigma=None, k=6, mode="nearest", lower=None,
+                 upper=None, tol=1e-10, threads=0) -> dict:
+    """The configuration a solve runs with: config (see load_config) overridden by the keywords in
+    `given` and by sigma / lower / upper where not None (python/hsolve/__init__.py:241-278), then
+    validated and completed by config.parse_config."""
+    cfg = load_config(config)
     target = cfg.get("target", {})
     if not isinstance(target, dict):
         raise InvalidInputError("config: 'target' must be an object")
     target = dict(target)
-    explicit_mode = not isinstance(mode, _Default)
-    interval_bounds = lower is not None or upper is not None
-    if explicit_mode:
-        if not isinstance(mode, str):
-            raise TypeError(f"mode must be a string, got {type(mode).__name__}")
-        if mode not in _MODES:
-            raise InvalidInputError(f"mode = {mode!r} is not one of {_MODES}")
-        target["mode"] = mode
-    elif interval_bounds:
-        if sigma is not None:
-            raise InvalidInputError("give sigma (nearest) or lower/upper (interval), not both, or pass mode=")
-        target["mode"] = "interval"
-    elif sigma is not None:
-        target["mode"] = "nearest"
-    if sigma is not None:
-        target["sigma"] = _real(sigma, "sigma")
-    if lower is not None:
-        target["lower"] = _real(lower, "lower")
-    if upper is not None:
-        target["upper"] = _real(upper, "upper")
-    if not isinstance(k, _Default):
-        target["k"] = _integer(k, "k")
+    m = _target_mode(given, mode, sigma, lower, upper)
+    if m is not None:
+        target["mode"] = m
+    for key, value in (("sigma", sigma), ("lower", lower), ("upper", upper)):
+        if value is not None:
+            target[key] = as_real(value, key)
+    if "k" in given:
+        target["k"] = as_int(k, "k")
     if target:
         cfg["target"] = target
-    if not isinstance(tol, _Default):
-        cfg["tol"] = _real(tol, "tol")
-    if not isinstance(threads, _Default):
-        cfg["threads"] = _integer(threads, "threads")
+    if "tol" in given:
+        cfg["tol"] = as_real(tol, "tol")
+    if "threads" in given:
+        cfg["threads"] = as_int(threads, "threads")
     return cfgmod.parse_config(cfg)
 
 
@@ -166,7 +190,8 @@ def _matrix(m, name: str, storage: str, notes: list | None = None, hermitian_tol
         M = sp.csr_matrix(m)
     else:
         if hasattr(m, "matvec") and not isinstance(m, np.ndarray):
-            raise TypeError(f"{name}: hsolve factorizes the matrix and needs its entries, not a LinearOperator")
+            raise TypeError(f"{name}: a matrix-free operator cannot be factorized; pass the matrix entries "
+                            "(scipy.sparse, a dense array or a file path)")
         arr = np.asarray(m)
         if arr.dtype == object:
             raise InvalidInputError(f"{name}: cannot interpret {type(m).__name__} as a numeric matrix")
@@ -184,20 +209,29 @@ def _matrix(m, name: str, storage: str, notes: list | None = None, hermitian_tol
     return _mirror(M, storage)
 
 
-def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = _Default(6),
-          mode: str = _Default("nearest"), lower: float | None = None, upper: float | None = None,
-          tol: float = _Default(1e-10), config=None, threads: int = _Default(0), verbose: int = 0,
-          storage: str = "auto") -> SolveResult:
+def _records_keywords(fn):
+    """Calls fn with _given = the set of keyword names the caller wrote (values aside)."""
+    @functools.wraps(fn)
+    def call(*args, **kwargs):
+        return fn(*args, _given=frozenset(kwargs), **kwargs)
+    return call
+
+
+@_records_keywords
+def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = 6, mode: str = "nearest",
+          lower: float | None = None, upper: float | None = None, tol: float = 1e-10, config=None,
+          threads: int = 0, verbose: int = 0, storage: str = "auto", _given: frozenset = frozenset()) -> SolveResult:
     """Solve A x = lambda B x (B=None: A x = lambda x) for Hermitian A and HPD B on the GPU.
 
-    Arguments as hsolve.solve (python/hsolve/__init__.py:347-419). Raises an HsolveError subclass
-    on failure; a run that finishes without meeting tol returns with success False and issues an
-    HsolveWarning.
+    Arguments as hsolve.solve (python/hsolve/__init__.py:347-419); the keyword rules are in the
+    module docstring. Raises an HsolveError subclass on failure; a run that finishes without
+    meeting tol returns with success False and issues an HsolveWarning.
     """
-    if storage not in _STORAGES:
-        raise InvalidInputError(f"storage = {storage!r} is not one of {_STORAGES}")
-    cfg = merge_config(config, sigma, k, mode, lower, upper, tol, threads)
-    _integer(verbose, "verbose")
+    if storage not in STORAGES:
+        raise InvalidInputError(f"storage = {storage!r} is not one of {STORAGES}")
+    cfg = merge_config(config, _given, sigma=sigma, k=k, mode=mode, lower=lower, upper=upper, tol=tol,
+                       threads=threads)
+    as_int(verbose, "verbose")
     notes: list = []
     Am = _matrix(A, "A", storage, notes, cfg["hermitian_tol"])
     Bm = _matrix(B, "B", storage, notes, cfg["hermitian_tol"])
@@ -213,7 +247,7 @@ def solve(A: Any, B: Any = None, *, sigma: float | None = None, k: int = _Defaul
                       np.zeros((v.shape[0], 0), dtype=np.complex128, order="F"), report, cfg_out)
     if not res.success:
         warnings.warn("hsolve: the solve finished without meeting the tolerance (see result.report['pairs'] and "
-                      "result.report['warnings'])", HsolveWarning, stacklevel=2)
+                      "result.report['warnings'])", HsolveWarning, stacklevel=3)
     return res
 
 

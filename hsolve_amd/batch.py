@@ -28,29 +28,33 @@ from .solver import _one_blas_thread, _solve, prepare
 def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host", share_analysis: bool = True):
     """pencils: a list of (A, B) (B may be None) or of solver.Prepared; config: one configuration
     for all, or a list with one per pencil. Returns a list of (eigenvalues, eigenvectors, report,
-    config) in the pencils' order. Every factorization keeps its numeric storage until the batch
-    ends (a free in the middle would synchronize the device and stall the other workers)."""
+    config) in the pencils' order.
+
+    Device memory: the analyses (one per distinct pattern) live until the batch ends; a pencil's
+    numeric factorization is a clone of its pattern's analysis made when a worker takes the pencil
+    and freed as soon as its solve ends (also when it fails), so at most `workers` factorizations
+    are held at a time whatever the number of pencils. The freed blocks stay in the library's
+    cache, ordered after the worker's stream, and the next pencil's clone reuses them."""
     n = len(pencils)
     cfgs = config if isinstance(config, list) else [config] * n
     if len(cfgs) != n:
         raise ValueError("solve_batch: one configuration per pencil")
     cfgs = [cfgmod.parse_config(c or {}) for c in cfgs]
     prepared = [p if hasattr(p, "pencil") else prepare(p[0], p[1], cfgs[i]) for i, p in enumerate(pencils)]
-    # one analysis per distinct factorized pattern, clones for the other pencils of that pattern
+    # one analysis per distinct factorized pattern (analysis objects are never factorized themselves)
     heads: list[GpuFactorization] = []
-    facs = []
+    head_of = []
     for p in prepared:
         head = next((f for f in heads if same_pattern(f.pencil, p.pencil)), None) if share_analysis else None
         if head is None:
             head = GpuFactorization(p.pencil)
             heads.append(head)
-            facs.append(head)
-        else:
-            facs.append(head.clone_for(p.pencil))
+        head_of.append(head)
     results = [None] * n
     errors = [None] * n
     nxt = [0]
     lock = threading.Lock()
+    live = [0, 0]  # factorizations held now, most held at once
     workers = max(1, min(int(workers), n))
     streams = stream_pool.checkout(workers)
 
@@ -64,10 +68,20 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
                         i = n
                 if i >= n:
                     break
+                fac = None
                 try:
-                    results[i] = _solve(None, None, cfgs[i], prepared[i], vectors, 1, facs[i])
+                    fac = head_of[i].clone_for(prepared[i].pencil)
+                    with lock:
+                        live[0] += 1
+                        live[1] = max(live[1], live[0])
+                    results[i] = _solve(None, None, cfgs[i], prepared[i], vectors, 1, fac)
                 except BaseException as e:  # re-raised in pencil order below
                     errors[i] = e
+                finally:
+                    if fac is not None:
+                        fac.close()
+                        with lock:
+                            live[0] -= 1
             streams[w].synchronize()
 
     try:
@@ -81,13 +95,14 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
             torch.cuda.synchronize()
     finally:
         stream_pool.checkin(streams)
+        for h in heads:
+            h.close()
     for e in errors:
         if e is not None:
             raise e
-    for f in facs:
-        f.release_factors()
-    for i, r in enumerate(results):
+    for r in results:
         r[2].pop("_events", None)
         r[2]["parallel"]["notes"].append(
-            f"batch of {n} pencils on {workers} concurrent streams; {len(heads)} analyses shared by pattern")
+            f"batch of {n} pencils on {workers} concurrent streams; {len(heads)} analyses shared by pattern; "
+            f"at most {live[1]} factorizations held at once")
     return results

@@ -178,12 +178,12 @@ int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_valu
   });
 }
 
-int hsg_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream) {
+int hsg_factor_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream) {
   return guarded([&] {
-    require(f && x, "hsg_solve: NULL argument");
-    require(nrhs >= 0, "hsg_solve: nrhs < 0");
-    require(ldx >= f->info.n, "hsg_solve: ldx < n");
-    if (!hsg::Engine::has_factor(f->num)) throw std::runtime_error("hsg_solve: no usable factorization");
+    require(f && x, "hsg_factor_solve: NULL argument");
+    require(nrhs >= 0, "hsg_factor_solve: nrhs < 0");
+    require(ldx >= f->info.n, "hsg_factor_solve: ldx < n");
+    if (!hsg::Engine::has_factor(f->num)) throw std::runtime_error("hsg_factor_solve: no usable factorization");
     f->engine->solve(f->num, x, nrhs, ldx, as_stream(stream));
   });
 }
@@ -196,6 +196,13 @@ int hsg_release_factors(hsg_factor* f) {
 }
 
 void hsg_factor_destroy(hsg_factor* f) { delete f; }
+
+int hsg_release_cached_memory(int64_t* bytes) {
+  return guarded([&] {
+    const size_t b = hsg::release_cached_blocks();
+    if (bytes) *bytes = (int64_t)b;
+  });
+}
 
 int hsg_factor_clone(const hsg_factor* src, hsg_factor** out) {
   return guarded([&] {

@@ -35,6 +35,8 @@ struct Numeric {
   Numeric(const Numeric&) = delete;
   Numeric& operator=(const Numeric&) = delete;
   void free_all();
+  void note_stream(hipStream_t st);  // a stream this factorization's kernels were queued on
+  std::vector<hipStream_t> streams_;  // streams whose queued work may use the buffers below
   int precision_ = -1;
   bool valid_ = false;
   void *d_L_ = nullptr, *d_CB_ = nullptr, *d_W_ = nullptr, *d_dval_ = nullptr, *d_doff_ = nullptr;
@@ -135,6 +137,9 @@ class Engine {
   int64_t part_slots_ = 0;       // slots per group of kMaxRhsChunk right-hand sides
   mutable std::mutex device_mu_;  // ensure_device from factorizations running on several threads
 };
+
+// Returns the idle cached device blocks of the current device to the driver; the bytes returned.
+size_t release_cached_blocks();
 
 // Throws std::runtime_error with the HIP error string on failure.
 void hip_check(hipError_t e, const char* what);

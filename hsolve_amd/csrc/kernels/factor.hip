@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <list>
 #include <mutex>
 #include <stdexcept>
@@ -462,9 +463,10 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ rel,
 }
 
 // Extend-add into the large fronts of one level: workgroup (x, y) adds, child by child, the
-// entries of the children's update matrices that land in 16 parent columns of front list[y]:
-// part 0 covers the panel columns [16 x, 16 x + 16) (before the front is factorized), part 1 the
-// columns ncol + [16 x, 16 x + 16) of the front's own update matrix, after the front's update
+// entries of the children's update matrices that land in kEaCols (8) parent columns of front
+// list[y]: part 0 covers the panel columns [kEaCols x, kEaCols x + kEaCols) (before the front is
+// factorized), part 1 the columns ncol + [kEaCols x, kEaCols x + kEaCols) of the front's own
+// update matrix, after the front's update
 // product has been stored there. The child columns that map there are a contiguous range
 // (relmap is increasing); the four waves find the ranges of two children at a time, one end
 // each. A child's (column, row) pairs of the lower triangle are then spread over the workgroup,
@@ -1015,13 +1017,37 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
 //   mode 0 (trailing update of panel p): rows [c1, m) x cols [c1, ncol) of the panel,
 //          A = W(:, c0:c1), B = L(:, c0:c1);
 //   mode 1 (contribution block): rows/cols [ncol, m) into CB, A = W(:, 0:ncol), B = L(:, 0:ncol).
-// 64 x 64 output tiles, 256 threads with 4 x 4 outputs each, K staged through LDS in steps of 16.
+// 64 x 64 output tiles, K staged through LDS in steps of 16. The products run on the matrix cores:
+// each of the four waves owns a 32 x 32 quarter of the tile as 2 x 2 blocks of 16 x 16, one
+// V_MFMA_*_16X16X4 per block and 4-wide K slice (FP64: v_mfma_f64_16x16x4_f64, FP32:
+// v_mfma_f32_16x16x4_f32, accumulation in the factor's precision). The operands are swapped
+// (the matrix core's row operand is the L side, its column operand the W side) so that a lane's
+// accumulator column indexes a tile row: lane l holds rows 16 b + l % 16 of the columns
+// 4 (l / 16) + v, v = 0..3, and the 16 lanes of one column store 128 contiguous bytes.
 constexpr int kTile = 64;
 constexpr int kKStep = 16;
 
 template <class T>
+struct MatrixCore;
+template <>
+struct MatrixCore<double> {
+  typedef double acc4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc4 fma(double a, double b, acc4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MatrixCore<float> {
+  typedef float acc4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc4 fma(float a, float b, acc4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <class T>
 __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                          int mode, int p, int nb) {
+  typedef typename MatrixCore<T>::acc4 acc4;
   const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s];
   int row0, col0, k0, kn;
@@ -1053,12 +1079,14 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
   __shared__ T As[kKStep][kTile + 1];
   __shared__ T Bs[kKStep][kTile + 1];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  T acc[4][4];
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);  // the wave's quarter of the tile
+  const int l16 = lane & 15, lk = lane >> 4;
+  acc4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc4{(T)0, (T)0, (T)0, (T)0};
   const int rbase = row0 + ti * kTile, cbase = col0 + tj * kTile;
   const int clim = mode == 0 ? ncol : m;
   // staging: thread (lr, lt) carries row / column lr of the tile for k offsets lt + 4 v; the next
@@ -1090,16 +1118,17 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
     __syncthreads();
     if (kk + kKStep < kn) stage(kk + kKStep);
 #pragma unroll
-    for (int t = 0; t < kKStep; ++t) {
-      T av[4], bv[4];
+    for (int q = 0; q < kKStep / 4; ++q) {
+      const int kq = 4 * q + lk;
+      T wv[2], lv[2];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[t][tx + 16 * a];
+      for (int a = 0; a < 2; ++a) wv[a] = As[kq][wr + 16 * a + l16];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = Bs[t][ty + 16 * b];
+      for (int b = 0; b < 2; ++b) lv[b] = Bs[kq][wc + 16 * b + l16];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+        for (int b = 0; b < 2; ++b) acc[a][b] = MatrixCore<T>::fma(lv[b], wv[a], acc[a][b]);
     }
     __syncthreads();
   }
@@ -1108,22 +1137,26 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
   // product is stored as it is: C = -acc, with nothing read and no zero fill beforehand.
   const int cb = m - ncol;
   T* const Cbase = mode == 0 ? N.L + S.panel_off[s] : N.CB + S.cb_off[s];
-  T* dst[4][4];
-  T cv[4][4];
+  T* dst[2][2][4];
+  T cv[2][2][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int r = rbase + tx + 16 * a, c = cbase + ty + 16 * b;
-      const bool ok = r < m && r >= c && c < clim;
-      dst[a][b] = ok ? Cbase + (mode == 0 ? (int64_t)c * m + r : (int64_t)(c - ncol) * cb + (r - ncol)) : nullptr;
-      cv[a][b] = (ok && mode == 0) ? *dst[a][b] : (T)0;
-    }
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+      for (int v = 0; v < 4; ++v) {
+        const int r = rbase + wr + 16 * a + l16, c = cbase + wc + 16 * b + 4 * lk + v;
+        const bool ok = r < m && r >= c && c < clim;
+        dst[a][b][v] = ok ? Cbase + (mode == 0 ? (int64_t)c * m + r : (int64_t)(c - ncol) * cb + (r - ncol)) : nullptr;
+        cv[a][b][v] = (ok && mode == 0) ? *dst[a][b][v] : (T)0;
+      }
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (dst[a][b]) *dst[a][b] = cv[a][b] - acc[a][b];
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (dst[a][b][v]) *dst[a][b][v] = cv[a][b][v] - acc[a][b][v];
 }
 
 // ------------------------------------------------------------------------------ host engine
@@ -1137,25 +1170,36 @@ namespace {
 // Device blocks of the analysis and of factorization objects are kept for reuse instead of being
 // returned to the driver: repeated solves of a pattern allocate the same sizes, and after a few
 // hundred GB of hipMalloc / hipFree cycles the driver's calls took up to seconds each. A cached
-// block is handed out again only for exactly its size. Freeing synchronizes the device first (as
-// hipFree does), so no queued kernel still uses a block when a new owner receives it. At most
-// kCacheCap idle bytes stay cached per device (the longest-idle blocks go back first); when
-// hipMalloc runs out of memory, the device's idle blocks are returned and the call is retried.
+// block is handed out again only for exactly its size. A block freed with the streams that used it
+// carries one event per stream, recorded at the free: its next owner waits for those events (the
+// work queued on those streams before the free), not for the whole device, so a worker that frees
+// its factorization does not stall the others. A block freed without streams synchronizes the
+// device first (as hipFree does). At most cap() idle bytes stay cached per device (the longest-idle
+// blocks go back first; HSG_CACHE_CAP_GB sets the cap, default 32 GiB); hsg_release_cached_memory
+// returns every idle block, and when hipMalloc runs out of memory the device's idle blocks are
+// returned and the call is retried.
 class BlockCache {
  public:
   void* alloc(size_t bytes, const char* what) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), what);
+    Block hit;
+    bool found = false;
     {
       std::lock_guard<std::mutex> lk(mu_);
       for (auto it = idle_.begin(); it != idle_.end(); ++it)
         if (it->dev == dev && it->bytes == bytes) {
-          void* p = it->ptr;
+          hit = *it;
           idle_bytes_[dev] -= bytes;
           idle_.erase(it);
-          live_[p] = {dev, bytes};
-          return p;
+          live_[hit.ptr] = {dev, bytes};
+          found = true;
+          break;
         }
+    }
+    if (found) {
+      wait_events(hit);
+      return hit.ptr;
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -1169,36 +1213,63 @@ class BlockCache {
     live_[p] = {dev, bytes};
     return p;
   }
-  void free(void* p) {
+  // streams: the streams whose queued work may still use the block (empty: synchronize the device)
+  void free(void* p, const std::vector<hipStream_t>& streams) {
     Block b;
     {
       std::lock_guard<std::mutex> lk(mu_);
       auto it = live_.find(p);
       if (it == live_.end()) return;
-      b = {it->second.dev, it->second.bytes, p};
+      b.dev = it->second.dev;
+      b.bytes = it->second.bytes;
+      b.ptr = p;
       live_.erase(it);
     }
     int cur = 0;
     (void)hipGetDevice(&cur);
     if (cur != b.dev) (void)hipSetDevice(b.dev);
-    (void)hipDeviceSynchronize();
-    std::vector<void*> evict;
+    bool synced = streams.empty();
+    for (hipStream_t st : streams) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, st) != hipSuccess) {
+        if (ev) (void)hipEventDestroy(ev);
+        (void)hipGetLastError();
+        synced = true;  // no event: fall back to a device synchronization
+        break;
+      }
+      b.events.push_back(ev);
+    }
+    if (synced) {
+      for (hipEvent_t ev : b.events) (void)hipEventDestroy(ev);
+      b.events.clear();
+      (void)hipDeviceSynchronize();
+    }
+    std::vector<Block> evict;
     {
       std::lock_guard<std::mutex> lk(mu_);
       idle_.push_back(b);
       idle_bytes_[b.dev] += b.bytes;
-      for (auto it = idle_.begin(); it != idle_.end() && idle_bytes_[b.dev] > kCacheCap;) {
+      for (auto it = idle_.begin(); it != idle_.end() && idle_bytes_[b.dev] > cap();) {
         if (it->dev == b.dev) {
           idle_bytes_[b.dev] -= it->bytes;
-          evict.push_back(it->ptr);
+          evict.push_back(*it);
           it = idle_.erase(it);
         } else {
           ++it;
         }
       }
     }
-    for (void* q : evict) (void)hipFree(q);
+    for (Block& q : evict) {
+      wait_events(q);
+      (void)hipFree(q.ptr);
+    }
     if (cur != b.dev) (void)hipSetDevice(cur);
+  }
+  // Returns the idle blocks of the current device to the driver; the bytes returned.
+  size_t release_current() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    return release_idle(dev);
   }
 
  private:
@@ -1206,19 +1277,36 @@ class BlockCache {
     int dev = 0;
     size_t bytes = 0;
     void* ptr = nullptr;
+    std::vector<hipEvent_t> events;  // work that may still use the block
   };
   struct Owner {
     int dev;
     size_t bytes;
   };
-  static constexpr size_t kCacheCap = size_t(128) << 30;
-  void release_idle(int dev) {
-    std::vector<void*> out;
+  static size_t cap() {
+    static const size_t c = [] {
+      const char* e = std::getenv("HSG_CACHE_CAP_GB");
+      const double gb = e ? std::atof(e) : 32.0;
+      return gb > 0 ? (size_t)(gb * double(size_t(1) << 30)) : size_t(0);
+    }();
+    return c;
+  }
+  static void wait_events(Block& b) {
+    for (hipEvent_t ev : b.events) {
+      (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+    }
+    b.events.clear();
+  }
+  size_t release_idle(int dev) {
+    std::vector<Block> out;
+    size_t bytes = 0;
     {
       std::lock_guard<std::mutex> lk(mu_);
       for (auto it = idle_.begin(); it != idle_.end();) {
         if (it->dev == dev) {
-          out.push_back(it->ptr);
+          bytes += it->bytes;
+          out.push_back(*it);
           it = idle_.erase(it);
         } else {
           ++it;
@@ -1226,8 +1314,11 @@ class BlockCache {
       }
       idle_bytes_[dev] = 0;
     }
-    (void)hipDeviceSynchronize();
-    for (void* q : out) (void)hipFree(q);
+    for (Block& q : out) {
+      wait_events(q);
+      (void)hipFree(q.ptr);
+    }
+    return bytes;
   }
   std::mutex mu_;
   std::unordered_map<void*, Owner> live_;
@@ -1256,7 +1347,18 @@ V* upload(const std::vector<V>& h) {
 }
 
 void dfree(void* p) {
-  if (p) block_cache().free(p);
+  if (p) block_cache().free(p, {});
+}
+
+}  // namespace
+
+size_t release_cached_blocks() { return block_cache().release_current(); }
+
+namespace {
+
+// frees a block that only work queued on `streams` may still use
+void dfree_after(void* p, const std::vector<hipStream_t>& streams) {
+  if (p) block_cache().free(p, streams);
 }
 
 int blocks_for(int64_t n, int per = 256, int cap = 65536) {
@@ -1569,11 +1671,18 @@ Engine::~Engine() {
 
 Numeric::~Numeric() { free_all(); }
 
+void Numeric::note_stream(hipStream_t st) {
+  for (hipStream_t s : streams_)
+    if (s == st) return;
+  streams_.push_back(st);
+}
+
 void Numeric::free_all() {
   for (void* p : {d_L_, d_CB_, d_W_, d_dval_, d_doff_, (void*)d_ptype_, (void*)d_piv_, (void*)d_ipiv_,
                   (void*)d_bperm_, (void*)d_cnt_, d_Xp_, d_Y_, d_U_, d_G_, d_Gt_, d_T_, d_part_,
                   (void*)d_pos_, (void*)d_rpos_, (void*)d_zpull_})
-    dfree(p);
+    dfree_after(p, streams_);
+  streams_.clear();
   d_L_ = d_CB_ = d_W_ = d_dval_ = d_doff_ = d_Xp_ = d_Y_ = d_U_ = d_G_ = d_Gt_ = d_T_ = d_part_ = nullptr;
   d_pos_ = d_rpos_ = nullptr;
   d_zpull_ = nullptr;
@@ -1688,6 +1797,7 @@ InertiaResult Engine::factorize(Numeric& nm, double alpha, double beta, const do
   nm.g_ready_ = false;
   ensure_device();
   alloc_numeric(nm, precision);
+  nm.note_stream(stream);
   InertiaResult r = precision == 0 ? factorize_t<double>(nm, alpha, beta, av, bv, null_tol, stream)
                                    : factorize_t<float>(nm, alpha, beta, av, bv, null_tol, stream);
   nm.valid_ = !r.singular;
@@ -1744,7 +1854,7 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
   if (nm.ws_cols_ < nrhs) {
-    for (void* q : {nm.d_Xp_, nm.d_Y_, nm.d_U_, nm.d_part_}) dfree(q);
+    for (void* q : {nm.d_Xp_, nm.d_Y_, nm.d_U_, nm.d_part_}) dfree_after(q, nm.streams_);
     nm.d_Xp_ = nm.d_Y_ = nm.d_U_ = nm.d_part_ = nullptr;
     const int64_t nrc = (nrhs + kMaxRhsChunk - 1) / kMaxRhsChunk;
     dalloc_into(nm.d_Xp_, (size_t)sym_.n * nrhs * sizeof(T), "hipMalloc (solve workspace)");
@@ -1813,6 +1923,7 @@ void Engine::solve(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_
   if (!nm.valid_) throw std::runtime_error("solve without a valid factorization");
   if (nrhs <= 0) return;
   if (ldx < sym_.n) throw std::runtime_error("solve: ldx < n");
+  nm.note_stream(stream);
   if (nm.precision_ == 0) solve_t<double>(nm, x, nrhs, ldx, stream);
   else solve_t<float>(nm, x, nrhs, ldx, stream);
 }

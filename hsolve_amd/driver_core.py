@@ -27,7 +27,6 @@ import torch
 from .errors import InternalError, SingularError
 from .factor import GpuFactorization, Pencil
 from .krylov import KrylovOptions, lanczos_krylov_schur
-from . import hostla
 from .ops import DeviceOps, InexactFactorFailure, ShiftInvertOperator, SolveRefinement
 
 EPS = float(np.finfo(np.float64).eps)
@@ -205,13 +204,13 @@ def rayleigh_ritz_definite(blk: Block, drop_tol: float = DEPENDENT_GRAM) -> int:
     h = ops.gram(blk.x, blk.bx).cpu().numpy().T
     g = 0.5 * (g + g.conj().T)
     h = 0.5 * (h + h.conj().T)
-    mu, V = hostla.eigh(h)
+    mu, V = np.linalg.eigh(h)
     keep = [q for q in range(m) if mu[q] > drop_tol * mu[-1] and mu[q] > 0.0]
     r = len(keep)
     Y = V[:, keep] / np.sqrt(mu[keep])
     gr = Y.conj().T @ g @ Y
     gr = 0.5 * (gr + gr.conj().T)
-    theta, Z = hostla.eigh(gr)
+    theta, Z = np.linalg.eigh(gr)
     coef = Y @ Z
     blk.x = _replace_by_product(ops, blk.x, coef)
     blk.ax = _replace_by_product(ops, blk.ax, coef)

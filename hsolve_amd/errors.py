@@ -1,10 +1,14 @@
-"""Exception classes of the hsolve API (python/hsolve/__init__.py:48-130), keyed by the
-hs::Error::Code names (include/hsolve/error.hpp:15-25)."""
+"""Exception classes raised by hsolve_amd.
+
+The class names, their base classes and the ``code`` strings are the reference's Python error
+contract (python/hsolve/__init__.py:48-130), so callers that catch the reference's errors catch
+these; the code names follow hs::Error::Code (include/hsolve/error.hpp:15-25), whose order is also
+the C ABI's error numbering (include/hsg.h: code = 1 + position in ERROR_CLASSES)."""
 from __future__ import annotations
 
 
 class HsolveError(Exception):
-    """Base class of every hsolve error. ``code`` is the error code name (snake_case)."""
+    """Root of the hierarchy; ``code`` names the failure category in snake_case."""
 
     code = "unknown"
 
@@ -15,59 +19,64 @@ class HsolveError(Exception):
 
 
 class InvalidInputError(HsolveError, ValueError):
-    """Malformed or unsupported input: bad shapes, dtypes, configuration values, k, ..."""
+    """The request itself is wrong: a non-square or non-numeric matrix, an out-of-range k, a
+    configuration key with a bad value."""
 
     code = "invalid_input"
 
 
 class HsolveIOError(HsolveError, OSError):
-    """File system or file format error (matrix files, config.json)."""
+    """Reading a matrix or config file failed, or its contents are not in a readable format."""
 
     code = "io"
 
 
 class NotHermitianError(HsolveError, ValueError):
-    """A (or B) fails the Hermitian check: ||M - M^H||_F / ||M||_F > hermitian_tol."""
+    """A or B is further from Hermitian than hermitian_tol allows (relative Frobenius defect)."""
 
     code = "not_hermitian"
 
 
 class UnsupportedError(HsolveError, NotImplementedError):
-    """A valid request this build cannot serve (e.g. indefinite B)."""
+    """The request is well formed but outside what this engine solves, for instance a B that is
+    not positive definite."""
 
     code = "unsupported"
 
 
 class FactorizationFailedError(HsolveError, RuntimeError):
-    """The sparse LDL^T factorization failed."""
+    """The sparse LDL^T of A - sigma B could not be completed."""
 
     code = "factorization_failed"
 
 
 class SingularError(HsolveError, ArithmeticError):
-    """A - sigma B is numerically singular at and near the requested shift."""
+    """A - sigma B stayed numerically singular at the shift and at every nearby shift tried."""
 
     code = "singular"
 
 
 class OutOfMemoryBudgetError(HsolveError, MemoryError):
-    """The predicted memory exceeds the memory budget."""
+    """The factorization needs more memory than the budget (device memory here) provides."""
 
     code = "out_of_memory_budget"
 
 
 class NotConvergedError(HsolveError, RuntimeError):
+    """The eigensolver ran out of iterations before the requested pairs converged."""
+
     code = "not_converged"
 
 
 class InternalError(HsolveError, RuntimeError):
-    """A broken internal invariant (a bug)."""
+    """An invariant of the engine itself was violated: a defect in this package."""
 
     code = "internal"
 
 
 class HsolveWarning(UserWarning):
-    """Issued when a solve finishes without meeting the tolerance (report['success'] is False)."""
+    """Warns that solve() returned pairs without reaching the requested tolerance
+    (report['success'] is False)."""
 
 
 ERROR_CLASSES = (InvalidInputError, HsolveIOError, NotHermitianError, UnsupportedError, FactorizationFailedError,

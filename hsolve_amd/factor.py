@@ -5,7 +5,7 @@ one union pattern of A and B serves every shift. GpuFactorization mirrors the fa
 contract SparseFactorization<T> (include/hsolve/factor.hpp:341-443): analyze once, factorize
 for any (alpha, beta), a singular K makes factorize() return False (never raises for null
 pivots), count() gives the Sylvester inertia, solve() works in place on column-major blocks.
-The numeric work runs in libhsg on the GPU (hsg_factorize / hsg_solve); device buffers are torch
+The numeric work runs in libhsg on the GPU (hsg_factorize / hsg_factor_solve); device buffers are torch
 tensors. Blocks of vectors are torch tensors of shape (p, n): row c is column c of the n x p
 column-major block the C ABI sees (leading dimension n).
 """
@@ -279,8 +279,11 @@ class GpuFactorization:
                   host_ptr(out["m"]), host_ptr(out["parent"]), host_ptr(out["rowoff"]), host_ptr(out["rows"]))
         return out
 
-    def estimate(self) -> dict:
-        bytes_per = 8 if self.precision == "double" else 4
+    def estimate(self, precision: str | None = None) -> dict:
+        """Estimates of the analysis (FactorEstimate, include/hsolve/factor.hpp:302-310): factor entries,
+        flops and the device memory (MB) of one factorization in `precision` (default: the
+        object's): panels plus contribution blocks."""
+        bytes_per = 8 if (precision or self.precision) == "double" else 4
         return {"factor_entries": int(self.info.factor_entries), "flops": float(self.info.flops),
                 "incore_mb": (self.info.panel_bytes_double + self.info.cb_bytes_double) / 8 * bytes_per / 1e6,
                 "ooc_mb": 0.0}
@@ -340,7 +343,7 @@ class GpuFactorization:
         if x2.dtype != want or not x2.is_contiguous() or x2.shape[1] != self.pencil.n:
             raise InvalidInputError(f"solve expects a contiguous {want} (p, n) device tensor")
         t0 = time.perf_counter()
-        _lib.call("hsg_solve", self._handle, ptr(x2), x2.shape[0], self.pencil.f_n, stream_ptr())
+        _lib.call("hsg_factor_solve", self._handle, ptr(x2), x2.shape[0], self.pencil.f_n, stream_ptr())
         self.stats.solve_s += time.perf_counter() - t0
         self.stats.solves += 1
         self.stats.rhs_solved += int(x2.shape[0])

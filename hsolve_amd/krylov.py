@@ -23,10 +23,15 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.linalg as sla
 import torch
 
-from . import hostla
 from .ops import DeviceOps
+
+# Host LAPACK (projected eigenproblems, Cholesky factors of small Gram matrices) is called from the
+# threads of concurrent spectrum slices without a lock: numpy's and scipy's bundled LAPACK builds
+# are reentrant, and a lock would make one slice's 8 x 8 Cholesky wait behind another slice's
+# Rayleigh-Ritz eigensolve.
 
 EPS = float(np.finfo(np.float64).eps)
 BREAKDOWN_TOL = 64 * EPS      # block_lanczos.hpp:29
@@ -233,7 +238,7 @@ class BlockLanczos:
         if np.all(np.diag(G).real > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
             for it in range(2):
                 try:
-                    Rk = hostla.cholesky_upper(G)
+                    Rk = sla.cholesky(G, lower=False)
                 except np.linalg.LinAlgError:
                     ok = False
                     break
@@ -241,7 +246,7 @@ class BlockLanczos:
                 if d.min() <= 1e-7 * d.max():  # too ill-conditioned for Cholesky-QR
                     ok = False
                     break
-                Rinv = hostla.inv(Rk)
+                Rinv = np.linalg.inv(Rk)
                 tmp = ops.empty(p)
                 ops.update(w, ops.coef(Rinv), tmp, 1.0, 0.0)
                 w.copy_(tmp)
@@ -332,7 +337,7 @@ class BlockLanczos:
         m, p = self.m, self.p
         Hm = self.H[:m, :m]
         Hm = 0.5 * (Hm + Hm.conj().T)
-        theta, S = hostla.eigh(Hm)
+        theta, S = np.linalg.eigh(Hm)
         self.coupling = self.rlast @ S[m - p:m, :]
         est = np.linalg.norm(self.coupling, axis=0)
         self.rr = RitzPairs(theta, S, est, self.opt.tol, self.which)

@@ -202,10 +202,10 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     ctx = Ctx(cfg=cfg, rep=rep, clock=clock, pencil=pencil, ops=ops, fac=fac, n2a=n2a, n2b=n2b, scale=scale)
     ctx.streams = slice_streams(fac, streams)
     requested = cfg["factor_precision"]
-    ctx.precision = "single" if requested == "single" else "double"
     rep["precision"]["requested"] = requested
-    rep["precision"]["decision"] = ("double analysis expected" if requested != "auto" else
-                                    "double: in-core estimate fits the device memory")
+    ctx.precision, rep["precision"]["decision"] = choose_precision(fac, cfg, ctx.streams)
+    if requested == "auto" and ctx.precision == "single":
+        ctx.warn("factor_precision auto: single-precision factors (" + rep["precision"]["decision"] + ")")
     mode = cfg["target"]["mode"]
     result = None
     if ctx.precision == "single":
@@ -278,6 +278,41 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
         pass
     rep["_events"] = ctx.events
     return lam, vecs, rep, cfg
+
+
+def memory_budget_mb(cfg) -> tuple[float, str]:
+    """The device memory a solve may hold (MB) and where the number comes from: memory_budget_gb
+    when set, else the device memory free now (hipMemGetInfo)."""
+    mb = cfg["memory_budget_gb"]
+    if mb != "auto" and mb > 0:
+        return float(mb) * 1024.0, "memory_budget_gb"
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    return free / 2 ** 20, "device free memory"
+
+
+def choose_precision(fac: GpuFactorization, cfg, slots: int = 1) -> tuple[str, str]:
+    """Arithmetic of the working factorizations and the reason (report precision.decision).
+
+    "single" and "double" are taken as requested. "auto" follows the reference's rule
+    (choose_precision, src/driver/steps.cpp:149-180): double factors when the in-core estimate of
+    the concurrently held factorizations (`slots`, one per worker stream) fits the memory budget,
+    single when only the single estimate fits, double otherwise (then the allocation reports the
+    shortage). Certificate counts are double in every case."""
+    requested = cfg["factor_precision"]
+    if requested != "auto":
+        return requested, f"{requested} requested"
+    est_d = fac.estimate("double")["incore_mb"] * max(1, slots)
+    est_s = fac.estimate("single")["incore_mb"] * max(1, slots)
+    cap, source = memory_budget_mb(cfg)
+    if est_d <= cap:
+        return "double", (f"double: in-core estimate {est_d:.0f} MB for {slots} factorization(s) fits the "
+                          f"{source} ({cap:.0f} MB)")
+    if est_s <= cap:
+        return "single", (f"single: in-core estimates double {est_d:.0f} MB, single {est_s:.0f} MB, "
+                          f"{source} {cap:.0f} MB")
+    return "double", (f"double: in-core estimates double {est_d:.0f} MB, single {est_s:.0f} MB, both above the "
+                      f"{source} ({cap:.0f} MB)")
 
 
 def _solve_definite(ctx: Ctx, mode: str):

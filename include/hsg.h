@@ -131,12 +131,20 @@ int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_valu
  * device with the last successful factorization (replaces solve, factor.hpp:394-399; MUMPS JOB=3
  * at mumps_factorization.cpp:506). x is FP64; a single-precision factor rounds it to FP32 for
  * the triangular solves (as the reference's single factorization does). Asynchronous on stream. */
-int hsg_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream);
+int hsg_factor_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream);
 
 /* Frees the numerical factors, keeps the analysis (release_factors, factor.hpp:411-418). */
 int hsg_release_factors(hsg_factor* f);
 
 void hsg_factor_destroy(hsg_factor* f);
+
+/* Device blocks freed by factorization objects and analyses are kept by the library for reuse
+ * (same sizes recur from solve to solve). This returns the idle ones of the current device to the
+ * driver and writes the byte count to *bytes (may be NULL). No counterpart in the reference, whose
+ * factor memory is MUMPS' own; a serving loop calls it when it goes idle so that other processes
+ * on the device can use the memory. The cache holds at most HSG_CACHE_CAP_GB (environment,
+ * default 32) GiB of idle blocks per device. */
+int hsg_release_cached_memory(int64_t* bytes);
 
 /* A new factorization object over the analysis of src (shared, not recomputed): its own numeric
  * storage, the same pattern, ordering and supernodal plan. The counterpart of the reference's

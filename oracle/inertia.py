@@ -8,8 +8,8 @@ and uses in count_below (src/driver/steps.cpp:549-568).
 * sparse_inertia: SuperLU (scipy) in symmetric mode with diagonal pivots only, so P K P^T = L U
   with U = D L^T; the signs of diag(U) give the inertia by Sylvester's law. An independent sparse
   elimination (SuperLU's own ordering and supernodes), usable at 1e4..1e5 unknowns.
-Pinned by tests/test_oracle_inertia.py: both agree with each other and with the closed-form Yee
-spectra (oracle/yee_exact.py) at gap points.
+Pinned by tests/test_oracle_yee.py::test_sparse_inertia_matches_dense_and_closed_form: both
+agree with each other and with the closed-form Yee spectra (oracle/yee_exact.py) at gap points.
 """
 from __future__ import annotations
 

@@ -21,7 +21,7 @@ def _declared():
 
 def test_header_declares_the_entry_points():
     decl = _declared()
-    for name in ("hsg_analyze", "hsg_factorize", "hsg_solve", "hsg_release_factors", "hsg_factor_destroy",
+    for name in ("hsg_analyze", "hsg_factorize", "hsg_factor_solve", "hsg_release_factors", "hsg_factor_destroy",
                  "hsg_last_error_message", "hsg_version", "hsg_timing_read"):
         assert name in decl
 
@@ -56,7 +56,7 @@ def test_binding_keeps_the_gil_only_across_launch_only_calls():
     import ctypes as C
     from hsolve_amd import _lib
     assert _lib.RELEASES_GIL <= set(_lib.SIGNATURES)
-    assert {"hsg_analyze", "hsg_factorize", "hsg_solve"} <= _lib.RELEASES_GIL
+    assert {"hsg_analyze", "hsg_factorize", "hsg_factor_solve"} <= _lib.RELEASES_GIL
     if not _lib.LIB_PATH.exists():
         pytest.skip("libhsg.so not built")
     b = _lib.Bound(_lib.LIB_PATH)
