@@ -1022,8 +1022,11 @@ __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N,
 // V_MFMA_*_16X16X4 per block and 4-wide K slice (FP64: v_mfma_f64_16x16x4_f64, FP32:
 // v_mfma_f32_16x16x4_f32, accumulation in the factor's precision). The operands are swapped
 // (the matrix core's row operand is the L side, its column operand the W side) so that a lane's
-// accumulator column indexes a tile row: lane l holds rows 16 b + l % 16 of the columns
-// 4 (l / 16) + v, v = 0..3, and the 16 lanes of one column store 128 contiguous bytes.
+// accumulator column indexes a tile row: lane l holds row 16 b + l % 16 of four columns, and the
+// 16 lanes of one column store 128 contiguous bytes. Which four: accumulator register v of lane l
+// is output row 4 (l / 16) + v of the FP32 instruction but row 4 v + l / 16 of the FP64 one
+// (MatrixCore::row; the same register layout composable_kernel's mfma_type tables describe as
+// group_size 4 x 1 group for f32_16x16x4 and group_size 1 x 4 groups for f64_16x16x4).
 constexpr int kTile = 64;
 constexpr int kKStep = 16;
 
@@ -1035,6 +1038,8 @@ struct MatrixCore<double> {
   static __device__ __forceinline__ acc4 fma(double a, double b, acc4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row (of the 16 x 16 block) of accumulator register v in lane group g = lane / 16
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * v + g; }
 };
 template <>
 struct MatrixCore<float> {
@@ -1042,6 +1047,7 @@ struct MatrixCore<float> {
   static __device__ __forceinline__ acc4 fma(float a, float b, acc4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
 };
 
 template <class T>
@@ -1145,7 +1151,7 @@ __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N,
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = rbase + wr + 16 * a + l16, c = cbase + wc + 16 * b + 4 * lk + v;
+        const int r = rbase + wr + 16 * a + l16, c = cbase + wc + 16 * b + MatrixCore<T>::row(lk, v);
         const bool ok = r < m && r >= c && c < clim;
         dst[a][b][v] = ok ? Cbase + (mode == 0 ? (int64_t)c * m + r : (int64_t)(c - ncol) * cb + (r - ncol)) : nullptr;
         cv[a][b][v] = (ok && mode == 0) ? *dst[a][b][v] : (T)0;
