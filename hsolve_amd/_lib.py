@@ -53,6 +53,7 @@ SIGNATURES = {
     "hsg_shifted_residual": (C.c_int, [_I64, _P, _P, _P, _P, _P, _P, _P, _D, _P, _I64, _P, _I64, _P, _I64, _I64,
                                        _P, _P, _P]),
     "hsg_gram": (C.c_int, [_I64, _P, _I64, _I64, _P, _P, _I64, _I64, _P, _I64, _P]),
+    "hsg_gram_noalloc": (C.c_int, [_I64, _P, _I64, _I64, _P, _P, _I64, _I64, _P, _I64, _P]),
     "hsg_block_update": (C.c_int, [_I64, _P, _I64, _I64, _P, _I64, _P, _I64, _I64, _D, _D, _P]),
     "hsg_column_dots": (C.c_int, [_I64, _P, _I64, _P, _I64, _I64, _P, _P, _P]),
     "hsg_axpby": (C.c_int, [_I64, _D, _P, _I64, _D, _P, _I64, _I64, _P]),

@@ -255,6 +255,16 @@ int hsg_gram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w
   });
 }
 
+int hsg_gram_noalloc(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w,
+                     int64_t ldw, int64_t p, double* c, int64_t ldc, void* stream) {
+  return guarded([&] {
+    require(ldv >= n && ldw >= n && ldc >= k && k >= 0 && p >= 0, "hsg_gram_noalloc: invalid sizes");
+    require(hsg::gram_noalloc_bytes(n, k, p) <= sizeof(double) * (size_t)hsg::kNoAllocScratchDoubles,
+            "hsg_gram_noalloc: k * p exceeds the fixed scratch (16 MiB of partial sums; use hsg_gram)");
+    hsg::gram_noalloc(n, v, ldv, k, w_diag, w, ldw, p, c, ldc, as_stream(stream));
+  });
+}
+
 int hsg_block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,
                      int64_t ldw, int64_t p, double alpha, double beta, void* stream) {
   return guarded([&] {

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -485,15 +486,19 @@ void shifted_residual_t(Scratch& scratch, int64_t n, const int64_t* arp, const i
   check(hipGetLastError(), "shifted_residual");
 }
 
+// Row groups of the Gram kernel for an n x k V: enough workgroups to fill the chip, each over many
+// row tiles (the partial sums take groups * p * k scalars).
 template <class S>
-void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, const double* wd, const S* w, int64_t ldw,
-            int64_t p, S* c, int64_t ldc, hipStream_t st) {
-  if (k <= 0 || p <= 0) return;
+int gram_groups(int64_t n, int64_t k) {
+  const int kt = (int)((k + kGramCols - 1) / kGramCols);
+  return grid_for(n, gram_rows<S>() * 4, std::max(32, 2048 / kt));
+}
+
+template <class S>
+void gram_launch(S* part, int groups, int64_t n, const S* v, int64_t ldv, int64_t k, const double* wd, const S* w,
+                 int64_t ldw, int64_t p, S* c, int64_t ldc, hipStream_t st) {
   constexpr int parts = sizeof(S) / sizeof(double);  // doubles per scalar
   const int kt = (int)((k + kGramCols - 1) / kGramCols);
-  // row groups: enough workgroups to fill the chip, each over many row tiles
-  int groups = grid_for(n, gram_rows<S>() * 4, std::max(32, 2048 / kt));
-  S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p * k));
   if (p <= 4)
     hipLaunchKernelGGL((k_gram_partial<S, 4>), dim3(groups, kt), dim3(256), 0, st, n, v, ldv, (int)k, wd, w, ldw,
                        (int)p, part);
@@ -507,6 +512,15 @@ void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, con
                      (int)(parts * k), (int)p, reinterpret_cast<const double*>(part), reinterpret_cast<double*>(c),
                      parts * ldc);
   check(hipGetLastError(), "gram");
+}
+
+template <class S>
+void gram_t(Scratch& scratch, int64_t n, const S* v, int64_t ldv, int64_t k, const double* wd, const S* w, int64_t ldw,
+            int64_t p, S* c, int64_t ldc, hipStream_t st) {
+  if (k <= 0 || p <= 0) return;
+  const int groups = gram_groups<S>(n, k);
+  S* part = static_cast<S*>(scratch.get(sizeof(S) * (size_t)groups * p * k));
+  gram_launch<S>(part, groups, n, v, ldv, k, wd, w, ldw, p, c, ldc, st);
 }
 
 template <class S>
@@ -632,6 +646,55 @@ void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* 
   if (n <= 0 || p <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * p, 256, 16384)), dim3(256), 0, st, n, a, x, ldx, b, y, ldy, (int)p);
   check(hipGetLastError(), "axpby");
+}
+
+// ------------------------------------------------------------------------------ no-allocation gram
+
+// Partial sums of gram_noalloc: a fixed device array instead of the per-stream arena, so the call
+// allocates nothing. Each device that runs the library holds its own copy until the process ends.
+// Host code never passes the array's name to a kernel (in host code it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per device, kept in
+// g_noalloc_ptr.
+namespace {
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+}  // namespace hsg
+
+static __device__ __align__(256) double scratch[hsg::kNoAllocScratchDoubles];
+
+namespace hsg {
+
+void* noalloc_scratch() {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("gram_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (gram_noalloc scratch)");
+    if (!ptr) throw std::runtime_error("gram_noalloc: no device address for the scratch array");
+    g_noalloc_ptr[dev] = ptr;
+  }
+  return g_noalloc_ptr[dev];
+}
+
+size_t gram_noalloc_bytes(int64_t n, int64_t k, int64_t p) {
+  if (k <= 0 || p <= 0) return 0;
+  return sizeof(double) * (size_t)gram_groups<double>(n, k) * (size_t)p * (size_t)k;
+}
+
+void gram_noalloc(int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w, int64_t ldw,
+                  int64_t p, double* c, int64_t ldc, hipStream_t st) {
+  if (k <= 0 || p <= 0) return;
+  const size_t need = gram_noalloc_bytes(n, k, p);
+  if (need > sizeof(double) * (size_t)kNoAllocScratchDoubles)
+    throw std::invalid_argument("gram_noalloc: k * p too large for the fixed scratch (" + std::to_string(need) +
+                                " bytes needed, " + std::to_string(sizeof(double) * kNoAllocScratchDoubles) +
+                                " available)");
+  double* part = static_cast<double*>(noalloc_scratch());
+  gram_launch<double>(part, gram_groups<double>(n, k), n, v, ldv, k, wd, w, ldw, p, c, ldc, st);
 }
 
 }  // namespace hsg

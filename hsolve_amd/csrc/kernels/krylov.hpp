@@ -36,6 +36,14 @@ void column_dots(Scratch& scratch, int64_t n, const double* x, int64_t ldx, cons
                  const double* wd, double* out, hipStream_t st);
 void pair_sums(Scratch& scratch, int64_t n, const double* ax, const double* bx, const double* x, int64_t ld, int64_t p,
                const double* lambda, double* out, hipStream_t st);
+// Gram without allocation: the partial sums go to a fixed device array of kNoAllocScratchDoubles
+// doubles (16 MiB) per device instead of the per-stream arena; throws std::invalid_argument when
+// gram_noalloc_bytes(n, k, p) exceeds it. Calls that overlap on one device share the array.
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 21;
+constexpr int kMaxDevices = 64;
+size_t gram_noalloc_bytes(int64_t n, int64_t k, int64_t p);
+void gram_noalloc(int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w, int64_t ldw,
+                  int64_t p, double* c, int64_t ldc, hipStream_t st);
 void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
            hipStream_t st);
 void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,

@@ -183,6 +183,17 @@ int hsg_shifted_residual(int64_t n, const int64_t* a_rowptr, const int32_t* a_co
 int hsg_gram(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w,
              int64_t ldw, int64_t p, double* c, int64_t ldc, void* stream);
 
+/* hsg_gram without allocation: the same kernels and the same result bit for bit, with their
+ * partial sums in a fixed 16 MiB device array that the library holds on each device it runs on
+ * (until the process ends) instead of a workspace it allocates. No counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Returns HSG_INVALID_INPUT when the partial sums of this
+ * shape would exceed the array (about p MiB for real V; any k when p <= 16 and k <= 4096). All
+ * calls on one device share the array: calls that overlap in time on one device (different
+ * streams or threads) race and must be ordered by the caller. */
+int hsg_gram_noalloc(int64_t n, const double* v, int64_t ldv, int64_t k, const double* w_diag, const double* w,
+                     int64_t ldw, int64_t p, double* c, int64_t ldc, void* stream);
+
 /* W = beta W + alpha V C for V: n x k, C: k x p (device, column-major), W: n x p. The update of
  * block CGS2 and the basis rotation of a thick restart (rotate_in_place,
  * src/eigen/block_lanczos.hpp:96-98). W must not overlap V. */
