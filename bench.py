@@ -163,6 +163,12 @@ def main():
                     help="box48 (default, the headline interval solve) or bloch16 (64 Bloch k-points of a "
                          "photonic cell, complex Hermitian, solved as one batch)")
     ap.add_argument("--workers", type=int, default=8, help="bloch16: concurrent pencils of the batch")
+    ap.add_argument("--extras", default="single,bloch16,serving",
+                    help="comma-separated extra measurements added to the box48 line as keys: single (the "
+                         "single-precision working-factor form of box48), bloch16 (the 64-k-point batch, one "
+                         "step), serving (a Poisson request stream), box80 (the million-order nearest solve); "
+                         "'' for none")
+    ap.add_argument("--serve-requests", type=int, default=40, help="serving: requests in the stream")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -284,10 +290,99 @@ def main():
                            for form, r in (("chosen", chosen), ("plain", plain)) if r is not None},
         "cpu_baseline": cpu,
     }
+    extras = [e for e in args.extras.split(",") if e]
+    if "single" in extras and not args.plain_only:
+        out["single_form"] = guarded_extra("single", lambda: single_form(prepared, base_cfg, check, args))
+    if "bloch16" in extras:
+        out["bloch16_batch"] = guarded_extra("bloch16", lambda: bloch_batch_extra(args))
+    if "serving" in extras:
+        out["serving"] = guarded_extra("serving", lambda: serving_extra(args))
+    if "box80" in extras:
+        out["box80"] = guarded_extra("box80", lambda: box80_extra(args))
     print(json.dumps(out), flush=True)
 
 
-def bench_bloch(args):
+def guarded_extra(name, fn):
+    """An extra measurement; a failure is reported in its key and does not lose the headline line."""
+    t0 = time.perf_counter()
+    try:
+        r = fn()
+    except Exception as e:  # noqa: BLE001 (reported in the JSON line)
+        log(f"extra {name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+    r["wall_s"] = time.perf_counter() - t0
+    log(f"extra {name} done in {r['wall_s']:.1f} s")
+    return r
+
+
+def single_form(prepared, base_cfg, check, args):
+    """box48 with single-precision working factors (FP32 on the matrix cores) refined to FP64 in
+    every operator application; counts stay FP64. One warm-up and up to two timed steps."""
+    from hsolve_amd import solver
+    cfg = dict(base_cfg, factor_precision="single")
+    r = run_form("single", prepared, solver_cfg(cfg), min(args.steps, 2), 1, check, None)
+    sm = summary(r)
+    rep = r["reps"][-1]
+    fs = rep["factor_stats"]
+    return {"value": sm["value"], "unit": UNIT, "ms_per_step": sm["ms_per_step"],
+            "certified_per_step": sm["certified_per_step"], "precision_used": rep["precision"]["used"],
+            "fallback": rep["precision"]["fallback"], "factorizations": fs["factorizations"],
+            "solves": fs["solves"], "rhs_solved": fs["rhs_solved"],
+            "solve_phases_s": {ph: rep[f"time_{ph}_s"] for ph in ("setup", "factor", "certificate", "iterate")}}
+
+
+def bloch_batch_extra(args):
+    """bloch16 chosen form (solve_batch), one warm-up and one timed step."""
+    import argparse as _ap
+    a = _ap.Namespace(**vars(args))
+    a.steps, a.warmup, a.no_plain, a.no_cpu_baseline = 1, 1, True, True
+    res = bench_bloch(a, emit=False)
+    keep = ("value", "unit", "ms_per_step", "certified_per_step", "config")
+    return {k: res[k] for k in keep if k in res}
+
+
+def serving_extra(args):
+    """A scaled-down serving stream (the million-request stream's mix at smaller sizes): PEC boxes of
+    5e3 - 1.5e5 unknowns at random sigma and k in 5..40, 20 % Bloch 16^3 cells over their window,
+    Poisson arrivals at 6 requests per second, 4 slots; every result certified (eta <= 1e-10
+    recomputed on the host, certificate passed)."""
+    import scipy.sparse.linalg as sla
+    from hsolve_amd import serve, workloads
+    reqs = workloads.serving_stream(args.serve_requests, seed=SEED, rate=6.0, n_max=1.5e5)
+    out, sm = serve.run_stream(reqs, slots=4, vectors="host")
+    certified = pairs = 0
+    for r, res in zip(reqs, out):
+        if not isinstance(res, serve.ServeResult):
+            continue
+        lam, X = res.eigenvalues, res.eigenvectors
+        pairs += lam.size
+        if not lam.size:
+            continue
+        A, B = r["A"], r["B"]
+        v0 = np.random.default_rng(SEED).uniform(-1, 1, A.shape[0])
+        na = float(np.max(np.abs(sla.eigsh(A, k=1, which="LM", tol=1e-6, v0=v0, return_eigenvectors=False))))
+        nb = 1.0 if B is None else float(np.max(np.abs(B.diagonal())))
+        BX = X if B is None else B @ X
+        eta = np.linalg.norm(A @ X - BX * lam[None, :], axis=0) / ((na + np.abs(lam) * nb) * np.linalg.norm(X, axis=0))
+        if res.report["certificate"]["passed"]:
+            certified += int(np.count_nonzero(eta <= TOL))
+    kinds = {}
+    for r in reqs:
+        kinds[r["kind"]] = kinds.get(r["kind"], 0) + 1
+    return {"requests": len(reqs), "kinds": kinds, "n_range": [min(r["n"] for r in reqs), max(r["n"] for r in reqs)],
+            "rate_per_s": 6.0, "slots": 4, "completed": sm["completed"], "failed": sm["failed"],
+            "latency_p50_ms": sm["latency_p50_ms"], "latency_p99_ms": sm["latency_p99_ms"],
+            "throughput_req_s": sm["throughput_req_s"], "pairs": pairs, "certified_pairs": certified,
+            "certified_pairs_per_s": certified / sm["makespan_s"], "makespan_s": sm["makespan_s"],
+            "analyses": sm["analyses"], "analysis_hits": sm["analysis_hits"], "peak_running": sm["peak_running"],
+            "peak_predicted_gb": sm["peak_predicted_bytes"] / 2 ** 30, "budget_gb": sm["budget_gb"]}
+
+
+def box80_extra(args):
+    raise NotImplementedError("box80 is measured once the contribution blocks are allocated level by level")
+
+
+def bench_bloch(args, emit=True):
     """bloch16: the 64 k-points of a 4 x 4 x 4 grid over [0, pi/a]^3 of a periodic 16^3 Yee cell with
     eps = 12 inside a centred sphere of radius 0.3 a (n = 12,288 complex unknowns per k-point), all
     eigenvalues in [0.01, 1.0] (2 pi / a)^2 per k-point. One step solves the whole batch: chosen =
@@ -407,7 +502,9 @@ def bench_bloch(args):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def solver_cfg(cfg):

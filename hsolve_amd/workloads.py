@@ -8,6 +8,10 @@
              radius 0.3 a, Bloch phases of a 4 x 4 x 4 grid over [0, pi/a]^3 (endpoints included),
              window [0.01, 1.0] (2 pi / a)^2 per k-point
   box80      PEC box on 80 x 70 x 64 cells (n = 1,045,014), jittered mass, sigma mid-spectrum, k = 20
+  serving    a seeded request stream with Poisson arrivals: PEC boxes of aspect ratios U(1, 2) with
+             n log-uniform over [n_min, n_max] (12..60 cells per side), B = I or jittered diagonal
+             mass, k uniform in {5, ..., 40}, sigma uniform inside the closed-form spectrum; a
+             fraction of Bloch requests (the bloch16 cell at a random k-point of its grid, its window)
 The inputs are generated by hsolve_amd.gen.yee (seed 2026 unless stated).
 """
 from __future__ import annotations
@@ -103,3 +107,53 @@ def box80(seed: int = 2026, k: int = 20) -> Workload:
     sigma = 0.5 * (12.0 / hmin ** 2) * 0.5
     return Workload("box80", A, B, {"mode": "nearest", "sigma": sigma, "k": k},
                     {"n": A.shape[0], "cells": [80, 70, 64], "seed": seed})
+
+
+def _box_spectrum_range(cells, lengths) -> tuple[float, float]:
+    """Smallest nonzero and largest eigenvalue of the homogeneous PEC box pencil: per dimension the
+    1-D terms (2 / h sin(m pi / 2N))^2, m = 0..N-1; the smallest mode has one index 0 and two 1's
+    (a mode needs two nonzero indices), the largest all indices N - 1."""
+    t1 = sorted((2.0 * c / L * math.sin(math.pi / (2 * c))) ** 2 for c, L in zip(cells, lengths))
+    top = sum((2.0 * c / L * math.sin((c - 1) * math.pi / (2 * c))) ** 2 for c, L in zip(cells, lengths))
+    return t1[0] + t1[1], top
+
+
+def serving_stream(count: int = 2000, seed: int = 2026, rate: float = 4.0, n_min: float = 5e3, n_max: float = 6e5,
+                   bloch_fraction: float = 0.2, bloch_cells: int = 16) -> list[dict]:
+    """Requests of the serving workload, sorted by arrival: dicts with A, B, config (target,
+    tol 1e-10), arrival (seconds; exponential gaps of mean 1 / rate), kind and n. Box cells per side
+    are the lengths (pi times U(1, 2) each) over one spacing chosen so that 3 Nx Ny Nz is the drawn
+    n, clipped to 12..60 cells per side. Matrices are built here, so keep count modest (a 6e5 box
+    takes about a second to assemble)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    kpts = bloch_kpoints(4)
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.uniform() < bloch_fraction:
+            kp = kpts[int(rng.integers(len(kpts)))]
+            w = bloch16(kp) if bloch_cells == 16 else None
+            if w is None:
+                A, B, _ = yee.bloch_cell(bloch_cells, kp)
+                scale = (2 * math.pi) ** 2
+                w = Workload("bloch", A, B, {"mode": "interval", "lower": 0.01 * scale, "upper": scale}, {})
+            out.append({"A": w.A, "B": w.B, "config": {"target": dict(w.target), "tol": 1e-10}, "arrival": t,
+                        "kind": "bloch", "n": int(w.A.shape[0]), "kpoint": list(kp)})
+            continue
+        aspect = rng.uniform(1.0, 2.0, 3)
+        lengths = tuple(float(math.pi * a) for a in aspect)
+        n_want = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        h = (float(np.prod(lengths)) / (n_want / 3.0)) ** (1.0 / 3.0)
+        cells = tuple(int(min(60, max(12, round(L / h)))) for L in lengths)
+        jitter = bool(rng.uniform() < 0.5)
+        A, B, g = yee.jittered_box(cells, lengths, seed=int(rng.integers(1 << 31)), jitter=0.05 if jitter else 0.0)
+        if not jitter:
+            A, B = yee.standard_form(A, B), None
+        lo, hi = _box_spectrum_range(cells, lengths)
+        sigma = float(rng.uniform(lo, hi))
+        k = int(rng.integers(5, 41))
+        out.append({"A": A, "B": B, "config": {"target": {"mode": "nearest", "sigma": sigma, "k": k}, "tol": 1e-10},
+                    "arrival": t, "kind": "box-ghep" if jitter else "box-hep", "n": int(A.shape[0]),
+                    "cells": list(cells)})
+    return out
