@@ -22,7 +22,8 @@ class AnalysisInfo(C.Structure):
                 ("max_front", C.c_int64), ("max_pivots", C.c_int64), ("factor_entries", C.c_double),
                 ("flops", C.c_double), ("panel_bytes_double", C.c_double), ("cb_bytes_double", C.c_double),
                 ("analyze_s", C.c_double), ("ordering_s", C.c_double), ("small_fronts", C.c_int64),
-                ("large_fronts", C.c_int64), ("front_rows", C.c_int64)]
+                ("large_fronts", C.c_int64), ("front_rows", C.c_int64), ("update_flops", C.c_double),
+                ("cb_bytes_unshared", C.c_double), ("workspace_bytes_double", C.c_double)]
 
 
 class Inertia(C.Structure):

@@ -120,13 +120,16 @@ int hsg_analyze(int64_t n, const int64_t* rowptr, const int32_t* colind, const i
     in.factor_entries = sym.factor_entries;
     in.flops = sym.flops;
     in.panel_bytes_double = 8.0 * static_cast<double>(sym.panel_off[sym.nsn]);
-    in.cb_bytes_double = 8.0 * static_cast<double>(sym.cb_off[sym.nsn]);
+    in.cb_bytes_double = 8.0 * static_cast<double>(sym.cb_total);
     in.analyze_s = std::chrono::duration<double>(t1 - t0).count();
     in.ordering_s = 0.0;
     f->engine = std::make_shared<hsg::Engine>(std::move(sym), std::move(dst), std::move(ia), std::move(ib), eo);
     in.front_rows = static_cast<int64_t>(f->engine->symbolic().rows.size());
     in.small_fronts = f->engine->small_fronts();
     in.large_fronts = f->engine->large_fronts();
+    in.update_flops = f->engine->update_flops();
+    in.cb_bytes_unshared = 8.0 * static_cast<double>(f->engine->symbolic().cb_sum);
+    in.workspace_bytes_double = 8.0 * static_cast<double>(f->engine->workspace_entries());
     *out = f.release();
   });
 }
@@ -360,7 +363,10 @@ int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, doubl
 }
 
 int hsg_timing_enable(int on) {
-  return guarded([&] { hsg::timing_enable(on != 0); });
+  return guarded([&] {
+    if (on < 0 || on > 2) throw InvalidInput("timing level must be 0, 1 or 2");
+    hsg::timing_enable(on);
+  });
 }
 
 int hsg_timing_read(int phase, double* ms, int64_t* launches, double* work, int64_t* calls) {

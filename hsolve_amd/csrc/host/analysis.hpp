@@ -54,7 +54,12 @@ struct Symbolic {
   std::vector<int32_t> relmap;      // parallel to rows: for CB rows, the local index in the parent front
   std::vector<int32_t> child_ptr, child_list;  // children of each supernode (ascending)
   std::vector<int64_t> panel_off;   // m x ncol panel of L per supernode (size nsn + 1)
-  std::vector<int64_t> cb_off;      // cb x cb contribution block per supernode (size nsn + 1)
+  // cb x cb contribution block per supernode: offsets into one array of cb_total entries shared by
+  // blocks whose lifetimes (own level to parent level) do not overlap (plan_contribution_blocks);
+  // cb_off[nsn] = cb_total
+  std::vector<int64_t> cb_off;
+  int64_t cb_total = 0;             // entries of the contribution-block array (the plan's peak)
+  int64_t cb_sum = 0;               // sum of all blocks' entries (what one block per front would take)
   std::vector<int32_t> level;       // height in the assembly tree (leaves 0)
   int32_t nlevels = 0;
   std::vector<int32_t> level_ptr, level_sn;  // supernodes grouped by level, ascending
@@ -69,6 +74,10 @@ struct Symbolic {
 Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind,
                          const OrderingOptions& oopt, const AmalgamationOptions& aopt,
                          const int32_t* given_order /* may be null: new -> old */);
+
+/// Offsets of the contribution blocks (cb_off, cb_total) from the level schedule: blocks live from
+/// their front's level to their parent's level and share storage otherwise.
+void plan_contribution_blocks(Symbolic& s);
 
 /// Assembly destinations of the original entries: for every entry e of the CSR pattern, the panel
 /// index (sn's panel_off + local_col * m + local_row) of K(i, j) when, in the new numbering, row

@@ -9,6 +9,7 @@
 //   5. front row structures bottom-up (own columns, children's contribution rows, original
 //      entries), extend-add maps, storage offsets and the level schedule (height in the tree).
 #include <algorithm>
+#include <map>
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
@@ -374,18 +375,87 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
   }
   // offsets, statistics
   S.panel_off.assign(static_cast<size_t>(S.nsn) + 1, 0);
-  S.cb_off.assign(static_cast<size_t>(S.nsn) + 1, 0);
   for (int32_t s = 0; s < S.nsn; ++s) {
     const int64_t m = S.sn_m[s], nc = S.sn_ncol[s], cb = m - nc;
     S.panel_off[s + 1] = S.panel_off[s] + m * nc;
-    S.cb_off[s + 1] = S.cb_off[s] + cb * cb;
+    S.cb_sum += cb * cb;
     S.factor_entries += static_cast<double>(m * nc) - 0.5 * static_cast<double>(nc * (nc - 1));
     const double pp = static_cast<double>(nc), c = static_cast<double>(cb);
     S.flops += pp * pp * pp / 3.0 + pp * pp * c + pp * c * c;
     S.max_front = std::max<int32_t>(S.max_front, static_cast<int32_t>(m));
     S.max_ncol = std::max<int32_t>(S.max_ncol, static_cast<int32_t>(nc));
   }
+  plan_contribution_blocks(S);
   return S;
+}
+
+void plan_contribution_blocks(Symbolic& S) {
+  // A front's contribution block is written while its level is factorized and read while its
+  // parent's level is (levels run one after the other): it is live over the levels
+  // [level(s), level(parent)]. Offsets are assigned level by level with a first-fit allocator
+  // over one address range: at the start of level l the blocks whose parent level was l - 1 are
+  // released (coalescing free neighbours), then the level's blocks are placed, largest first, in
+  // the lowest free range that holds them. Blocks whose lifetimes overlap never share storage.
+  const int32_t nsn = S.nsn;
+  S.cb_off.assign(static_cast<size_t>(nsn) + 1, 0);
+  std::vector<std::vector<int32_t>> ending(static_cast<size_t>(std::max(1, S.nlevels)));
+  std::map<int64_t, int64_t> free_ranges;  // offset -> length
+  int64_t top = 0;
+  auto release = [&](int64_t off, int64_t len) {
+    if (len == 0) return;
+    auto it = free_ranges.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != free_ranges.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_ranges.erase(nx);
+    }
+    if (it != free_ranges.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_ranges.erase(it);
+        it = pv;
+      }
+    }
+    if (it->first + it->second == top) {  // the range ends at the top: give it back
+      top = it->first;
+      free_ranges.erase(it);
+    }
+  };
+  auto place = [&](int64_t len) -> int64_t {
+    for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it)
+      if (it->second >= len) {
+        const int64_t off = it->first;
+        const int64_t rest = it->second - len;
+        free_ranges.erase(it);
+        if (rest > 0) free_ranges.emplace(off + len, rest);
+        return off;
+      }
+    const int64_t off = top;
+    top += len;
+    return off;
+  };
+  int64_t peak = 0;
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    if (l > 0)
+      for (int32_t s : ending[l - 1]) {
+        const int64_t cb = S.sn_m[s] - S.sn_ncol[s];
+        release(S.cb_off[s], cb * cb);
+      }
+    std::vector<int32_t> here(S.level_sn.begin() + S.level_ptr[l], S.level_sn.begin() + S.level_ptr[l + 1]);
+    std::stable_sort(here.begin(), here.end(), [&](int32_t a, int32_t b) {
+      return S.sn_m[a] - S.sn_ncol[a] > S.sn_m[b] - S.sn_ncol[b];
+    });
+    for (int32_t s : here) {
+      const int64_t cb = S.sn_m[s] - S.sn_ncol[s];
+      S.cb_off[s] = cb > 0 ? place(cb * cb) : 0;
+      const int32_t end = S.sn_parent[s] >= 0 ? S.level[S.sn_parent[s]] : l;
+      ending[static_cast<size_t>(end)].push_back(s);
+    }
+    peak = std::max(peak, top);
+  }
+  S.cb_off[nsn] = peak;
+  S.cb_total = peak;
 }
 
 std::vector<int64_t> assembly_map(const Symbolic& S, int32_t n, const int64_t* rowptr, const int32_t* colind) {

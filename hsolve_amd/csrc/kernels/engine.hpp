@@ -62,6 +62,8 @@ class Engine {
 
   const Symbolic& symbolic() const { return sym_; }
   int64_t small_fronts() const { return n_small_; }
+  double update_flops() const { return update_flops_; }  // one factorization's front-update products
+  int64_t workspace_entries() const { return w_total_; }
   int64_t large_fronts() const { return n_large_; }
 
   // Numeric factorization into nm; returns the inertia counts (synchronizes the stream).
@@ -89,6 +91,8 @@ class Engine {
     int fwd_tiles = 0, fwd_chunks = 0;   // forward split-GEMV grid
     int64_t bt_begin = 0, bt_count = 0;  // backward tasks of the level
     int64_t bh_begin = 0, bh_count = 0;  // its split column tiles (their chunk-0 tasks)
+    std::vector<double> upd_flops;       // algorithmic flops of the trailing update after panel p
+    double cb_flops = 0.0;               // and of the contribution-block product
   };
   struct TaskRange {
     int64_t begin = 0, count = 0;
@@ -128,6 +132,7 @@ class Engine {
   int64_t n_small_ = 0, n_large_ = 0;
   int64_t panel_total_ = 0, cb_total_ = 0, w_total_ = 0, rows_total_ = 0;
   int max_large_per_level_ = 0;
+  double update_flops_ = 0.0;
   // device symbolic arrays
   int32_t *d_first_ = nullptr, *d_ncol_ = nullptr, *d_m_ = nullptr, *d_child_ptr_ = nullptr,
           *d_child_list_ = nullptr, *d_relmap_ = nullptr, *d_rows_orig_ = nullptr;

@@ -1380,16 +1380,23 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
   if (opt_.small_front_max < 1 || opt_.small_front_max > 128) throw std::runtime_error("small_front_max must be in [1, 128]");
   nnz_ = (int64_t)h_asm_.size();
   const int nsn = sym_.nsn;
-  // W workspace offsets for large fronts
+  // W = L D workspace of the large fronts: used only while the front's level is factorized, so
+  // the levels share one array (offsets restart at 0 on every level; its size is the largest
+  // level's sum)
   h_w_off_.assign(nsn, -1);
-  for (int s = 0; s < nsn; ++s) {
-    if (sym_.sn_m[s] > opt_.small_front_max) {
-      h_w_off_[s] = w_total_;
-      w_total_ += (int64_t)sym_.sn_m[s] * sym_.sn_ncol[s];
+  for (int l = 0; l < sym_.nlevels; ++l) {
+    int64_t off = 0;
+    for (int q = sym_.level_ptr[l]; q < sym_.level_ptr[l + 1]; ++q) {
+      const int s = sym_.level_sn[q];
+      if (sym_.sn_m[s] > opt_.small_front_max) {
+        h_w_off_[s] = off;
+        off += (int64_t)sym_.sn_m[s] * sym_.sn_ncol[s];
+      }
     }
+    w_total_ = std::max(w_total_, off);
   }
   panel_total_ = sym_.panel_off[nsn];
-  cb_total_ = sym_.cb_off[nsn];
+  cb_total_ = sym_.cb_total;
   rows_total_ = (int64_t)sym_.rows.size();
   levels_.resize(sym_.nlevels);
   for (int l = 0; l < sym_.nlevels; ++l) {
@@ -1411,6 +1418,25 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
       L.max_m = std::max(L.max_m, sym_.sn_m[s2]);
       L.max_ncol = std::max(L.max_ncol, sym_.sn_ncol[s2]);
     }
+    // algorithmic flops of the front-update products (lower-triangle outputs only): trailing
+    // update after panel p: 2 kn sum_{c = c1}^{ncol - 1} (m - c); contribution block:
+    // 2 ncol cb (cb + 1) / 2
+    const int nbw = opt_.panel_width;
+    L.upd_flops.assign((L.large_max_ncol + nbw - 1) / nbw, 0.0);
+    for (int s2 : L.h_large) {
+      const double m = sym_.sn_m[s2], nc = sym_.sn_ncol[s2];
+      for (int pp = 0; pp < (int)L.upd_flops.size(); ++pp) {
+        const double c0 = (double)pp * nbw, c1 = std::min(nc, c0 + nbw);
+        if (c0 >= nc || c1 >= nc) continue;
+        const double cols = nc - c1;
+        const double rows_sum = cols * m - 0.5 * (c1 + nc - 1) * cols;  // sum_{c=c1}^{nc-1} (m - c)
+        L.upd_flops[pp] += 2.0 * (c1 - c0) * rows_sum;
+      }
+      const double cbn = m - nc;
+      L.cb_flops += nc * cbn * (cbn + 1.0);
+    }
+    for (double f : L.upd_flops) update_flops_ += f;
+    update_flops_ += L.cb_flops;
     L.n_all = (int)L.h_all.size();
     L.n_small = (int)L.h_small.size();
     L.n_large = (int)L.h_large.size();
@@ -1769,13 +1795,17 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
           const int tc = (L.large_max_ncol - (p + 1) * nb + kTile - 1) / kTile;
+          const int64_t ut = timing_begin(kPhaseUpdate, st);
           HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * tc, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 0, p,
                              nb);
+          timing_end(ut, st, 1, L.upd_flops[p]);
         }
       }
       if (L.large_max_cb > 0) {
         const int t = (L.large_max_cb + kTile - 1) / kTile;
+        const int64_t ut = timing_begin(kPhaseUpdate, st);
         HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * t, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0, nb);
+        timing_end(ut, st, 1, L.cb_flops);
         HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_cb + kEaCols - 1) / kEaCols, L.n_large),
                            dim3(kThreads), 0, st, S, N, L.d_large, 1);
       }

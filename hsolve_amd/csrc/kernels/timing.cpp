@@ -18,7 +18,7 @@ struct Record {
 };
 
 std::mutex g_mu;
-bool g_on = false;
+int g_level = 0;
 std::vector<Record> g_records;
 
 void check(hipError_t e, const char* what) {
@@ -35,20 +35,20 @@ void clear_locked() {
 
 }  // namespace
 
-void timing_enable(bool on) {
+void timing_enable(int level) {
   std::lock_guard<std::mutex> lk(g_mu);
   clear_locked();
-  g_on = on;
+  g_level = level;
 }
 
-bool timing_enabled() {
+int timing_level() {
   std::lock_guard<std::mutex> lk(g_mu);
-  return g_on;
+  return g_level;
 }
 
 int64_t timing_begin(int phase, hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_on) return -1;
+  if (g_level == 0 || (phase == kPhaseUpdate && g_level < 2)) return -1;
   Record r;
   r.phase = phase;
   check(hipEventCreate(&r.start), "hipEventCreate (timing)");

@@ -15,7 +15,9 @@ enum TimedPhase : int {
   kPhaseInverse = 1,   // inverse panels G, Gt from L (first solve after a factorization): flops
   kPhaseForward = 2,   // forward sweep of a solve (products with their gathers, reductions): bytes
   kPhaseBackward = 3,  // backward sweep of a solve (transposed products): bytes
-  kPhaseCount = 4
+  kPhaseUpdate = 4,    // the front-update products (k_gemm_update launches) of factorizations: flops
+                       // of their lower-triangle outputs; recorded per launch at detail level 2 only
+  kPhaseCount = 5
 };
 
 struct PhaseTotals {
@@ -25,8 +27,9 @@ struct PhaseTotals {
   int64_t calls = 0;
 };
 
-void timing_enable(bool on);  // clears the records either way
-bool timing_enabled();
+// level 0: off, 1: phases, 2: phases and every front-update launch; clears the records either way
+void timing_enable(int level);
+int timing_level();
 // Records the start event of a phase on `st`; returns a token for timing_end (-1: timing off).
 int64_t timing_begin(int phase, hipStream_t st);
 void timing_end(int64_t token, hipStream_t st, int64_t launches, double work);

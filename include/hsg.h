@@ -71,12 +71,19 @@ typedef struct hsg_analysis_info {
   double factor_entries;     /* lower-trapezoid entries of L incl. D (INFOG(20)) */
   double flops;              /* sum over fronts of p^3/3 + p^2 c + p c^2 (RINFOG(1) analogue) */
   double panel_bytes_double; /* device bytes of the FP64 factor panels */
-  double cb_bytes_double;    /* device bytes of the FP64 contribution blocks */
+  double cb_bytes_double;    /* device bytes of the FP64 contribution-block array: blocks whose
+                                lifetimes (own level to parent level) do not overlap share storage */
   double analyze_s;          /* host wall time of ordering + symbolic analysis */
   double ordering_s;
   int64_t small_fronts;      /* fronts handled by the one-workgroup LDS kernel */
   int64_t large_fronts;
   int64_t front_rows;        /* sum over fronts of their order (length of the row lists) */
+  double update_flops;       /* flops of the front-update products of one factorization (the
+                                matrix-core GEMMs: trailing updates and contribution blocks, lower
+                                triangles only); the rest of `flops` is panel factorization work
+                                and the fronts factorized whole in LDS */
+  double cb_bytes_unshared;  /* what one block per front would take (the sum of all blocks), FP64 */
+  double workspace_bytes_double; /* the L D workspace of the large fronts (shared by the levels), FP64 */
 } hsg_analysis_info;
 
 /* Inertia of the last factorization (Inertia, factor.hpp:295-300). valid == 0 after a singular
@@ -248,9 +255,12 @@ int hsg_zcolumn_dots(int64_t n, const double* x, int64_t ldx, const double* y, i
  * on each call's stream: phase 0 numeric factorization (work = flops of the symbolic analysis),
  * 1 inverse panels built on the first solve after a factorization (flops), 2 forward sweep and
  * 3 backward sweep of a solve (bytes: every stored factor entry once plus the right-hand sides
- * read and written once). hsg_timing_enable clears the records (on = 0 also stops recording);
+ * read and written once), 4 the front-update GEMM launches of factorizations, one record per
+ * launch (flops of its lower-triangle outputs; recorded only at level 2).
+ * hsg_timing_enable(level) clears the records; level 0 stops recording, 1 records phases 0-3,
+ * 2 also phase 4;
  * hsg_timing_read synchronizes the recorded events and returns the sums over all calls. */
-int hsg_timing_enable(int on);
+int hsg_timing_enable(int level);
 int hsg_timing_read(int phase, double* ms, int64_t* launches, double* work, int64_t* calls);
 
 #ifdef __cplusplus
