@@ -28,6 +28,7 @@ import scipy.sparse as sp
 import torch
 
 from . import config as cfgmod
+from . import subspace
 from .driver_core import Ctx, PhaseClock, estimate_norm2
 from .errors import InvalidInputError, NotHermitianError, UnsupportedError
 from .factor import GpuFactorization, Pencil
@@ -251,6 +252,8 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
                              "rel_residual": m.rel_residual, "backward_error": m.backward_error, "accepted": acc,
                              "subspace_sin_bound": None})
     rep["success"] = bool(all_ok)
+    with clock.phase("verify"):
+        subspace.attach(ctx, rep, sel, cfg)
     fs = fac.stats
     rep["factor_options"]["precision"] = ctx.precision
     rep["factor_options"]["null_pivot_tol"] = fac.NULL_PIVOT_TOL[0 if ctx.precision == "double" else 1]
