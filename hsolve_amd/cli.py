@@ -28,7 +28,12 @@ from .errors import HsolveError, InvalidInputError
 def _parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser(prog="hsolve", description=__doc__.split("\n\n")[0],
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--A", dest="a", help="matrix A (.npz scipy sparse, .npy dense, .mtx); required except with --version")
+    ap.add_argument("--A", dest="a", help="matrix A (.npz scipy sparse, .npy dense, .mtx); required except with --version "
+                                          "or --case")
+    ap.add_argument("--case", help="a case of the reference's data layout (a directory with A.npz, optional B.npz, "
+                                   "config.json; or a case name under ROOT/cases): its matrices and configuration "
+                                   "(command-line overrides apply on top)")
+    ap.add_argument("--root", help="data root of --case names (default: $HS_DATA_ROOT or ./data)")
     ap.add_argument("--B", dest="b", help="matrix B (generalized problem)")
     ap.add_argument("--config", help="config.json")
     ap.add_argument("--out", default="hsolve-out", help="output directory (default: %(default)s)")
@@ -124,6 +129,17 @@ def main(argv=None) -> int:
     if args.version:
         print(f"hsolve_amd {cfgmod.VERSION} (MI355X engine)")
         return 0
+    if args.case:
+        from .case import _case_dir
+        d = _case_dir(args.case, args.root)
+        if not (d / "A.npz").exists():
+            print(f"hsolve: error [io]: no case at {d}", file=sys.stderr)
+            return 2
+        args.a = str(d / "A.npz")
+        if args.b is None and (d / "B.npz").exists():
+            args.b = str(d / "B.npz")
+        if args.config is None and (d / "config.json").exists():
+            args.config = str(d / "config.json")
     if not args.a:
         print("hsolve: error [invalid_input]: --A is required", file=sys.stderr)
         return 2

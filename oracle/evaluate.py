@@ -141,3 +141,71 @@ def evaluate(A, B, lam, X, target: dict, ref: np.ndarray | None, tol: float = 1e
     out["passed"] = (out["all_accepted"] and not out["missing"] and out["deficit"] == 0 and not out["spurious"]
                      and out["orthogonality"] <= 1e-6)
     return out
+
+
+def evaluate_case(case, result_dir, root=None, tol: float | None = None) -> dict:
+    """Judges a result directory against a case (and its reference when one exists) in the
+    reference's data layout (tools/eval/evaluate.py:805-840): the case's matrices and target, the
+    reference eigenvalues (ref.json 'complete' decides whether unmatched values are spurious), the
+    result's eigenvalues and eigenvectors. Returns the verdict document written by main()."""
+    from hsolve_amd import case as cio  # the layout reader only; no solver code
+    c = cio.load_case(case, root)
+    ref = cio.load_ref(c.path if (c.path / "eigenvalues.npy").exists() else c.name, root)
+    res = cio.load_result(result_dir)
+    target = dict(c.config.get("target", {}))
+    rep_target = res.report.get("config", {}).get("target") if isinstance(res.report.get("config"), dict) else None
+    if not target and rep_target:
+        target = rep_target
+    tol = tol if tol is not None else float(c.config.get("tol", 1e-10))
+    lam = np.asarray(res.eigenvalues, dtype=float)
+    X = np.asarray(res.eigenvectors)
+    rv = None if ref is None else np.real(np.asarray(ref.eigenvalues))
+    out = evaluate(c.A, c.B, lam, X, target, rv, tol=tol,
+                   ref_complete=bool(ref.info.get("complete", False)) if ref is not None else False)
+    reasons = []
+    if not out["all_accepted"]:
+        reasons.append(f"backward error above tol (max {out['max_eta']:.3g} > {tol:g})")
+    if out["missing"]:
+        reasons.append(f"{len(out['missing'])} required eigenvalue(s) missing")
+    if out["deficit"]:
+        reasons.append(f"{out['deficit']} tied eigenvalue(s) at the target boundary missing")
+    if out["spurious"]:
+        reasons.append(f"{len(out['spurious'])} eigenvalue(s) not in the complete reference")
+    if out["orthogonality"] > 1e-6:
+        reasons.append(f"vectors not B-orthonormal ({out['orthogonality']:.3g})")
+    return {"case": c.name, "result": str(result_dir), "n": c.n, "target": target, "tol": tol,
+            "reference": None if ref is None else {"kind": ref.info.get("kind"),
+                                                   "complete": ref.info.get("complete"),
+                                                   "count": int(np.size(ref.eigenvalues))},
+            "summary": {"n_pairs": int(lam.size), "max_backward_error": out["max_eta"],
+                        "orthogonality": out["orthogonality"]},
+            "missing": out["missing"], "spurious": out["spurious"], "deficit": out["deficit"],
+            "verdict": "pass" if out["passed"] else "fail", "reasons": reasons}
+
+
+def main(argv=None) -> int:
+    """python -m oracle.evaluate CASE RESULT [--root R] [--out FILE] [--tol T]: writes the verdict
+    JSON (default RESULT/evaluation.json) and prints a one-line summary; exit 0 when it passes."""
+    import argparse
+    import json
+    from pathlib import Path
+    ap = argparse.ArgumentParser(description="Evaluate a solver result directory against a case.")
+    ap.add_argument("case", help="case directory, or a case name under ROOT/cases")
+    ap.add_argument("result", type=Path, help="result directory")
+    ap.add_argument("--root", help="data root (default: $HS_DATA_ROOT or ./data)")
+    ap.add_argument("--out", type=Path, help="output JSON (default: RESULT/evaluation.json)")
+    ap.add_argument("--tol", type=float, help="acceptance threshold on eta (default: the case's config tol)")
+    args = ap.parse_args(argv)
+    doc = evaluate_case(args.case, args.result, args.root, args.tol)
+    out = args.out or args.result / "evaluation.json"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    out.write_text(json.dumps(doc, indent=2) + "\n")
+    s = doc["summary"]
+    print(f"{doc['case']}: {doc['verdict']}  pairs={s['n_pairs']} max_eta={s['max_backward_error']:.3g} -> {out}")
+    for r in doc["reasons"]:
+        print(f"  - {r}")
+    return 0 if doc["verdict"] == "pass" else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -64,3 +64,19 @@ def test_cli_solves_file_input(gpu, tmp_path):
     assert not (out / "complex_eigenvalues.npy").exists()
     res = ev.evaluate(A, None, lam, X, {"mode": "nearest", "sigma": 9.0, "k": 10}, ref)
     assert res["passed"], res
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["pec-cavity-multiplicity", "interval-slicing"])
+def test_cli_solves_a_case_directory_and_the_evaluator_passes_it(gpu, tmp_path, name):
+    # the reference's data layout end to end: a shipped case (matrices, configuration, reference
+    # spectrum) solved through the CLI's --case, the result directory judged by the evaluator's
+    # command line
+    from oracle import evaluate as ev
+    case_dir = Path(__file__).parent / "golden" / "examples" / name
+    out = tmp_path / "result"
+    assert cli.main(["--case", str(case_dir), "--out", str(out), "-q"]) == 0
+    assert ev.main([str(case_dir), str(out)]) == 0
+    doc = json.loads((out / "evaluation.json").read_text())
+    print(doc["case"], doc["verdict"], doc["summary"], flush=True)
+    assert doc["verdict"] == "pass" and doc["reference"]["complete"]
