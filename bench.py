@@ -163,8 +163,9 @@ def main():
                     help="box48 (default, the headline interval solve) or bloch16 (64 Bloch k-points of a "
                          "photonic cell, complex Hermitian, solved as one batch)")
     ap.add_argument("--workers", type=int, default=8, help="bloch16: concurrent pencils of the batch")
-    ap.add_argument("--extras", default="single,bloch16,serving,box80",
-                    help="comma-separated extra measurements added to the box48 line as keys: single (the "
+    ap.add_argument("--extras", default="autotune,single,bloch16,serving,box80",
+                    help="comma-separated extra measurements added to the box48 line as keys: autotune (the "
+                         "launch-plan autotuner, run before the timed steps; the chosen form uses its plan), single (the "
                          "single-precision working-factor form of box48), bloch16 (the 64-k-point batch, one "
                          "step), serving (a Poisson request stream), box80 (the million-order nearest solve); "
                          "'' for none")
@@ -221,10 +222,16 @@ def main():
     def check(lam, X):
         return backward_errors(A_dev, b_dev, lam, X, na, nb)
 
-    # chosen: the engine's own choices (working precision, concurrent slice streams);
-    # plain: FP64 factors, one slice after the other
+    # chosen: the engine's own choices (working precision, concurrent slice streams, the launch
+    # plan its autotuner measured for this pattern); plain: FP64 factors, one slice after the
+    # other, the default launch plan
+    extras = [e for e in args.extras.split(",") if e]
+    tuning = None
+    if "autotune" in extras and not args.plain_only:
+        tuning = guarded_extra("autotune", lambda: autotune_extra(prepared, 0.5 * (LOWER + upper)))
     if args.plain_only:
         chosen_cfg, args.no_plain = plain_cfg, False
+    os.environ["HSG_AUTOTUNE"] = "0" if args.plain_only else "1"
     chosen = run_form("plain" if args.plain_only else "chosen", prepared, solver_cfg(chosen_cfg), args.steps,
                       args.warmup, check, 1 if args.plain_only else None)
     cs = summary(chosen)
@@ -232,7 +239,9 @@ def main():
     same_form = rep["precision"]["used"] == "double"
     plain, plain_s = None, None
     if not args.no_plain and not args.plain_only:
+        os.environ["HSG_AUTOTUNE"] = "0"
         plain = run_form("plain", prepared, solver_cfg(plain_cfg), args.steps, args.warmup, check, 1)
+        os.environ["HSG_AUTOTUNE"] = "1"
         plain_s = summary(plain)
 
     # roofline: the backward sweep's kernels (the dominant ones of this workload), timed in the
@@ -279,7 +288,8 @@ def main():
                            for form, r in (("chosen", chosen), ("plain", plain)) if r is not None},
         "cpu_baseline": cpu,
     }
-    extras = [e for e in args.extras.split(",") if e]
+    if tuning is not None:
+        out["autotune"] = tuning
     if "single" in extras and not args.plain_only:
         out["single_form"] = guarded_extra("single", lambda: single_form(prepared, base_cfg, check, args))
     if "bloch16" in extras:
@@ -368,6 +378,20 @@ def factor_roofline(prepared, sigma, reps=3):
                      "factorization_flops": fa["work"] / reps}
     F.close()
     return out
+
+
+def autotune_extra(prepared, sigma):
+    """The engine's autotuner on the box48 pattern (untimed, once per pattern, like a warm-up): the
+    measured launch plans and the one the chosen form then uses (cache in a temporary file)."""
+    import tempfile
+    from hsolve_amd import autotune
+    os.environ.setdefault("HSG_TUNE_CACHE", os.path.join(tempfile.mkdtemp(prefix="hsg-tune-"), "tuning.json"))
+    t0 = time.perf_counter()
+    r = autotune.tune(prepared.pencil, sigma=sigma)
+    return {"chosen": r["options"], "cost_s": r["cost"], "default_cost_s": r["default_cost"],
+            "tuning_s": time.perf_counter() - t0,
+            "candidates": [{"options": c["options"], "factor_ms": 1e3 * c["factor_s"], "solve8_ms": 1e3 * c["solve_s"],
+                            "analysis_s": c["analysis_s"], "flops": c["flops"]} for c in r["table"]]}
 
 
 def single_form(prepared, base_cfg, check, args):

@@ -19,6 +19,7 @@ import threading
 
 import torch
 
+from . import autotune
 from . import config as cfgmod
 from . import streams as stream_pool
 from .factor import GpuFactorization, same_pattern
@@ -47,7 +48,7 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
     for p in prepared:
         head = next((f for f in heads if same_pattern(f.pencil, p.pencil)), None) if share_analysis else None
         if head is None:
-            head = GpuFactorization(p.pencil)
+            head = GpuFactorization(p.pencil, **autotune.options_for(p.pencil))
             heads.append(head)
         head_of.append(head)
     results = [None] * n

@@ -40,6 +40,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from . import autotune
 from . import config as cfgmod
 from . import streams as stream_pool
 from .errors import OutOfMemoryBudgetError
@@ -200,7 +201,7 @@ class Server:
                     self._pins[key] = self._pins.get(key, 0) + 1
                     self.stats["analysis_hits"] += 1
                     return key, head, True
-            head = GpuFactorization(pencil)
+            head = GpuFactorization(pencil, **autotune.options_for(pencil))
             with self._lock:
                 self._analyses[key] = head
                 self._pins[key] = self._pins.get(key, 0) + 1

@@ -27,6 +27,7 @@ import numpy as np
 import scipy.sparse as sp
 import torch
 
+from . import autotune
 from . import config as cfgmod
 from . import subspace
 from .driver_core import Ctx, PhaseClock, estimate_norm2
@@ -190,7 +191,10 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     own_fac = fac is None
     with clock.phase("factor"):
         if fac is None:
-            fac = GpuFactorization(pencil)
+            plan = autotune.options_for(pencil)
+            fac = GpuFactorization(pencil, **plan)
+            if plan:
+                rep["parallel"]["notes"].append(f"launch plan from the autotuner: {plan}")
         elif fac.pencil is not pencil:
             raise InvalidInputError("solve: the given factorization object belongs to another pencil")
     if B is not None and bk == "general":
