@@ -390,7 +390,8 @@ def autotune_extra(prepared, sigma):
     r = autotune.tune(prepared.pencil, sigma=sigma)
     return {"chosen": r["options"], "cost_s": r["cost"], "default_cost_s": r["default_cost"],
             "tuning_s": time.perf_counter() - t0,
-            "candidates": [{"options": c["options"], "factor_ms": 1e3 * c["factor_s"], "solve8_ms": 1e3 * c["solve_s"],
+            "candidates": [{"options": c["options"], "factor_ms": 1e3 * c["factor_s"],
+                            "first_solve8_ms": 1e3 * c["first_solve_s"], "solve8_ms": 1e3 * c["solve_s"],
                             "analysis_s": c["analysis_s"], "flops": c["flops"]} for c in r["table"]]}
 
 

@@ -7,8 +7,9 @@ The reference calibrates its CPU cost model per machine (src/driver/steps.cpp:61
 1236-1377); here the candidates are measured on the device itself: for each candidate the pattern
 is analysed with those options and one factorization plus one block solve of `p` right-hand
 sides at a representative shift are timed (HIP events; the first factorization of each candidate
-is a warm-up). The cost is factor_weight * t_factor + solve_weight * t_solve, weighted as a
-spectrum slice uses the factor (one factorization, tens of block solves). The best options are
+is a warm-up). The cost is factor_weight * (t_factor + t_first_solve) + solve_weight * t_solve
+(the first solve after a factorization also builds the inverse panels), weighted as a spectrum
+slice uses the factor (one factorization, tens of block solves). The best options are
 kept per pattern (the hash of the factorized pattern and value maps) in a JSON cache, and
 GpuFactorization objects created by the solver for that pattern use them from then on.
 
@@ -104,22 +105,27 @@ def _time_candidate(pencil: Pencil, opts: dict, sigma: float, p: int, reps: int)
     y = x.clone()
     F.solve(y)
     torch.cuda.synchronize()
-    tf, ts = [], []
+    tf, t1, ts = [], [], []
     for _ in range(reps):
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
         F.factorize(1.0, -sigma, precision="double")
-        e1.record()
+        ev[1].record()
+        y.copy_(x)
+        F.solve(y)  # the first solve after a factorization also builds the inverse panels
+        ev[2].record()
         y.copy_(x)
         F.solve(y)
-        e2.record()
+        ev[3].record()
         torch.cuda.synchronize()
-        tf.append(e0.elapsed_time(e1) * 1e-3)
-        ts.append(e1.elapsed_time(e2) * 1e-3)
+        tf.append(ev[0].elapsed_time(ev[1]) * 1e-3)
+        t1.append(ev[1].elapsed_time(ev[2]) * 1e-3)
+        ts.append(ev[2].elapsed_time(ev[3]) * 1e-3)
     info = F.info
     F.close()
-    return {"options": opts, "factor_s": float(np.median(tf)), "solve_s": float(np.median(ts)),
-            "analysis_s": t_ana, "flops": float(info.flops), "factor_entries": float(info.factor_entries)}
+    return {"options": opts, "factor_s": float(np.median(tf)), "first_solve_s": float(np.median(t1)),
+            "solve_s": float(np.median(ts)), "analysis_s": t_ana, "flops": float(info.flops),
+            "factor_entries": float(info.factor_entries)}
 
 
 def tune(A, B=None, sigma: float = 0.0, p: int = 8, reps: int = 3, factor_weight: float = 1.0,
@@ -133,14 +139,14 @@ def tune(A, B=None, sigma: float = 0.0, p: int = 8, reps: int = 3, factor_weight
                       for sf, pw in product(SMALL_FRONT_MAX, PANEL_WIDTH)]
     table = [_time_candidate(pencil, c, sigma, p, reps) for c in candidates]
     for r in table:
-        r["cost"] = factor_weight * r["factor_s"] + solve_weight * r["solve_s"]
+        r["cost"] = factor_weight * (r["factor_s"] + r["first_solve_s"]) + solve_weight * r["solve_s"]
     best = min(table, key=lambda r: r["cost"])
     if default_grid:
         # then the ordering depth at the best kernel split
         extra = [dict(best["options"], leaf_size=ls) for ls in LEAF_SIZE if ls != best["options"]["leaf_size"]]
         more = [_time_candidate(pencil, c, sigma, p, reps) for c in extra]
         for r in more:
-            r["cost"] = factor_weight * r["factor_s"] + solve_weight * r["solve_s"]
+            r["cost"] = factor_weight * (r["factor_s"] + r["first_solve_s"]) + solve_weight * r["solve_s"]
         table += more
         best = min(table, key=lambda r: r["cost"])
     default = next((r for r in table if r["options"] == DEFAULTS), None)

@@ -392,47 +392,63 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
 void plan_contribution_blocks(Symbolic& S) {
   // A front's contribution block is written while its level is factorized and read while its
   // parent's level is (levels run one after the other): it is live over the levels
-  // [level(s), level(parent)]. Offsets are assigned level by level with a first-fit allocator
-  // over one address range: at the start of level l the blocks whose parent level was l - 1 are
-  // released (coalescing free neighbours), then the level's blocks are placed, largest first, in
-  // the lowest free range that holds them. Blocks whose lifetimes overlap never share storage.
+  // [level(s), level(parent)]. Offsets are assigned level by level over one address range: at the
+  // start of level l the blocks whose parent level was l - 1 are released (coalescing free
+  // neighbours; a free range that reaches the top lowers the top), then the level's blocks are
+  // placed, largest first, each in the smallest free range that holds it (best fit, found by
+  // length in a sorted map), else at the top. Blocks whose lifetimes overlap never share storage.
   const int32_t nsn = S.nsn;
   S.cb_off.assign(static_cast<size_t>(nsn) + 1, 0);
   std::vector<std::vector<int32_t>> ending(static_cast<size_t>(std::max(1, S.nlevels)));
-  std::map<int64_t, int64_t> free_ranges;  // offset -> length
+  std::map<int64_t, int64_t> by_off;            // free ranges: offset -> length
+  std::multimap<int64_t, int64_t> by_len;       // the same ranges: length -> offset
   int64_t top = 0;
+  auto drop_len = [&](int64_t len, int64_t off) {
+    auto r = by_len.equal_range(len);
+    for (auto it = r.first; it != r.second; ++it)
+      if (it->second == off) {
+        by_len.erase(it);
+        return;
+      }
+  };
   auto release = [&](int64_t off, int64_t len) {
     if (len == 0) return;
-    auto it = free_ranges.emplace(off, len).first;
-    auto nx = std::next(it);
-    if (nx != free_ranges.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_ranges.erase(nx);
+    auto nx = by_off.lower_bound(off);
+    if (nx != by_off.end() && off + len == nx->first) {  // merge with the range after
+      len += nx->second;
+      drop_len(nx->second, nx->first);
+      nx = by_off.erase(nx);
     }
-    if (it != free_ranges.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_ranges.erase(it);
-        it = pv;
+    if (nx != by_off.begin()) {  // merge with the range before
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        off = pv->first;
+        len += pv->second;
+        drop_len(pv->second, pv->first);
+        by_off.erase(pv);
       }
     }
-    if (it->first + it->second == top) {  // the range ends at the top: give it back
-      top = it->first;
-      free_ranges.erase(it);
+    if (off + len == top) {
+      top = off;
+      return;
     }
+    by_off.emplace(off, len);
+    by_len.emplace(len, off);
   };
   auto place = [&](int64_t len) -> int64_t {
-    for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it)
-      if (it->second >= len) {
-        const int64_t off = it->first;
-        const int64_t rest = it->second - len;
-        free_ranges.erase(it);
-        if (rest > 0) free_ranges.emplace(off + len, rest);
-        return off;
-      }
-    const int64_t off = top;
-    top += len;
+    auto it = by_len.lower_bound(len);
+    if (it == by_len.end()) {
+      const int64_t off = top;
+      top += len;
+      return off;
+    }
+    const int64_t flen = it->first, off = it->second;
+    by_len.erase(it);
+    by_off.erase(off);
+    if (flen > len) {
+      by_off.emplace(off + len, flen - len);
+      by_len.emplace(flen - len, off + len);
+    }
     return off;
   };
   int64_t peak = 0;
