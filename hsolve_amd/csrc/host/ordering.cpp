@@ -223,23 +223,62 @@ int64_t cut_of(const WGraph& g, const std::vector<int8_t>& part) {
 // passes, each moving vertices by best gain with rollback to the best prefix.
 constexpr size_t kFmStall = 100;
 void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int passes) {
+  // Gains and external weights are computed once and then kept exact by the same update a move
+  // makes (also when a pass rolls moves back), so a pass costs its moves, not a sweep of the
+  // graph; a pass starts from the boundary vertices (external weight > 0), kept in a list that
+  // only grows (entries whose external weight fell to 0 are skipped). The result is the one a
+  // full recount per pass gives: the heap order depends on (gain, vertex) values only.
   const int32_t n = g.n;
   const int64_t maxw = static_cast<int64_t>(max_frac * static_cast<double>(g.total_vw)) + 1;
   std::vector<int64_t> pw(2, 0);
   for (int32_t v = 0; v < n; ++v) pw[part[v]] += g.vw[v];
   std::vector<int32_t> gain(n), ext(n);
-  std::vector<char> moved(n);
-  for (int pass = 0; pass < passes; ++pass) {
-    for (int32_t v = 0; v < n; ++v) {
-      int32_t e = 0, i = 0;
-      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) (part[g.adj[p]] != part[v] ? e : i) += g.ew[p];
-      gain[v] = e - i;
-      ext[v] = e;
+  std::vector<char> moved(n, 0), listed(n, 0);
+  std::vector<int32_t> boundary;
+  for (int32_t v = 0; v < n; ++v) {
+    int32_t e = 0, i = 0;
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) (part[g.adj[p]] != part[v] ? e : i) += g.ew[p];
+    gain[v] = e - i;
+    ext[v] = e;
+    if (e > 0) {
+      boundary.push_back(v);
+      listed[v] = 1;
     }
-    std::fill(moved.begin(), moved.end(), 0);
+  }
+  // moves v to the other side and updates its neighbours' gains and external weights
+  auto flip = [&](int32_t v) {
+    const int to = 1 - part[v];
+    pw[part[v]] -= g.vw[v];
+    pw[to] += g.vw[v];
+    part[v] = static_cast<int8_t>(to);
+    gain[v] = -gain[v];
+    int32_t e = 0;
+    for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+      const int32_t u = g.adj[p];
+      const int32_t w = g.ew[p];
+      if (part[u] == to) {
+        gain[u] -= 2 * w;
+        ext[u] -= w;
+      } else {
+        gain[u] += 2 * w;
+        ext[u] += w;
+        e += w;
+        if (!listed[u]) {
+          listed[u] = 1;
+          boundary.push_back(u);
+        }
+      }
+    }
+    ext[v] = e;
+    if (e > 0 && !listed[v]) {
+      listed[v] = 1;
+      boundary.push_back(v);
+    }
+  };
+  for (int pass = 0; pass < passes; ++pass) {
     using Item = std::pair<int32_t, int32_t>;  // (gain, -vertex) max-heap
     std::priority_queue<Item> heap[2];
-    for (int32_t v = 0; v < n; ++v)
+    for (int32_t v : boundary)
       if (ext[v] > 0) heap[part[v]].push({gain[v], -v});
     std::vector<int32_t> seq;
     int64_t cur = 0, best = 0;
@@ -268,22 +307,12 @@ void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int 
         if (v < 0) from = 1 - from;
       }
       if (v < 0) break;
-      const int to = 1 - part[v];
       cur -= gain[v];
-      pw[part[v]] -= g.vw[v];
-      pw[to] += g.vw[v];
-      part[v] = static_cast<int8_t>(to);
+      flip(v);
       moved[v] = 1;
       seq.push_back(v);
-      gain[v] = -gain[v];
       for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
         const int32_t u = g.adj[p];
-        const int32_t w = g.ew[p];
-        if (part[u] == to) {
-          gain[u] -= 2 * w;
-        } else {
-          gain[u] += 2 * w;
-        }
         if (!moved[u]) heap[part[u]].push({gain[u], -u});
       }
       if (cur < best) {
@@ -291,13 +320,9 @@ void fm_refine(const WGraph& g, std::vector<int8_t>& part, double max_frac, int 
         best_len = seq.size();
       }
     }
-    // roll back beyond the best prefix
-    for (size_t k = seq.size(); k > best_len; --k) {
-      const int32_t v = seq[k - 1];
-      pw[part[v]] -= g.vw[v];
-      part[v] = static_cast<int8_t>(1 - part[v]);
-      pw[part[v]] += g.vw[v];
-    }
+    // roll back beyond the best prefix (the inverse moves, in reverse order)
+    for (size_t k = seq.size(); k > best_len; --k) flip(seq[k - 1]);
+    for (int32_t v : seq) moved[v] = 0;
     if (best_len == 0) break;
   }
 }
