@@ -44,6 +44,7 @@ SIGNATURES = {
     "hsg_analysis_get_order": (C.c_int, [_P, _P]),
     "hsg_analysis_get_fronts": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "hsg_factorize": (C.c_int, [_P, _D, _D, _P, _P, C.c_int32, _D, _P, C.POINTER(Inertia), C.POINTER(C.c_int32)]),
+    "hsg_factorize_batch": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.c_int32, _D, _P, C.POINTER(Inertia)]),
     "hsg_factor_solve": (C.c_int, [_P, _P, _I64, _I64, _P]),
     "hsg_release_factors": (C.c_int, [_P]),
     "hsg_factor_destroy": (None, [_P]),
@@ -71,7 +72,7 @@ for _name in ("csr_spmm", "diag_spmm", "shifted_residual", "gram", "block_update
 # point only enqueues one or two kernels on a stream and returns within microseconds: those keep the
 # GIL. Concurrent spectrum slices drive the device from several Python threads, and a thread that
 # gave up the GIL for a short call could wait a full interpreter switch interval to get it back.
-RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_factor_solve", "hsg_factor_clone", "hsg_release_factors",
+RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_factorize_batch", "hsg_factor_solve", "hsg_factor_clone", "hsg_release_factors",
                           "hsg_factor_destroy", "hsg_release_cached_memory"})
 
 

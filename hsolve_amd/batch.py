@@ -8,6 +8,12 @@ report. What the batch adds:
     factorize on factorization objects cloned from one analysis (GpuFactorization.clone_for), the
     counterpart of the reference's helper processes holding the problem's analysis
     (src/driver/helpers.cpp); only the first solve of a pattern pays the analysis;
+  * batched window-end counts: before the solves, the exact Sylvester counts at the ends of every
+    interval target are factorized for all pencils of a pattern together, up to 64 factorizations
+    in the same kernel launches (GpuFactorization.count_batch: the pencil index is the launches' z
+    dimension, the level schedule is the pattern's), instead of two factorizations per pencil in
+    its own solve. A count equals the one the pencil's solve would make at that end (same pivots);
+    an end that is singular with null-pivot detection is left to the solve, which moves it;
   * concurrency: `workers` host threads, each with a stream of its own, take the pencils in order;
     a pencil's result does not depend on which worker solved it or on the others (each solve owns
     its factorization, Krylov scratch and report).
@@ -26,7 +32,34 @@ from .factor import GpuFactorization, same_pattern
 from .solver import _one_blas_thread, _solve, prepare
 
 
-def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host", share_analysis: bool = True):
+def _window_end_counts(heads, head_of, prepared, cfgs):
+    """Per pencil, the (shift, nu) counts at the ends of its interval target, batched per pattern;
+    a singular end is left out (the solve moves it as usual)."""
+    counted = [[] for _ in prepared]
+    for head in heads:
+        items, owner = [], []
+        for i, p in enumerate(prepared):
+            t = cfgs[i]["target"]
+            if head_of[i] is not head or t["mode"] != "interval":
+                continue
+            for s in dict.fromkeys((float(t["lower"]), float(t["upper"]))):
+                items.append((p.pencil, 1.0, -s))
+                owner.append((i, s))
+        if not items:
+            continue
+        counter = head.clone()
+        try:
+            got = counter.count_batch(items)
+        finally:
+            counter.close()
+        for (i, s), inn in zip(owner, got):
+            if inn.valid:
+                counted[i].append((s, inn.neg))
+    return counted
+
+
+def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host", share_analysis: bool = True,
+                batch_counts: bool = True):
     """pencils: a list of (A, B) (B may be None) or of solver.Prepared; config: one configuration
     for all, or a list with one per pencil. Returns a list of (eigenvalues, eigenvectors, report,
     config) in the pencils' order.
@@ -35,7 +68,9 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
     numeric factorization is a clone of its pattern's analysis made when a worker takes the pencil
     and freed as soon as its solve ends (also when it fails), so at most `workers` factorizations
     are held at a time whatever the number of pencils. The freed blocks stay in the library's
-    cache, ordered after the worker's stream, and the next pencil's clone reuses them."""
+    cache, ordered after the worker's stream, and the next pencil's clone reuses them.
+    batch_counts: count the ends of interval targets for all pencils of a pattern in batched
+    factorizations before the solves (the results are those of solves that count themselves)."""
     n = len(pencils)
     cfgs = config if isinstance(config, list) else [config] * n
     if len(cfgs) != n:
@@ -51,6 +86,7 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
             head = GpuFactorization(p.pencil, **autotune.options_for(p.pencil))
             heads.append(head)
         head_of.append(head)
+    counted = [[] for _ in prepared]
     results = [None] * n
     errors = [None] * n
     nxt = [0]
@@ -75,7 +111,7 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
                     with lock:
                         live[0] += 1
                         live[1] = max(live[1], live[0])
-                    results[i] = _solve(None, None, cfgs[i], prepared[i], vectors, 1, fac)
+                    results[i] = _solve(None, None, cfgs[i], prepared[i], vectors, 1, fac, counted[i])
                 except BaseException as e:  # re-raised in pencil order below
                     errors[i] = e
                 finally:
@@ -88,6 +124,8 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
     try:
         with _one_blas_thread():
             torch.cuda.synchronize()
+            if batch_counts:
+                counted = _window_end_counts(heads, head_of, prepared, cfgs)
             threads = [threading.Thread(target=worker, args=(w,), name=f"hsolve-batch-{w}") for w in range(workers)]
             for t in threads:
                 t.start()
@@ -105,5 +143,6 @@ def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host"
         r[2].pop("_events", None)
         r[2]["parallel"]["notes"].append(
             f"batch of {n} pencils on {workers} concurrent streams; {len(heads)} analyses shared by pattern; "
-            f"at most {live[1]} factorizations held at once")
+            f"at most {live[1]} factorizations held at once; "
+            f"{sum(len(c) for c in counted)} window-end counts made in batched factorizations")
     return results

@@ -181,6 +181,32 @@ int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_valu
   });
 }
 
+int hsg_factorize_batch(hsg_factor* f, int32_t count, const double* alphas, const double* betas,
+                        const double* const* a_values, const double* const* b_values, int32_t precision,
+                        double null_pivot_tol, void* stream, hsg_inertia* inertias) {
+  return guarded([&] {
+    require(f && alphas && betas && a_values && inertias, "hsg_factorize_batch: NULL argument");
+    require(count >= 1 && count <= HSG_MAX_BATCH, "hsg_factorize_batch: count must be in [1, 64]");
+    require(precision == HSG_DOUBLE || precision == HSG_SINGLE, "hsg_factorize_batch: precision must be 0 or 1");
+    require(null_pivot_tol >= 0.0 && null_pivot_tol < 1.0, "hsg_factorize_batch: null_pivot_tol must be in [0, 1)");
+    std::vector<const double*> bv(count, nullptr);
+    for (int32_t z = 0; z < count; ++z) {
+      require(a_values[z] != nullptr, "hsg_factorize_batch: a_values[z] is NULL");
+      if (b_values) bv[z] = b_values[z];
+    }
+    std::vector<hsg::InertiaResult> r(count);
+    f->engine->factorize_batch(f->num, count, alphas, betas, a_values, bv.data(), precision, null_pivot_tol,
+                               as_stream(stream), r.data());
+    for (int32_t z = 0; z < count; ++z) {
+      inertias[z].neg = r[z].neg;
+      inertias[z].zero = r[z].zero;
+      inertias[z].pos = r[z].pos;
+      inertias[z].valid = r[z].singular ? 0 : 1;
+      inertias[z].two_by_two = static_cast<int32_t>(r[z].two);
+    }
+  });
+}
+
 int hsg_factor_solve(hsg_factor* f, double* x, int64_t nrhs, int64_t ldx, void* stream) {
   return guarded([&] {
     require(f && x, "hsg_factor_solve: NULL argument");

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -15,6 +16,8 @@
 #include "timing.hpp"
 
 namespace hsg {
+
+struct BatchValues;  // the values of a batched factorization launch (front_common.hpp)
 
 struct EngineOptions {
   int panel_width = 64;       // nb of the large-front path (<= 64)
@@ -38,6 +41,7 @@ struct Numeric {
   void note_stream(hipStream_t st);  // a stream this factorization's kernels were queued on
   std::vector<hipStream_t> streams_;  // streams whose queued work may use the buffers below
   int precision_ = -1;
+  int batch_ = 1;  // factorizations held side by side (batched launches); solves use batch 1 only
   bool valid_ = false;
   void *d_L_ = nullptr, *d_CB_ = nullptr, *d_W_ = nullptr, *d_dval_ = nullptr, *d_doff_ = nullptr;
   int8_t* d_ptype_ = nullptr;
@@ -69,6 +73,12 @@ class Engine {
   // Numeric factorization into nm; returns the inertia counts (synchronizes the stream).
   InertiaResult factorize(Numeric& nm, double alpha, double beta, const double* a_values, const double* b_values,
                           int precision, double null_tol, hipStream_t stream);
+  // Z (1..64) factorizations of the same pattern, alpha_z A_z + beta_z B_z, in the same launches
+  // (grid z = factorization): inertia counts into out[0..Z); the factors are kept side by side in nm
+  // and are not solvable (nm is left without a valid factor). Synchronizes the stream.
+  void factorize_batch(Numeric& nm, int Z, const double* alphas, const double* betas, const double* const* a_values,
+                       const double* const* b_values, int precision, double null_tol, hipStream_t stream,
+                       InertiaResult* out);
   // In-place solve of nrhs FP64 columns with the last factorization held in nm.
   void solve(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
   void release(Numeric& nm);
@@ -98,15 +108,16 @@ class Engine {
     int64_t begin = 0, count = 0;
   };
   template <class T>
-  InertiaResult factorize_t(Numeric& nm, double alpha, double beta, const double* av, const double* bv,
-                            double null_tol, hipStream_t stream);
+  void factorize_t(Numeric& nm, int Z, const BatchValues& bv, double null_tol, hipStream_t stream, InertiaResult* out);
   template <class T>
   void solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStream_t stream);
   template <class T>
   void build_inverse_panels(Numeric& nm, hipStream_t stream);  // G, Gt from L (first solve after a factorization)
   void plan_solve();                              // host task lists of the solve phase (symbolic)
   void validate_solve_plan() const;
-  void alloc_numeric(Numeric& nm, int precision);
+  void alloc_numeric(Numeric& nm, int precision, int batch);
+  // entries of the Bunch-Kaufman block permutation array of one factorization
+  int64_t bperm_stride() const { return (int64_t)std::max(1, max_large_per_level_) * opt_.panel_width; }
   void ensure_device();  // uploads the analysis on first use (the analysis itself is host-only)
 
   Symbolic sym_;

@@ -65,9 +65,15 @@ __global__ void k_fill_u64(unsigned long long* p, int n) {
 // the counter slot kCntKnorm (bit pattern of a non-negative double: integer max is value max).
 template <class T>
 __global__ void k_assemble_original(int64_t nnz, const int64_t* __restrict__ dst, const int64_t* __restrict__ ia,
-                                    const int64_t* __restrict__ ib, const double* __restrict__ av,
-                                    const double* __restrict__ bv, double alpha, double beta, T* __restrict__ L,
-                                    unsigned long long* cnt) {
+                                    const int64_t* __restrict__ ib, BatchValues bvals, T* __restrict__ L,
+                                    int64_t zL, unsigned long long* cnt) {
+  // factorization z = blockIdx.y of a batch: its values, coefficients, panels and counters
+  const int z = blockIdx.y;
+  const double* __restrict__ av = bvals.a[z];
+  const double* __restrict__ bv = bvals.b[z];
+  const double alpha = bvals.alpha[z], beta = bvals.beta[z];
+  L += z * zL;
+  cnt += z * kCntSlots;
   double local_max = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t d = dst[e];
@@ -362,6 +368,7 @@ __device__ void flush_counts(const PivotCounts& pc, unsigned long long* cnt) {
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_factor_small(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                           double null_tol, int detect) {
+  N = N.at(blockIdx.z);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int s = list[blockIdx.x];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -476,6 +483,7 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ rel,
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_extend_add_large(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                               int part) {
+  N = N.at(blockIdx.z);
   __shared__ int jr[2][2];
   const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s];
@@ -605,8 +613,9 @@ constexpr int kBkCols = 64 / kBkWaves;  // columns of the block per wave
 
 template <class T>
 __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                           int p, int nb, int32_t* __restrict__ bperm, double null_tol,
-                                                           int detect) {
+                                                           int p, int nb, double null_tol, int detect) {
+  N = N.at(blockIdx.z);
+  int32_t* __restrict__ bperm = N.bperm;
   __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D)
   __shared__ T Lm[64 * kBkLd];  // multipliers of the pivot columns (strictly lower L11)
   __shared__ int perm[64];
@@ -883,7 +892,9 @@ __device__ __forceinline__ T quad_bcast(T v, int s) {
 // [c0, c0 + w) of the earlier panel columns [0, c0).
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_panel_apply(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
-                                                         int p, int nb, const int32_t* __restrict__ bperm) {
+                                                         int p, int nb) {
+  N = N.at(blockIdx.z);
+  const int32_t* __restrict__ bperm = N.bperm;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int f = blockIdx.y;
   const int s = list[f];
@@ -1053,6 +1064,7 @@ struct MatrixCore<float> {
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_gemm_update(DevSym S, DevNum<T> N, const int32_t* __restrict__ list,
                                                          int mode, int p, int nb) {
+  N = N.at(blockIdx.z);
   typedef typename MatrixCore<T>::acc4 acc4;
   const int s = list[blockIdx.y];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s];
@@ -1729,23 +1741,24 @@ void Numeric::free_all() {
 
 void Engine::release(Numeric& nm) { nm.free_all(); }
 
-void Engine::alloc_numeric(Numeric& nm, int precision) {
-  if (nm.precision_ == precision && nm.d_L_) return;
+void Engine::alloc_numeric(Numeric& nm, int precision, int batch) {
+  if (nm.precision_ == precision && nm.batch_ == batch && nm.d_L_) return;
   nm.free_all();
   const size_t ts = precision == 0 ? sizeof(double) : sizeof(float);
   const int64_t n = sym_.n;
-  dalloc_into(nm.d_L_, std::max<int64_t>(1, panel_total_) * ts, "hipMalloc (factor panels)");
-  dalloc_into(nm.d_CB_, std::max<int64_t>(1, cb_total_) * ts, "hipMalloc (contribution blocks)");
-  dalloc_into(nm.d_W_, std::max<int64_t>(1, w_total_) * ts, "hipMalloc (panel workspace)");
-  dalloc_into(nm.d_dval_, n * ts, "hipMalloc (D)");
-  dalloc_into(nm.d_doff_, n * ts, "hipMalloc (D)");
-  dalloc_into(nm.d_ptype_, n, "hipMalloc (pivots)");
-  dalloc_into(nm.d_piv_, n * sizeof(int32_t), "hipMalloc (pivots)");
-  dalloc_into(nm.d_ipiv_, n * sizeof(int32_t), "hipMalloc (pivots)");
-  dalloc_into(nm.d_bperm_, std::max(1, max_large_per_level_) * (size_t)opt_.panel_width * sizeof(int32_t),
-              "hipMalloc (pivots)");
-  dalloc_into(nm.d_cnt_, kCntSlots * sizeof(unsigned long long), "hipMalloc (counters)");
+  const size_t z = (size_t)std::max(1, batch);
+  dalloc_into(nm.d_L_, z * std::max<int64_t>(1, panel_total_) * ts, "hipMalloc (factor panels)");
+  dalloc_into(nm.d_CB_, z * std::max<int64_t>(1, cb_total_) * ts, "hipMalloc (contribution blocks)");
+  dalloc_into(nm.d_W_, z * std::max<int64_t>(1, w_total_) * ts, "hipMalloc (panel workspace)");
+  dalloc_into(nm.d_dval_, z * n * ts, "hipMalloc (D)");
+  dalloc_into(nm.d_doff_, z * n * ts, "hipMalloc (D)");
+  dalloc_into(nm.d_ptype_, z * n, "hipMalloc (pivots)");
+  dalloc_into(nm.d_piv_, z * n * sizeof(int32_t), "hipMalloc (pivots)");
+  dalloc_into(nm.d_ipiv_, z * n * sizeof(int32_t), "hipMalloc (pivots)");
+  dalloc_into(nm.d_bperm_, z * bperm_stride() * sizeof(int32_t), "hipMalloc (pivots)");
+  dalloc_into(nm.d_cnt_, z * kCntSlots * sizeof(unsigned long long), "hipMalloc (counters)");
   nm.precision_ = precision;
+  nm.batch_ = (int)z;
 }
 
 // a kernel launch counted in `nl` (launches of a timed phase)
@@ -1756,29 +1769,36 @@ void Engine::alloc_numeric(Numeric& nm, int precision) {
   } while (0)
 
 template <class T>
-InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const double* av, const double* bv, double null_tol,
-                                  hipStream_t st) {
+void Engine::factorize_t(Numeric& nm, int Z, const BatchValues& bv, double null_tol, hipStream_t st,
+                         InertiaResult* out) {
   DevSym S{sym_.n, sym_.nsn, d_first_, d_ncol_, d_m_, d_rowoff_, d_panel_off_, d_cb_off_, d_w_off_,
            d_child_ptr_, d_child_list_, d_relmap_, d_rows_orig_};
   DevNum<T> N{(T*)nm.d_L_, (T*)nm.d_CB_, (T*)nm.d_W_, (T*)nm.d_dval_, (T*)nm.d_doff_, nm.d_ptype_, nm.d_piv_, nm.d_ipiv_, nm.d_cnt_};
+  N.bperm = nm.d_bperm_;
+  N.zL = std::max<int64_t>(1, panel_total_);
+  N.zCB = std::max<int64_t>(1, cb_total_);
+  N.zW = std::max<int64_t>(1, w_total_);
+  N.zn = sym_.n;
+  N.zbp = bperm_stride();
   const int nb = opt_.panel_width;
   const int detect = null_tol > 0 ? 1 : 0;
   const int64_t tok = timing_begin(kPhaseFactor, st);
   int64_t nl = 0;
-  HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(panel_total_ / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.L, panel_total_);
-  HSG_LAUNCH_COUNTED(k_fill_u64, dim3(1), dim3(64), 0, st, nm.d_cnt_, (int)kCntSlots);
+  const int64_t lz = N.zL * Z;
+  HSG_LAUNCH_COUNTED(k_zero<T>, dim3(blocks_for(lz / (16 / sizeof(T)) + 1)), dim3(256), 0, st, N.L, lz);
+  HSG_LAUNCH_COUNTED(k_fill_u64, dim3((Z * kCntSlots + 63) / 64), dim3(64), 0, st, nm.d_cnt_, Z * (int)kCntSlots);
   if (nnz_ > 0)
-    HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 1024)), dim3(kThreads), 0, st, nnz_, d_asm_, d_ia_,
-                       d_ib_, av, bv, alpha, beta, N.L, nm.d_cnt_);
+    HSG_LAUNCH_COUNTED(k_assemble_original<T>, dim3(blocks_for(nnz_, 256, 1024), Z), dim3(kThreads), 0, st, nnz_, d_asm_,
+                       d_ia_, d_ib_, bv, N.L, N.zL, nm.d_cnt_);
   for (const Level& L : levels_) {
     if (L.n_small) {
       const int m = L.small_maxm;
       const size_t shm = (size_t)m * m * sizeof(T) + 4 * (size_t)m * sizeof(T) + (size_t)m * sizeof(int) + m + 16;
-      HSG_LAUNCH_COUNTED(k_factor_small<T>, dim3(L.n_small), dim3(kThreads), shm, st, S, N, L.d_small, null_tol,
+      HSG_LAUNCH_COUNTED(k_factor_small<T>, dim3(L.n_small, 1, Z), dim3(kThreads), shm, st, S, N, L.d_small, null_tol,
                          detect);
     }
     if (L.n_large) {
-      HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_ncol + kEaCols - 1) / kEaCols, L.n_large),
+      HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_ncol + kEaCols - 1) / kEaCols, L.n_large, Z),
                          dim3(kThreads), 0, st, S, N, L.d_large, 0);
       const int npan = (L.large_max_ncol + nb - 1) / nb;
       const size_t shm_bk = (size_t)nb * nb * sizeof(T) + 4 * (size_t)nb * sizeof(T) + nb * sizeof(int) + nb + 16;
@@ -1787,57 +1807,76 @@ InertiaResult Engine::factorize_t(Numeric& nm, double alpha, double beta, const 
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
-        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb, nm.d_bperm_,
+        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large, 1, Z), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb,
                            null_tol, detect);
-        HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large), dim3(kThreads), shm_pa, st, S, N, L.d_large, p,
-                           nb, (const int32_t*)nm.d_bperm_);
+        HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large, Z), dim3(kThreads), shm_pa, st, S, N, L.d_large,
+                           p, nb);
         const int rem = L.large_maxm - (p + 1) * nb;
         if (rem > 0 && (p + 1) * nb < L.large_max_ncol) {
           const int t = (rem + kTile - 1) / kTile;
           const int tc = (L.large_max_ncol - (p + 1) * nb + kTile - 1) / kTile;
           const int64_t ut = timing_begin(kPhaseUpdate, st);
-          HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * tc, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 0, p,
-                             nb);
+          HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * tc, L.n_large, Z), dim3(kThreads), 0, st, S, N, L.d_large, 0,
+                             p, nb);
           timing_end(ut, st, 1, L.upd_flops[p]);
         }
       }
       if (L.large_max_cb > 0) {
         const int t = (L.large_max_cb + kTile - 1) / kTile;
         const int64_t ut = timing_begin(kPhaseUpdate, st);
-        HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * t, L.n_large), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0, nb);
+        HSG_LAUNCH_COUNTED(k_gemm_update<T>, dim3(t * t, L.n_large, Z), dim3(kThreads), 0, st, S, N, L.d_large, 1, 0,
+                           nb);
         timing_end(ut, st, 1, L.cb_flops);
-        HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_cb + kEaCols - 1) / kEaCols, L.n_large),
+        HSG_LAUNCH_COUNTED(k_extend_add_large<T>, dim3((L.large_max_cb + kEaCols - 1) / kEaCols, L.n_large, Z),
                            dim3(kThreads), 0, st, S, N, L.d_large, 1);
       }
     }
   }
   hip_check(hipGetLastError(), "factorization kernels");
-  timing_end(tok, st, nl, sym_.flops);
-  unsigned long long h[kCntSlots];
-  hip_check(hipMemcpyAsync(h, nm.d_cnt_, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync (counters)");
+  timing_end(tok, st, nl, sym_.flops * Z);
+  std::vector<unsigned long long> h((size_t)Z * kCntSlots);
+  hip_check(hipMemcpyAsync(h.data(), nm.d_cnt_, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+            "hipMemcpyAsync (counters)");
   hip_check(hipStreamSynchronize(st), "factorization");
-  InertiaResult r;
-  r.neg = (int64_t)h[kCntNeg];
-  r.zero = (int64_t)h[kCntZero];
-  r.pos = (int64_t)h[kCntPos];
-  r.two = (int64_t)h[kCnt2x2];
-  r.exact_zero = (int64_t)h[kCntExactZero];
-  r.singular = r.zero > 0;
-  if (r.neg + r.zero + r.pos != sym_.n) throw std::runtime_error("factorization: pivot counts do not add up to n");
-  return r;
+  for (int z = 0; z < Z; ++z) {
+    const unsigned long long* c = h.data() + (size_t)z * kCntSlots;
+    InertiaResult& r = out[z];
+    r.neg = (int64_t)c[kCntNeg];
+    r.zero = (int64_t)c[kCntZero];
+    r.pos = (int64_t)c[kCntPos];
+    r.two = (int64_t)c[kCnt2x2];
+    r.exact_zero = (int64_t)c[kCntExactZero];
+    r.singular = r.zero > 0;
+    if (r.neg + r.zero + r.pos != sym_.n) throw std::runtime_error("factorization: pivot counts do not add up to n");
+  }
 }
 
 InertiaResult Engine::factorize(Numeric& nm, double alpha, double beta, const double* av, const double* bv, int precision,
                                 double null_tol, hipStream_t stream) {
+  InertiaResult r;
+  factorize_batch(nm, 1, &alpha, &beta, &av, &bv, precision, null_tol, stream, &r);
+  nm.valid_ = !r.singular;
+  return r;
+}
+
+void Engine::factorize_batch(Numeric& nm, int Z, const double* alphas, const double* betas, const double* const* avs,
+                             const double* const* bvs, int precision, double null_tol, hipStream_t stream,
+                             InertiaResult* out) {
+  if (Z < 1 || Z > kMaxBatch) throw std::runtime_error("factorize_batch: between 1 and 64 factorizations per batch");
   nm.valid_ = false;
   nm.g_ready_ = false;
   ensure_device();
-  alloc_numeric(nm, precision);
+  alloc_numeric(nm, precision, Z);
   nm.note_stream(stream);
-  InertiaResult r = precision == 0 ? factorize_t<double>(nm, alpha, beta, av, bv, null_tol, stream)
-                                   : factorize_t<float>(nm, alpha, beta, av, bv, null_tol, stream);
-  nm.valid_ = !r.singular;
-  return r;
+  BatchValues bv{};
+  for (int z = 0; z < Z; ++z) {
+    bv.a[z] = avs[z];
+    bv.b[z] = bvs[z];
+    bv.alpha[z] = alphas[z];
+    bv.beta[z] = betas[z];
+  }
+  if (precision == 0) factorize_t<double>(nm, Z, bv, null_tol, stream, out);
+  else factorize_t<float>(nm, Z, bv, null_tol, stream, out);
 }
 
 template <class T>

@@ -33,6 +33,17 @@ enum : int8_t { kPiv1 = 0, kPiv2First = 1, kPiv2Second = 2, kPivNull = 3 };
 // Counter slots of a factorization (unsigned long long, device).
 enum { kCntNeg = 0, kCntZero = 1, kCntPos = 2, kCnt2x2 = 3, kCntKnorm = 4, kCntExactZero = 5, kCntSlots = 8 };
 
+// Values of the factorizations of one batched launch (at most kMaxBatch): for factorization z,
+// K_z = alpha[z] A_z + beta[z] B_z with A_z's and B_z's values at a[z], b[z] (b[z] may be null when
+// the pattern has no B entries). Passed by value as a kernel argument (2 KiB).
+constexpr int kMaxBatch = 64;
+struct BatchValues {
+  const double* a[kMaxBatch];
+  const double* b[kMaxBatch];
+  double alpha[kMaxBatch];
+  double beta[kMaxBatch];
+};
+
 template <class T>
 struct DevNum {
   T* L;            // panels: m x ncol per front, column-major; diagonal block holds D
@@ -44,6 +55,27 @@ struct DevNum {
   int32_t* piv;    // pivot position (first + i) -> local front row index
   int32_t* ipiv;   // local front row index (first + r, r < ncol) -> pivot position offset i
   unsigned long long* cnt;
+  int32_t* bperm = nullptr;  // per (large front of a level, panel column): the panel's local pivots
+  // A batch of factorizations of one analysis (pencils sharing a pattern, or shifts of one
+  // pencil) keeps factorization z's arrays at these element offsets from the pointers above; the
+  // kernels of a batched launch take z = blockIdx.z.
+  int64_t zL = 0, zCB = 0, zW = 0, zn = 0, zbp = 0;
+  __device__ __forceinline__ DevNum at(int z) const {
+    DevNum o = *this;
+    if (z) {
+      o.L += z * zL;
+      o.CB += z * zCB;
+      o.W += z * zW;
+      o.dval += z * zn;
+      o.doff += z * zn;
+      o.ptype += z * zn;
+      o.piv += z * zn;
+      o.ipiv += z * zn;
+      o.cnt += z * kCntSlots;
+      o.bperm += z * zbp;
+    }
+    return o;
+  }
 };
 
 __device__ __forceinline__ double as_double_bits(unsigned long long b) { return __longlong_as_double((long long)b); }

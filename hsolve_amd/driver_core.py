@@ -104,6 +104,7 @@ class Ctx:
     streams: int = 1               # concurrent streams for spectrum slices
     events: list = field(default_factory=list)
     certified: list = field(default_factory=list)
+    counted: list = field(default_factory=list)  # (shift, nu): exact counts made ahead (batched)
 
     def warn(self, msg: str):
         self.rep["warnings"].append(msg)

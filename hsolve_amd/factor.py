@@ -330,6 +330,55 @@ class GpuFactorization:
         self.factorize(alpha, beta, precision="double")
         return self._inertia
 
+    MAX_BATCH = 64  # HSG_MAX_BATCH
+
+    def count_batch(self, items, memory_bytes: float | None = None) -> list[Inertia]:
+        """Sylvester inertia of alpha A_z + beta B_z for items [(pencil_z, alpha_z, beta_z), ...],
+        every pencil of this object's factorized pattern (same_pattern), from FP64 factorizations
+        made MAX_BATCH at a time in the same kernel launches (the pencil index is the launch's z
+        dimension), null-pivot detection as set. Each count equals count(alpha_z, beta_z) on that
+        pencil alone. memory_bytes bounds the device storage of one batch (panels, contribution
+        blocks and workspace of every factorization in it; default 1/4 of the free device memory).
+        The batched factors are released afterwards: this object then holds no usable
+        factorization."""
+        out: list[Inertia] = []
+        one = float(self.info.panel_bytes_double + self.info.cb_bytes_double + self.info.workspace_bytes_double)
+        if memory_bytes is None:
+            memory_bytes = 0.25 * torch.cuda.mem_get_info()[0]
+        per_launch = int(max(1, min(self.MAX_BATCH, memory_bytes // max(one, 1.0))))
+        for p, _, _ in items:
+            if p is not self.pencil and not same_pattern(self.pencil, p):
+                raise InvalidInputError("count_batch: a pencil's factorization pattern differs from the analysed one")
+        tol = self.NULL_PIVOT_TOL[self.DOUBLE] if self.detection else 0.0
+        self._inertia = Inertia(0, 0, 0, False)
+        self.sigma_held = None
+        try:
+            for at in range(0, len(items), per_launch):
+                part = items[at:at + per_launch]
+                z = len(part)
+                devs = [p.to_device() for p, _, _ in part]
+                alphas = (C.c_double * z)(*[float(a) for _, a, _ in part])
+                betas = (C.c_double * z)(*[float(b) for _, _, b in part])
+                av = (C.c_void_p * z)(*[ptr(d["f_a"]).value for d in devs])
+                bv = (C.c_void_p * z)(*[ptr(d["f_b"]).value for d in devs])
+                res = (_lib.Inertia * z)()
+                t0 = time.perf_counter()
+                _lib.call("hsg_factorize_batch", self._handle, z, alphas, betas, av, bv, self.DOUBLE, tol, stream_ptr(),
+                          res)
+                self.stats.factorize_s += time.perf_counter() - t0
+                self.stats.factorizations += z
+                for r in res:
+                    neg, zero, pos, valid = int(r.neg), int(r.zero), int(r.pos), bool(r.valid)
+                    if self.pencil.is_complex:  # as in factorize: each eigenvalue of K twice in E
+                        split = neg % 2 != 0
+                        neg, zero = neg // 2, (zero + 1) // 2
+                        pos = max(0, self.pencil.n - neg - zero)
+                        valid = valid and not split
+                    out.append(Inertia(neg, zero, pos, valid, int(r.two_by_two)))
+        finally:
+            _lib.call("hsg_release_factors", self._handle)
+        return out
+
     def solve(self, x: torch.Tensor) -> torch.Tensor:
         """In place: x (p, n) contiguous on the device (float64, or complex128 for a complex pencil,
         solved through its real view with the embedding), rows = right-hand sides."""

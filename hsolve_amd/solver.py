@@ -159,7 +159,7 @@ def _one_blas_thread():
         return contextlib.nullcontext()
 
 
-def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | None = None):
+def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | None = None, counted=()):
     cfg = cfgmod.parse_config(config or {})
     rep = cfgmod.new_report()
     clock = PhaseClock()
@@ -204,7 +204,8 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
             inn = fac.inertia()
             if not ok or inn.neg > 0:
                 raise UnsupportedError("B is not positive definite (GHIEP): not supported by this engine")
-    ctx = Ctx(cfg=cfg, rep=rep, clock=clock, pencil=pencil, ops=ops, fac=fac, n2a=n2a, n2b=n2b, scale=scale)
+    ctx = Ctx(cfg=cfg, rep=rep, clock=clock, pencil=pencil, ops=ops, fac=fac, n2a=n2a, n2b=n2b, scale=scale,
+              counted=list(counted))
     ctx.streams = slice_streams(fac, streams)
     requested = cfg["factor_precision"]
     rep["precision"]["requested"] = requested

@@ -105,6 +105,11 @@ class WindowSolver:
         self.pool_slice = []
         self.restarts = 0
         self.applies = 0
+        # counts made ahead of the solve (a batch's window ends, factorized with null-pivot
+        # detection at exactly these shifts): the points count() would have produced there
+        for s, nu in ctx.counted:
+            self.remember(CountPoint(float(s), int(nu)))
+            ctx.event("count", s=float(s), nu=int(nu), near_eigenvalue=False, batched=True)
 
     @property
     def c(self) -> Ctx:

@@ -134,6 +134,21 @@ int hsg_factorize(hsg_factor* f, double alpha, double beta, const double* a_valu
                   int32_t precision, double null_pivot_tol, void* stream, hsg_inertia* inertia,
                   int32_t* singular);
 
+/* count (1..HSG_MAX_BATCH) factorizations of the analysed pattern in the same kernel launches,
+ * K_z = alphas[z] A_z + betas[z] B_z, with A_z's / B_z's device value arrays at a_values[z] /
+ * b_values[z] (host arrays of device pointers; b_values may be NULL, or hold NULL entries, when
+ * no b_index is >= 0): the inertia of each into inertias[z], as hsg_factorize computes it for
+ * that K_z alone (same pivots, same counts). The factors are kept side by side in the object's
+ * numeric storage (count times one factorization's bytes, until hsg_release_factors or the next
+ * factorization) and are not solvable: hsg_factor_solve afterwards reports no usable
+ * factorization. Synchronizes the stream. It replaces a loop of SparseFactorization::factorize
+ * calls for inertia counts over pencils sharing one pattern (the Bloch k-points of a cell, shifts
+ * of one pencil), as the reference's helpers count for one analysis (src/driver/helpers.cpp:73). */
+#define HSG_MAX_BATCH 64
+int hsg_factorize_batch(hsg_factor* f, int32_t count, const double* alphas, const double* betas,
+                        const double* const* a_values, const double* const* b_values, int32_t precision,
+                        double null_pivot_tol, void* stream, hsg_inertia* inertias);
+
 /* Solve K X = X in place for nrhs columns (column-major, leading dimension ldx >= n) on the
  * device with the last successful factorization (replaces solve, factor.hpp:394-399; MUMPS JOB=3
  * at mumps_factorization.cpp:506). x is FP64; a single-precision factor rounds it to FP32 for
