@@ -46,6 +46,7 @@ SIGNATURES = {
     "hsg_factorize": (C.c_int, [_P, _D, _D, _P, _P, C.c_int32, _D, _P, C.POINTER(Inertia), C.POINTER(C.c_int32)]),
     "hsg_factorize_batch": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.c_int32, _D, _P, C.POINTER(Inertia)]),
     "hsg_factor_solve": (C.c_int, [_P, _P, _I64, _I64, _P]),
+    "hsg_lanczos_coefficients": (C.c_int, [C.c_int32, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P]),
     "hsg_release_factors": (C.c_int, [_P]),
     "hsg_factor_destroy": (None, [_P]),
     "hsg_factor_clone": (C.c_int, [_P, C.POINTER(_P)]),

@@ -8,12 +8,15 @@ report. What the batch adds:
     factorize on factorization objects cloned from one analysis (GpuFactorization.clone_for), the
     counterpart of the reference's helper processes holding the problem's analysis
     (src/driver/helpers.cpp); only the first solve of a pattern pays the analysis;
-  * batched window-end counts: before the solves, the exact Sylvester counts at the ends of every
-    interval target are factorized for all pencils of a pattern together, up to 64 factorizations
-    in the same kernel launches (GpuFactorization.count_batch: the pencil index is the launches' z
-    dimension, the level schedule is the pattern's), instead of two factorizations per pencil in
-    its own solve. A count equals the one the pencil's solve would make at that end (same pivots);
-    an end that is singular with null-pivot detection is left to the solve, which moves it;
+  * batched window-end counts (batch_counts=True): before the solves, the exact Sylvester counts
+    at the ends of every interval target are factorized for all pencils of a pattern together, up
+    to 64 factorizations in the same kernel launches (GpuFactorization.count_batch: the pencil
+    index is the launches' z dimension, the level schedule is the pattern's), instead of two
+    factorizations per pencil in its own solve. A count equals the one the pencil's solve would
+    make at that end (same pivots); an end that is singular with null-pivot detection is left to
+    the solve, which moves it. Off by default: on the 64 k-points of a 16^3 Bloch cell the batch
+    ran 4.6-5.2 s with it against 4.1-4.5 s without, because the pre-pass runs alone before the
+    workers start while the per-pencil counts overlap with other pencils' Krylov work;
   * concurrency: `workers` host threads, each with a stream of its own, take the pencils in order;
     a pencil's result does not depend on which worker solved it or on the others (each solve owns
     its factorization, Krylov scratch and report).
@@ -59,7 +62,7 @@ def _window_end_counts(heads, head_of, prepared, cfgs):
 
 
 def solve_batch(pencils, config=None, *, workers: int = 4, vectors: str = "host", share_analysis: bool = True,
-                batch_counts: bool = True):
+                batch_counts: bool = False):
     """pencils: a list of (A, B) (B may be None) or of solver.Prepared; config: one configuration
     for all, or a list with one per pencil. Returns a list of (eigenvalues, eigenvectors, report,
     config) in the pencils' order.

@@ -312,6 +312,51 @@ int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, in
   });
 }
 
+int hsg_lanczos_coefficients(int32_t is_complex, int64_t n, const int64_t* rowptr, const int32_t* colind,
+                             const double* values, const double* diag, double* basis, int64_t steps, double* coef,
+                             double* work, void* stream) {
+  return guarded([&] {
+    require(n > 0 && steps >= 1, "hsg_lanczos_coefficients: n and steps must be positive");
+    require((rowptr && colind && values) || diag, "hsg_lanczos_coefficients: M needs CSR arrays or a diagonal");
+    require(basis && coef && work, "hsg_lanczos_coefficients: NULL argument");
+    const hipStream_t st = as_stream(stream);
+    const bool cx = is_complex != 0;
+    const int64_t len = cx ? 2 * n : n;  // doubles per vector
+    StreamScratch& sc = scratch_for(st);
+    std::lock_guard<std::mutex> lock(sc.mu);
+    for (int64_t j = 0; j < steps; ++j) {
+      const double* v = basis + j * len;
+      double* w = basis + (j + 1) * len;  // the next vector is formed in place
+      if (rowptr) {
+        if (cx) hsg::zcsr_spmm(n, rowptr, colind, values, v, n, w, n, 1, 1.0, 0.0, st);
+        else hsg::csr_spmm(n, rowptr, colind, values, v, n, w, n, 1, 1.0, 0.0, st);
+      } else {
+        if (cx) hsg::zdiag_spmm(n, diag, v, n, w, n, 1, 1.0, 0.0, st);
+        else hsg::diag_spmm(n, diag, v, n, w, n, 1, 1.0, 0.0, st);
+      }
+      // alpha_j = Re v_j^H w (the real part is the first double of a complex dot)
+      if (cx) hsg::zcolumn_dots(sc.arena, n, v, n, w, n, 1, nullptr, work, st);
+      else hsg::column_dots(sc.arena, n, v, n, w, n, 1, nullptr, work, st);
+      hsg::hip_check(hipMemcpyAsync(coef + j, work, sizeof(double), hipMemcpyDeviceToDevice, st),
+                "hsg_lanczos_coefficients: hipMemcpyAsync");
+      // full reorthogonalization against v_0..v_j, classical Gram-Schmidt twice
+      for (int pass = 0; pass < 2; ++pass) {
+        if (cx) {
+          hsg::zgram(sc.arena, n, basis, n, j + 1, nullptr, w, n, 1, work, j + 1, st);
+          hsg::zblock_update(n, basis, n, j + 1, work, j + 1, w, n, 1, -1.0, 1.0, st);
+        } else {
+          hsg::gram(sc.arena, n, basis, n, j + 1, nullptr, w, n, 1, work, j + 1, st);
+          hsg::block_update(n, basis, n, j + 1, work, j + 1, w, n, 1, -1.0, 1.0, st);
+        }
+      }
+      if (j + 1 == steps) break;
+      if (cx) hsg::zcolumn_dots(sc.arena, n, w, n, w, n, 1, nullptr, work, st);
+      else hsg::column_dots(sc.arena, n, w, n, w, n, 1, nullptr, work, st);
+      hsg::scale_by_norm(len, w, work, coef + steps + j, st);
+    }
+  });
+}
+
 int hsg_zcsr_spmm(int64_t n, const int64_t* rowptr, const int32_t* colind, const double* values, const double* x,
                   int64_t ldx, double* y, int64_t ldy, int64_t p, double alpha, double beta, void* stream) {
   return guarded([&] {

@@ -57,7 +57,7 @@ __global__ void k_zero(T* __restrict__ p, int64_t n) {
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int n) {
-  const int i = threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
 

@@ -396,6 +396,17 @@ __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, int64
   }
 }
 
+// x *= 1 / b with b = sqrt(max(dot[0], 0)) read on the device (0 when b is 0), b stored to
+// beta_out: the normalization of a Lanczos vector without reading its norm back.
+__global__ void k_scale_by_norm(int64_t len, double* __restrict__ x, const double* __restrict__ dot,
+                                double* __restrict__ beta_out) {
+  const double b = sqrt(fmax(dot[0], 0.0));
+  const double inv = b > 0.0 ? 1.0 / b : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *beta_out = b;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = x[i] * inv;
+}
+
 // Per column c: sums of |ax - lambda_c bx|^2, |bx|^2 and |x|^2 (pair_metrics), partials per workgroup.
 __global__ void __launch_bounds__(256)
     k_pair_sums(int64_t n, const double* __restrict__ ax, const double* __restrict__ bx, const double* __restrict__ x,
@@ -646,6 +657,12 @@ void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* 
   if (n <= 0 || p <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(grid_for(n * p, 256, 16384)), dim3(256), 0, st, n, a, x, ldx, b, y, ldy, (int)p);
   check(hipGetLastError(), "axpby");
+}
+
+void scale_by_norm(int64_t len, double* x, const double* dot, double* beta_out, hipStream_t st) {
+  if (len <= 0) return;
+  hipLaunchKernelGGL(k_scale_by_norm, dim3(grid_for(len, 256, 4096)), dim3(256), 0, st, len, x, dot, beta_out);
+  check(hipGetLastError(), "scale_by_norm");
 }
 
 // ------------------------------------------------------------------------------ no-allocation gram

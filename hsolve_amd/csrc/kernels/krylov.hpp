@@ -44,6 +44,8 @@ constexpr int kMaxDevices = 64;
 size_t gram_noalloc_bytes(int64_t n, int64_t k, int64_t p);
 void gram_noalloc(int64_t n, const double* v, int64_t ldv, int64_t k, const double* wd, const double* w, int64_t ldw,
                   int64_t p, double* c, int64_t ldc, hipStream_t st);
+// x *= 1 / sqrt(max(dot[0], 0)) (0 when that is 0), the square root stored to beta_out (device)
+void scale_by_norm(int64_t len, double* x, const double* dot, double* beta_out, hipStream_t st);
 void axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
            hipStream_t st);
 void block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const double* c, int64_t ldc, double* w,

@@ -24,8 +24,9 @@ import numpy as np
 import scipy.linalg as sla
 import torch
 
+from . import _lib
 from .errors import InternalError, SingularError
-from .factor import GpuFactorization, Pencil
+from .factor import GpuFactorization, Pencil, ptr, stream_ptr
 from .krylov import KrylovOptions, lanczos_krylov_schur
 from .ops import DeviceOps, InexactFactorFailure, ShiftInvertOperator, SolveRefinement
 
@@ -118,7 +119,14 @@ class Ctx:
 
 def estimate_norm2(ops: DeviceOps, apply, steps: int = 30, seed: int = 2026) -> float:
     """Largest |Ritz value| of `steps` plain Lanczos steps (norms.cpp:60-95). GPU products and dots,
-    full reorthogonalization of the few vectors, tridiagonal eigenvalues on the host."""
+    full reorthogonalization of the few vectors, tridiagonal eigenvalues on the host.
+
+    For A and B (apply = ops.apply_a / ops.apply_b) the whole recurrence is one library call
+    (hsg_lanczos_coefficients): it runs on the device without reading anything back, and the
+    coefficients are read once at the end, where the breakdown test truncates the tridiagonal at
+    the first beta_j <= 1e-13 max(|alpha_j|, beta_{j-1}) exactly as a loop stopping there would
+    (the steps before a breakdown do not depend on the ones after it). Other operators run the
+    same recurrence step by step from Python."""
     n = ops.n
     steps = min(steps, n)
     rng = np.random.default_rng([seed, 0x6E6F726D])
@@ -126,21 +134,40 @@ def estimate_norm2(ops: DeviceOps, apply, steps: int = 30, seed: int = 2026) -> 
     v.mul_(1.0 / math.sqrt(float(ops.dots(v, v).cpu()[0])))
     basis = torch.empty((steps + 1, n), dtype=ops.dtype, device=ops.dev)
     basis[0:1].copy_(v)
+    matrix = ops.lanczos_matrix(apply)
+    if matrix is not None:
+        coef = torch.zeros(2 * steps, dtype=torch.float64, device=ops.dev)
+        work = torch.empty(2 * (steps + 1), dtype=torch.float64, device=ops.dev)
+        rowptr, col, val, diag = matrix
+        _lib.call("hsg_lanczos_coefficients", int(ops.is_complex), n, ptr(rowptr), ptr(col), ptr(val), ptr(diag),
+                  ptr(basis), steps, ptr(coef), ptr(work), stream_ptr())
+        ab = coef.cpu().numpy()
+        a_all, b_all = ab[:steps], ab[steps:]
+    else:
+        a_all, b_all = np.zeros(steps), np.zeros(steps)
+        for j in range(steps):
+            w = apply(basis[j:j + 1])
+            a_all[j] = float(ops.dots(basis[j:j + 1], w).cpu()[0])
+            for _ in range(2):
+                c = ops.gram(basis[:j + 1], w)
+                ops.update(basis[:j + 1], c, w, alpha=-1.0, beta=1.0)
+            if j + 1 == steps:
+                break
+            b_all[j] = math.sqrt(max(float(ops.dots(w, w).cpu()[0]), 0.0))
+            if b_all[j] <= 1e-13 * max(abs(a_all[j]), b_all[j - 1] if j else 0.0, 1e-300):
+                break
+            basis[j + 1:j + 2].copy_(w * (1.0 / b_all[j]))
     alpha, beta = [], []
     for j in range(steps):
-        w = apply(basis[j:j + 1])
-        aj = float(ops.dots(basis[j:j + 1], w).cpu()[0])
+        aj = float(a_all[j])
         alpha.append(aj)
-        c = ops.gram(basis[:j + 1], w)
-        ops.update(basis[:j + 1], c, w, alpha=-1.0, beta=1.0)
-        c = ops.gram(basis[:j + 1], w)
-        ops.update(basis[:j + 1], c, w, alpha=-1.0, beta=1.0)
-        bj = math.sqrt(max(float(ops.dots(w, w).cpu()[0]), 0.0))
+        if j + 1 == steps:
+            break
+        bj = float(b_all[j])
         scale = max(abs(aj), beta[-1] if beta else 0.0)
-        if bj <= 1e-13 * max(scale, 1e-300) or j + 1 == steps:
+        if bj <= 1e-13 * max(scale, 1e-300):
             break
         beta.append(bj)
-        basis[j + 1:j + 2].copy_(w * (1.0 / bj))
     T = np.diag(alpha)
     for i, b in enumerate(beta[:len(alpha) - 1]):
         T[i + 1, i] = T[i, i + 1] = b
@@ -343,6 +370,21 @@ def count_once(ctx: Ctx, s: float, detection: bool):
     # the working factors are replaced by this count's factorization
     ctx.held = FactorHeld(s, not detection, "double") if ok else FactorHeld()
     return inn.neg if ok else None
+
+
+def count_batched(ctx: Ctx, shifts, limit: int):
+    """Exact counts nu(A - s B) at several shifts from FP64 factorizations with null-pivot
+    detection made together (GpuFactorization.count_batch), at most `limit` per batch (the memory
+    of `limit` concurrent factorizations): per shift nu, or None where K is singular there (the
+    caller then counts that shift with count_below, which moves it). A count equals count_once(s,
+    True). The factors ctx.fac held are replaced."""
+    fac = ctx.fac
+    one = float(fac.info.panel_bytes_double + fac.info.cb_bytes_double + fac.info.workspace_bytes_double)
+    with ctx.clock.phase("certificate"):
+        fac.set_null_pivot_detection(True)
+        ctx.held = FactorHeld()
+        got = fac.count_batch([(ctx.pencil, 1.0, -s) for s in shifts], memory_bytes=max(1, int(limit)) * one)
+    return [g.neg if g.valid else None for g in got]
 
 
 def count_below(ctx: Ctx, s: float, outward: float, step: float, s_is_gap: bool = False):

@@ -17,7 +17,7 @@ import math
 import numpy as np
 import torch
 
-from .driver_core import (Block, Ctx, DefiniteRun, block_metrics, count_below, factorize_near, keep_converged,
+from .driver_core import (Block, Ctx, DefiniteRun, block_metrics, count_batched, count_below, factorize_near, keep_converged,
                           lanczos_run, nearest_beyond, refactorize_at, select_nearest, verify_and_rescue)
 from .errors import InvalidInputError, SingularError
 from .krylov import LARGEST_MAGNITUDE
@@ -147,8 +147,13 @@ def solve_nearest(ctx: Ctx):
         short_by = min(k, n) - sel.size
         step = 1e-9 * max(ctx.scale, abs(t["sigma"]))
         t0_cert = ctx.clock.t["certificate"]
-        nu_a, a_end, _, _ = count_below(ctx, a_end, -1.0, step, s_is_gap=True)
-        nu_b, b_end, _, _ = count_below(ctx, b_end, +1.0, step, s_is_gap=True)
+        # both ends in one batched factorization when the solve may hold two at once; an end where
+        # K is singular is then counted on its own (count_below moves it)
+        nu_a, nu_b = count_batched(ctx, [a_end, b_end], 2) if ctx.streams > 1 else (None, None)
+        if nu_a is None:
+            nu_a, a_end, _, _ = count_below(ctx, a_end, -1.0, step, s_is_gap=True)
+        if nu_b is None:
+            nu_b, b_end, _, _ = count_below(ctx, b_end, +1.0, step, s_is_gap=True)
         cert["performed"] = True
         cert["lower"], cert["upper"] = a_end, b_end
         cert["count_inertia"] = nu_b - nu_a

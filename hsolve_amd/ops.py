@@ -90,6 +90,18 @@ class DeviceOps:
                       self.n, ptr(y), self.n, p, float(alpha), float(beta), stream_ptr())
         return y
 
+    def lanczos_matrix(self, apply):
+        """(rowptr, col, values, diag) of the matrix `apply` multiplies by, for
+        hsg_lanczos_coefficients: A, or B (CSR, or its diagonal); None for another operator."""
+        d = self.d
+        if getattr(apply, "__func__", None) is DeviceOps.apply_a and apply.__self__ is self:
+            return d["a_rowptr"], d["a_col"], d["a_val"], None
+        if getattr(apply, "__func__", None) is DeviceOps.apply_b and apply.__self__ is self:
+            if self.pencil.b_is_diagonal:
+                return None, None, None, d["b_diag"]
+            return d["b_rowptr"], d["b_col"], d["b_val"], None
+        return None
+
     def b_weight(self):
         """Diagonal weight of the B inner product (None for B = I)."""
         if self.pencil.b_is_identity:

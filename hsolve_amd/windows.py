@@ -570,7 +570,9 @@ class WindowSolver:
 
     def count_all(self, points):
         """Counts at several (shift, outward) points; with more than one stream the ones not yet
-        known are counted at the same time on the worker slots (one FP64 factorization each)."""
+        known are counted at the same time on the worker slots (one FP64 factorization each).
+        (Batched factorizations of these shifts, GpuFactorization.count_batch, were measured no
+        faster on box48 than the concurrent streams: both are bound by the same launch latencies.)"""
         with self._lock:
             known = {p.s for p in self.known}
         todo = list(dict.fromkeys(s for s, _ in points if s not in known))

@@ -239,6 +239,19 @@ int hsg_pair_sums(int64_t n, const double* ax, const double* bx, const double* x
 int hsg_axpby(int64_t n, double a, const double* x, int64_t ldx, double b, double* y, int64_t ldy, int64_t p,
               void* stream);
 
+/* The Lanczos recurrence of the norm estimate (estimate_norm2, src/driver/norms.cpp:60-95) run
+ * on the device without reading anything back: M is CSR (rowptr, colind, values) or diagonal
+ * (diag, when rowptr is NULL), real or complex (is_complex: interleaved values and vectors).
+ * basis: device, steps + 1 vectors of n scalars one after the other, the first one the
+ * normalized start vector. For j < steps: w = M v_j, alpha_j = Re v_j^H w, w orthogonalized
+ * twice against v_0..v_j (classical Gram-Schmidt), and for j + 1 < steps beta_j = ||w||,
+ * v_{j+1} = w / beta_j (0 when beta_j = 0). coef: device, 2 steps doubles, alpha_j at coef[j],
+ * beta_j at coef[steps + j]; work: device, 2 (steps + 1) doubles. The caller reads coef once and
+ * truncates the tridiagonal at the first breakdown, as the step-by-step loop would stop there. */
+int hsg_lanczos_coefficients(int32_t is_complex, int64_t n, const int64_t* rowptr, const int32_t* colind,
+                             const double* values, const double* diag, double* basis, int64_t steps, double* coef,
+                             double* work, void* stream);
+
 /* Complex Hermitian pencils: the same kernels for T = std::complex<double> (the reference's
  * templates above instantiated for cplx = std::complex<double>, include/hsolve/types.hpp:14).
  * Every matrix value, vector and

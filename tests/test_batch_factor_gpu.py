@@ -39,6 +39,9 @@ def test_bloch_kpoints_batch_equals_one_at_a_time(gpu):
         items = [(p, 1.0, -s) for p in pencils for s in shifts]
         got = F.count_batch(items)
         print(f"batch of {len(items)} (n = {pencils[0].n}, complex)", flush=True)
+        # the same batch again on the same object (its counters reused, not freshly allocated)
+        again = F.count_batch(items)
+        assert [(g.neg, g.zero, g.pos, g.valid) for g in again] == [(g.neg, g.zero, g.pos, g.valid) for g in got]
         for (p, a, b), g in zip(items, got):
             want = _single(F, p, a, b)
             assert (g.neg, g.zero, g.pos, g.valid, g.two_by_two) == \
