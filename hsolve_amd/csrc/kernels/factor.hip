@@ -582,6 +582,38 @@ __device__ __forceinline__ double wave_max_abs(double v) {
   return __longlong_as_double((long long)wave_max_key((unsigned long long)__double_as_longlong(fabs(v))));
 }
 
+// Maximum of a non-negative v over the 64 lanes and the lowest lane holding it (arg), returned to
+// every lane, exactly as wave_max_key + ballot would give, in about a third of the time: the DPP
+// reduction runs on the high 32 bits only (a 32-bit max per step, folded into the DPP move), the
+// lowest lane whose high word is the maximum is read in full, and a ballot checks that no lane
+// with the same high word holds a larger value; only then (equal high words, different low
+// words) the full 64-bit reduction runs.
+__device__ __forceinline__ double wave_argmax_nonneg(double v, int& arg) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned hi = (unsigned)(bits >> 32);
+  unsigned m = hi;
+#define HSG_DPP_UMAX_STEP(ctrl, rmask) \
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, ctrl, rmask, 0xf, true));
+  HSG_DPP_UMAX_STEP(0x111, 0xf)  // row_shr:1
+  HSG_DPP_UMAX_STEP(0x112, 0xf)  // row_shr:2
+  HSG_DPP_UMAX_STEP(0x114, 0xf)  // row_shr:4
+  HSG_DPP_UMAX_STEP(0x118, 0xf)  // row_shr:8
+  HSG_DPP_UMAX_STEP(0x142, 0xa)  // row_bcast:15 into rows 1, 3
+  HSG_DPP_UMAX_STEP(0x143, 0xc)  // row_bcast:31 into rows 2, 3
+#undef HSG_DPP_UMAX_STEP
+  const unsigned mh = (unsigned)__builtin_amdgcn_readlane((int)m, 63);
+  const unsigned long long top = __ballot(hi == mh);
+  int l0 = __ffsll((long long)top) - 1;
+  const unsigned lo0 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)bits, l0);
+  unsigned long long best = ((unsigned long long)mh << 32) | lo0;
+  if (__ballot(hi == mh && bits > best)) {
+    best = wave_max_key(hi == mh ? bits : 0ull);
+    l0 = __ffsll((long long)__ballot(bits == best)) - 1;
+  }
+  arg = l0;
+  return __longlong_as_double((long long)best);
+}
+
 // Value of v in lane src (src uniform across the wave).
 __device__ __forceinline__ double readlane_of(double v, int src) {
   const long long bits = __double_as_longlong(v);
@@ -595,16 +627,18 @@ __device__ __forceinline__ float readlane_of(float v, int src) {
 
 // Bunch-Kaufman on the nb x nb diagonal block of panel p of each front, eight 64-lane waves per
 // front: lane i of every wave stands for row i of the block (LDS, padded leading dimension).
-// Every wave reads the pivot column into a register and runs the same pivot search on it (DPP
-// wave reductions; argmax = lowest lane holding the maximum, by ballot), so all reach the same
-// decision without a barrier. Wave w then updates the columns [8 w, 8 w + 8) of the trailing
+// Every wave reads the pivot column into a register and runs the same pivot search on it
+// (wave_argmax_nonneg: maximum and lowest lane holding it), so all reach the same decision
+// without a barrier; the column maxima of the null-pivot test are formed only when a pivot is
+// small enough for the test to need them. Wave w then updates the columns [8 w, 8 w + 8) of the trailing
 // triangle with the step's multipliers formed from the pivot column (broadcast LDS reads), and
 // every lane stores all of its columns (the new value where it changes, the old one elsewhere),
 // so the update has no per-lane branches; a wave whose columns all lie left of the pivot skips
 // it. The pivot columns of F keep their unscaled values; their multipliers go to a separate
 // array Lm, written by wave 0 in the same step (nobody reads Lm during a step, and a stored-back
 // old value of F is never stale). Wave 0 alone swaps rows and columns (Lm for the columns
-// already pivoted, F for the rest), keeps the counts and the pivot data. One barrier per step
+// already pivoted, F for the rest); every lane keeps the same tallies, and lane i the pivot data
+// of position i. One barrier per step
 // (four when rows are swapped). Eight waves measured faster than four (shorter update per wave,
 // and waves drop out sooner) and than sixteen (the redundant searches crowd the SIMDs).
 constexpr int kBkLd = 65;
@@ -619,7 +653,6 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   __shared__ T F[64 * kBkLd];   // the block; pivot columns keep their unscaled values (and D)
   __shared__ T Lm[64 * kBkLd];  // multipliers of the pivot columns (strictly lower L11)
   __shared__ int perm[64];
-  __shared__ int8_t pairc[64];  // 1: column j starts a 2x2 step, F(j + 1, j) belongs to D
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -640,14 +673,12 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       F[lane + j * kBkLd] = (lane >= j && lane < w) ? v : (T)0;
     }
   }
-  if (lead) {
-    perm[lane] = lane;
-    pairc[lane] = 0;
-  }
+  if (lead) perm[lane] = lane;
   __syncthreads();
   const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
   const bool det = detect != 0;
-  int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // tallies (wave 0, lane 0)
+  int neg = 0, zero = 0, pos = 0, two = 0, exz = 0;  // tallies (the same in every lane)
+  unsigned long long pairs = 0;                      // bit j: column j starts a 2x2 step
   int8_t my_pt = kPiv1;
   T my_dv = (T)0, my_dof = (T)0;
   int k = 0;
@@ -656,9 +687,9 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     const T fk = F[lane + k * kBkLd];
     const double absakk = fabs((double)readlane_of(fk, k));
     const double v = (lane > k && lane < w) ? fabs((double)fk) : 0.0;
-    const double colmax = wave_max_abs(v);
-    const unsigned long long at_max = __ballot(lane > k && lane < w && v == colmax);
-    const int imax = (colmax > 0.0 && at_max) ? __ffsll((long long)at_max) - 1 : -1;
+    int imax;
+    const double colmax = wave_argmax_nonneg(v, imax);
+    if (!(colmax > 0.0)) imax = -1;
     int kind = 1, kp = k;
     // the row search reads entries of row imax that this step's update changes: when it runs and no
     // interchange follows, a barrier keeps every wave's reads ahead of the first update store (the
@@ -667,7 +698,8 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     if (row_search) {
       const T ra = F[imax + lane * kBkLd], rb = F[lane + imax * kBkLd];
       const double r = (lane >= k && lane < w && lane != imax) ? fabs((double)(lane < imax ? ra : rb)) : 0.0;
-      const double rowmax = wave_max_abs(r);
+      int rarg;
+      const double rowmax = wave_argmax_nonneg(r, rarg);
       if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
         kind = 1;
         kp = k;
@@ -716,23 +748,25 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     const T c1 = swapped ? F[lane + k * kBkLd] : fk;
     const T c2 = kind == 2 ? F[lane + (k + 1) * kBkLd] : (T)0;
     // ---- remaining column maxima for the null-pivot test (rows of the block below the pivot,
-    //      then lazily the panel rows below the block)
+    //      then the panel rows below the block), formed only when a pivot is small enough for the
+    //      test to need them (the conditions are uniform: every lane reaches the reductions)
     const int s0 = kind == 1 ? k + 1 : k + 2;
-    double m01 = colmax;  // kind 1 without a swap: the same column below the pivot
-    if (kind == 2 || swapped) {
+    auto max_below = [&]() -> double {
+      if (kind == 1 && !swapped) return colmax;  // the same column below the pivot
+      int unused;
       double mv = 0.0;
       if (lane >= s0 && lane < w) {
         mv = fabs((double)c1);
         if (kind == 2) mv = fmax(mv, fabs((double)c2));
       }
-      m01 = wave_max_abs(mv);
-    }
+      return wave_argmax_nonneg(mv, unused);
+    };
     const bool own = lead;  // wave 0 writes the multipliers of this step
     T* row = F + lane;
     if (kind == 1) {
       const double d = (double)readlane_of(c1, k);
       bool null_piv = d == 0.0;
-      if (det && fabs(d) <= tau && m01 <= tau) {
+      if (det && fabs(d) <= tau && max_below() <= tau) {
         double mb = 0.0;
         const int jc = perm[k];
         for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs((double)P[(int64_t)(c0 + jc) * m + r2]));
@@ -740,17 +774,14 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
         if (mb <= tau) null_piv = true;
       }
       const T inv = null_piv ? (T)0 : (T)(1.0 / d);
-      if (lead && lane == 0) {
-        if (d == 0.0) ++exz;
-        if (null_piv) ++zero;
-        else if (d < 0) ++neg;
-        else ++pos;
-      }
-      if (lane == k) {
-        my_pt = null_piv ? kPivNull : kPiv1;
-        my_dv = (T)d;
-        my_dof = (T)0;
-      }
+      // tallies and pivot data without lane branches (the tallies are the same in every lane)
+      exz += d == 0.0;
+      zero += null_piv;
+      neg += !null_piv && d < 0;
+      pos += !null_piv && !(d < 0);
+      my_pt = lane == k ? (null_piv ? kPivNull : kPiv1) : my_pt;
+      my_dv = lane == k ? (T)d : my_dv;
+      my_dof = lane == k ? (T)0 : my_dof;
       const bool below = lane > k && lane < w;
       const T wi = below ? c1 : (T)0;
       const T li = wi * inv;  // multiplier of row `lane`, zero outside the rows below the pivot
@@ -766,7 +797,7 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
 #pragma unroll
         for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - li * readlane_of(wi, jb + q);
       }
-      if (own && below) Lm[lane + k * kBkLd] = li;
+      if (own) Lm[lane + k * kBkLd] = li;  // zero above the rows below the pivot: never read
       __syncthreads();
       k += 1;
     } else {
@@ -775,7 +806,10 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       const double dt = a * c - b * b;
       const double mean = 0.5 * (a + c), rad = sqrt(0.25 * (a - c) * (a - c) + b * b);
       const double e1 = mean - rad, e2 = mean + rad;
-      const bool coupled = m01 > tau;
+      // coupled: the pair's columns have entries above tau below the pair (then a small
+      // eigenvalue of the pair is not a null pivot)
+      const bool maybe_null = det && (fabs(e1) <= tau || fabs(e2) <= tau);
+      const bool coupled = maybe_null && max_below() > tau;
       int nz = 0, nn = 0, np = 0;
       for (double e : {e1, e2}) {
         const bool z = (det && fabs(e) <= tau && !coupled) || e == 0.0;
@@ -785,24 +819,15 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
       }
       const bool dead = nz > 0 || dt == 0.0;
       const T i11 = dead ? (T)0 : (T)(c / dt), i12 = dead ? (T)0 : (T)(-b / dt), i22 = dead ? (T)0 : (T)(a / dt);
-      if (lead && lane == 0) {
-        if (dt == 0.0) ++exz;
-        zero += nz;
-        neg += nn;
-        pos += np;
-        ++two;
-        pairc[k] = 1;
-      }
-      if (lane == k) {
-        my_pt = dead ? kPivNull : kPiv2First;
-        my_dv = (T)a;
-        my_dof = (T)b;
-      }
-      if (lane == k + 1) {
-        my_pt = dead ? kPivNull : kPiv2Second;
-        my_dv = (T)c;
-        my_dof = (T)0;
-      }
+      exz += dt == 0.0;
+      zero += nz;
+      neg += nn;
+      pos += np;
+      ++two;
+      pairs |= 1ull << k;
+      my_pt = lane == k ? (dead ? kPivNull : kPiv2First) : (lane == k + 1 ? (dead ? kPivNull : kPiv2Second) : my_pt);
+      my_dv = lane == k ? (T)a : (lane == k + 1 ? (T)c : my_dv);
+      my_dof = lane == k ? (T)b : (lane == k + 1 ? (T)0 : my_dof);
       const bool below = lane > k + 1 && lane < w;
       const T w1 = below ? c1 : (T)0;
       const T w2 = below ? c2 : (T)0;
@@ -825,7 +850,7 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
 #pragma unroll
         for (int q = 0; q < kBkCols; ++q) row[(jb + q) * kBkLd] = old[q] - (w1 * l1v[q] + w2 * l2v[q]);
       }
-      if (own && below) {
+      if (own) {  // zero above the rows below the pair: never read
         Lm[lane + k * kBkLd] = w1 * i11 + w2 * i12;
         Lm[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
       }
@@ -838,7 +863,7 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
   for (int j = wv; j < w; j += kBkWaves)
     if (lane >= j && lane < w)
       P[(int64_t)(c0 + j) * m + c0 + lane] =
-          (lane == j || (lane == j + 1 && pairc[j])) ? F[lane + j * kBkLd] : Lm[lane + j * kBkLd];
+          (lane == j || (lane == j + 1 && ((pairs >> j) & 1))) ? F[lane + j * kBkLd] : Lm[lane + j * kBkLd];
   if (lead && lane < w) {
     N.dval[first + c0 + lane] = my_dv;
     N.doff[first + c0 + lane] = my_dof;
