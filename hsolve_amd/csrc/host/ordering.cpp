@@ -382,17 +382,18 @@ int32_t bfs_farthest(const WGraph& g, int32_t s) {
 
 // Multilevel 2-way edge partition of g; part[v] in {0,1}.
 void multilevel_bisect(const WGraph& g0, std::vector<int8_t>& part, Rng& rng) {
-  std::vector<WGraph> levels;
+  // coarse graphs by level; level 0 is g0 itself (not copied)
+  std::vector<WGraph> coarse;
   std::vector<std::vector<int32_t>> maps;
-  levels.push_back(g0);
-  while (levels.back().n > 160) {
+  auto level = [&](size_t l) -> const WGraph& { return l == 0 ? g0 : coarse[l - 1]; };
+  while (level(coarse.size()).n > 160) {
     WGraph c;
     std::vector<int32_t> cmap;
-    if (!coarsen(levels.back(), c, cmap, rng)) break;
-    levels.push_back(std::move(c));
+    if (!coarsen(level(coarse.size()), c, cmap, rng)) break;
+    coarse.push_back(std::move(c));
     maps.push_back(std::move(cmap));
   }
-  const WGraph& gc = levels.back();
+  const WGraph& gc = level(coarse.size());
   // initial partitions: grown from a few seeds (peripheral and random), keep the best cut
   std::vector<int8_t> best;
   int64_t best_cut = -1;
@@ -411,8 +412,8 @@ void multilevel_bisect(const WGraph& g0, std::vector<int8_t>& part, Rng& rng) {
     }
   }
   part = std::move(best);
-  for (size_t l = levels.size() - 1; l > 0; --l) {
-    const WGraph& fine = levels[l - 1];
+  for (size_t l = coarse.size(); l > 0; --l) {
+    const WGraph& fine = level(l - 1);
     const std::vector<int32_t>& cmap = maps[l - 1];
     std::vector<int8_t> fp(fine.n);
     for (int32_t v = 0; v < fine.n; ++v) fp[v] = part[cmap[v]];
@@ -518,18 +519,35 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
       for (int32_t a = 0; a < nv; ++a) perm[t.lo + a] = ord[a];
       return;
     }
-    // local weighted graph of the subgraph
+    // local weighted graph of the subgraph (large ones: degrees, then rows, over vertex ranges on
+    // several threads; the result is the same)
     for (int32_t a = 0; a < nv; ++a) local[t.verts[a]] = a;
     WGraph wg;
     wg.n = nv;
     wg.ptr.assign(static_cast<size_t>(nv) + 1, 0);
-    for (int32_t a = 0; a < nv; ++a) {
-      const int32_t v = t.verts[a];
-      for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
-        const int32_t u = g.adj[p];
-        if (label[u].load(std::memory_order_relaxed) == t.id) wg.adj.push_back(local[u]);
-      }
-      wg.ptr[a + 1] = static_cast<int64_t>(wg.adj.size());
+    {
+      auto inside = [&](int32_t u) { return label[u].load(std::memory_order_relaxed) == t.id; };
+      const int nt = worker_count(nv, 32768);
+      parallel_ranges(nv, nt, [&](int, int32_t a0, int32_t a1) {
+        for (int32_t a = a0; a < a1; ++a) {
+          const int32_t v = t.verts[a];
+          int64_t d = 0;
+          for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) d += inside(g.adj[p]);
+          wg.ptr[a + 1] = d;
+        }
+      });
+      for (int32_t a = 0; a < nv; ++a) wg.ptr[a + 1] += wg.ptr[a];
+      wg.adj.resize(static_cast<size_t>(wg.ptr[nv]));
+      parallel_ranges(nv, nt, [&](int, int32_t a0, int32_t a1) {
+        for (int32_t a = a0; a < a1; ++a) {
+          const int32_t v = t.verts[a];
+          int64_t q = wg.ptr[a];
+          for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
+            const int32_t u = g.adj[p];
+            if (inside(u)) wg.adj[q++] = local[u];
+          }
+        }
+      });
     }
     wg.ew.assign(wg.adj.size(), 1);
     wg.vw.assign(nv, 1);

@@ -22,26 +22,45 @@ namespace hsg {
 
 namespace {
 
-// Lower-triangular pattern (rows > col) of P A P^T in CSC form, new numbering.
+// body(i0, i1) over nt contiguous ranges of [0, n) on as many threads (range 0 on the caller).
+template <class F>
+void over_ranges(int32_t n, F&& body) {
+  const int nt = n < 32768 ? 1 : std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+  auto range = [&](int t) {
+    body(static_cast<int32_t>(int64_t{n} * t / nt), static_cast<int32_t>(int64_t{n} * (t + 1) / nt));
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back([&, t] { range(t); });
+  range(0);
+  for (auto& th : pool) th.join();
+}
+
+// Lower-triangular pattern (rows > col) of P A P^T in CSC form, new numbering. Columns are
+// independent: counted, then filled and sorted over column ranges on several threads.
 void permuted_lower(int32_t n, const Graph& g, const std::vector<int32_t>& order,
                     const std::vector<int32_t>& iorder, std::vector<int64_t>& cp, std::vector<int32_t>& ri) {
   cp.assign(static_cast<size_t>(n) + 1, 0);
-  for (int32_t jn = 0; jn < n; ++jn) {
-    const int32_t j = order[jn];
-    for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p)
-      if (iorder[g.adj[p]] > jn) ++cp[jn + 1];
-  }
+  over_ranges(n, [&](int32_t j0, int32_t j1) {
+    for (int32_t jn = j0; jn < j1; ++jn) {
+      const int32_t j = order[jn];
+      int64_t c = 0;
+      for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p) c += iorder[g.adj[p]] > jn;
+      cp[jn + 1] = c;
+    }
+  });
   for (int32_t j = 0; j < n; ++j) cp[j + 1] += cp[j];
   ri.resize(static_cast<size_t>(cp[n]));
-  std::vector<int64_t> fill(cp.begin(), cp.end() - 1);
-  for (int32_t jn = 0; jn < n; ++jn) {
-    const int32_t j = order[jn];
-    for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p) {
-      const int32_t in = iorder[g.adj[p]];
-      if (in > jn) ri[fill[jn]++] = in;
+  over_ranges(n, [&](int32_t j0, int32_t j1) {
+    for (int32_t jn = j0; jn < j1; ++jn) {
+      const int32_t j = order[jn];
+      int64_t q = cp[jn];
+      for (int64_t p = g.ptr[j]; p < g.ptr[j + 1]; ++p) {
+        const int32_t in = iorder[g.adj[p]];
+        if (in > jn) ri[q++] = in;
+      }
+      std::sort(ri.begin() + cp[jn], ri.begin() + cp[jn + 1]);
     }
-    std::sort(ri.begin() + cp[jn], ri.begin() + cp[jn + 1]);
-  }
+  });
 }
 
 // Elimination tree of the pattern in the new numbering (Liu's algorithm with path halving): row
@@ -356,21 +375,34 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
     for (int32_t s = 0; s < S.nsn; ++s) std::copy(rl[s].begin(), rl[s].end(), S.rows.begin() + S.sn_rowoff[s]);
   }
   // extend-add maps: local index of each child CB row in the parent's front
+  // (fronts are independent: over ranges of fronts on several threads, errors flagged per range)
   S.relmap.assign(S.rows.size(), -1);
-  for (int32_t c = 0; c < S.nsn; ++c) {
-    const int32_t p = S.sn_parent[c];
-    if (p < 0) {
-      if (S.sn_m[c] != S.sn_ncol[c]) throw std::runtime_error("analyze_pattern: root with contribution rows");
-      continue;
-    }
-    const int32_t* prow = &S.rows[S.sn_rowoff[p]];
-    const int32_t pm = S.sn_m[p];
-    int32_t q = 0;
-    for (int64_t k = S.sn_rowoff[c] + S.sn_ncol[c]; k < S.sn_rowoff[c + 1]; ++k) {
-      const int32_t i = S.rows[k];
-      while (q < pm && prow[q] < i) ++q;
-      if (q >= pm || prow[q] != i) throw std::runtime_error("analyze_pattern: child row missing in parent");
-      S.relmap[k] = q;
+  {
+    std::vector<char> bad_root(S.nsn, 0), bad_row(S.nsn, 0);
+    over_ranges(S.nsn, [&](int32_t c0, int32_t c1) {
+      for (int32_t c = c0; c < c1; ++c) {
+        const int32_t p = S.sn_parent[c];
+        if (p < 0) {
+          if (S.sn_m[c] != S.sn_ncol[c]) bad_root[c] = 1;
+          continue;
+        }
+        const int32_t* prow = &S.rows[S.sn_rowoff[p]];
+        const int32_t pm = S.sn_m[p];
+        int32_t q = 0;
+        for (int64_t k = S.sn_rowoff[c] + S.sn_ncol[c]; k < S.sn_rowoff[c + 1]; ++k) {
+          const int32_t i = S.rows[k];
+          while (q < pm && prow[q] < i) ++q;
+          if (q >= pm || prow[q] != i) {
+            bad_row[c] = 1;
+            break;
+          }
+          S.relmap[k] = q;
+        }
+      }
+    });
+    for (int32_t c = 0; c < S.nsn; ++c) {
+      if (bad_root[c]) throw std::runtime_error("analyze_pattern: root with contribution rows");
+      if (bad_row[c]) throw std::runtime_error("analyze_pattern: child row missing in parent");
     }
   }
   // offsets, statistics

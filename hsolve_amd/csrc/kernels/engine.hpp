@@ -99,6 +99,7 @@ class Engine {
     int large_max_cb = 0;
     int max_m = 0, max_ncol = 0;  // over all fronts of the level (solve launches)
     int fwd_tiles = 0, fwd_chunks = 0;   // forward split-GEMV grid
+    int fwd_split = 0;                   // columns per chunk of the forward products
     int64_t bt_begin = 0, bt_count = 0;  // backward tasks of the level
     int64_t bh_begin = 0, bh_count = 0;  // its split column tiles (their chunk-0 tasks)
     std::vector<double> upd_flops;       // algorithmic flops of the trailing update after panel p

@@ -1478,7 +1478,14 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
     L.n_small = (int)L.h_small.size();
     L.n_large = (int)L.h_large.size();
     L.fwd_tiles = (L.max_m + kGvTile - 1) / kGvTile;
-    L.fwd_chunks = (L.max_ncol + kGvSplit - 1) / kGvSplit;
+    // columns per chunk of the split products: halved from kGvSplit (down to 64) while the level's
+    // grid stays below ~6 workgroups per CU, so that the few large fronts of the top levels spread
+    // their G reads over the whole device instead of one workgroup per CU
+    L.fwd_split = kGvSplit;
+    while (L.fwd_split > 64 &&
+           (int64_t)L.fwd_tiles * L.n_all * ((L.max_ncol + L.fwd_split - 1) / L.fwd_split) < kFwdTargetGroups)
+      L.fwd_split /= 2;
+    L.fwd_chunks = (L.max_ncol + L.fwd_split - 1) / L.fwd_split;
     // scratch of the split products (slots and counters per group of right-hand sides)
     if (L.fwd_chunks > 1) part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks);
     n_small_ += L.n_small;
@@ -1986,11 +1993,11 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   const int32_t* cidx = (const int32_t*)d_cidx_;
   for (const Level& L : levels_) {
     hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
-                       (const T*)Xp, Y, U, part, (const int64_t*)nm.d_zpull_, cptr, cidx);
+                       L.fwd_split, (const T*)Xp, Y, U, part, (const int64_t*)nm.d_zpull_, cptr, cidx);
     ++nl;
     if (L.fwd_chunks > 1) {
-      hipLaunchKernelGGL(k_fwd_reduce<T>, dim3(L.fwd_tiles, L.n_all), dim3(256), 0, st, S, L.d_all, p, L.fwd_chunks, Y,
-                         U, (const T*)part, cptr, cidx);
+      hipLaunchKernelGGL(k_fwd_reduce<T>, dim3(L.fwd_tiles, L.n_all), dim3(256), 0, st, S, L.d_all, p, L.fwd_split,
+                         L.fwd_chunks, Y, U, (const T*)part, cptr, cidx);
       ++nl;
     }
   }

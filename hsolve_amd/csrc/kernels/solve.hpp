@@ -282,7 +282,8 @@ __global__ void k_permute_out(int n, int p, const T* __restrict__ Xp, const int3
 // chunks' partials in chunk order and writes the result: the sum does not depend on the order
 // the workgroups ran in, and stream order makes the partials visible (no cross-workgroup
 // synchronization inside a launch). Slot index: (slot of chunk) * rhs_groups + rhs_group.
-constexpr int kGvSplit = 256;
+constexpr int kGvSplit = 256;          // columns per chunk of a split forward product, at most
+constexpr int kFwdTargetGroups = 1536;  // workgroups a forward level's grid should reach (6 per CU)
 constexpr int kPartSlot = kGvTile * kMaxRhsChunk;  // values per partial slot
 
 // src[0] + src[stride] + ... (n terms), added in that order; the loads are issued eight at a time.
@@ -304,12 +305,13 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ src, int64_t str
 // < ncol (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution
 // rows get the children's update rows plus G21 z1 in U. z1 (right-hand side plus the children's
 // update rows, gathered while staging) goes through LDS, G is read along columns. 256 threads =
-// 64 rows x 4 column groups. grid (row tiles, fronts, column chunks).
+// 64 rows x 4 column groups. grid (row tiles, fronts, column chunks of `split` columns).
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict__ G, const int32_t* __restrict__ list,
-                                                   int p, const T* __restrict__ Xp, T* __restrict__ Y, T* __restrict__ U,
-                                                   T* __restrict__ part, const int64_t* __restrict__ zpull,
-                                                   const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
+                                                   int p, int split, const T* __restrict__ Xp, T* __restrict__ Y,
+                                                   T* __restrict__ U, T* __restrict__ part,
+                                                   const int64_t* __restrict__ zpull, const int64_t* __restrict__ cptr,
+                                                   const int32_t* __restrict__ cidx) {
   __shared__ T zs[kGvChunk][kMaxRhsChunk];
   __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
   const int f = blockIdx.y;
@@ -319,9 +321,9 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
   const int r0 = tile * kGvTile;
   if (r0 >= m) return;
   const int kend = min(ncol, r0 + kGvTile);
-  const int nchunk = (kend + kGvSplit - 1) / kGvSplit;
+  const int nchunk = (kend + split - 1) / split;
   if (chunk >= nchunk) return;
-  const int kbeg = chunk * kGvSplit, kstop = min(kend, kbeg + kGvSplit);
+  const int kbeg = chunk * split, kstop = min(kend, kbeg + split);
   const T* Gs = G + S.panel_off[s];
   const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
   const int i = r0 + lane;
@@ -371,7 +373,7 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict_
 // chunk, the chunks' partials added in chunk order, written like k_fwd_gemv's result.
 // grid (row tiles, fronts), chunks = the forward launch's grid.z.
 template <class T>
-__global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __restrict__ list, int p, int chunks,
+__global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __restrict__ list, int p, int split, int chunks,
                                                      T* __restrict__ Y, T* __restrict__ U, const T* __restrict__ part,
                                                      const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
   const int f = blockIdx.y;
@@ -381,7 +383,7 @@ __global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __r
   const int r0 = tile * kGvTile;
   if (r0 >= m) return;
   const int kend = min(ncol, r0 + kGvTile);
-  const int nchunk = (kend + kGvSplit - 1) / kGvSplit;
+  const int nchunk = (kend + split - 1) / split;
   if (nchunk <= 1) return;
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int idx = threadIdx.x; idx < kGvTile * p; idx += blockDim.x) {
