@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import math
 import resource
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -178,9 +179,32 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     rep["sigma_requested"] = cfg["target"]["sigma"]
     if cfg["problem"] == "generalized_indefinite":
         raise UnsupportedError("indefinite B (GHIEP) is not supported by this engine")
-    ops = DeviceOps(pencil)
-    n2a = estimate_norm2(ops, ops.apply_a, 30, cfg["seed"])
-    n2b = 1.0 if B is None else estimate_norm2(ops, ops.apply_b, 30, cfg["seed"])
+    # the symbolic analysis (host only, the library releases the GIL) runs on a thread of its own
+    # while the device estimates the norms; the factor phase then waits for it
+    analysis = {}
+
+    def analyze():
+        try:
+            plan = autotune.options_for(pencil)
+            analysis["plan"] = plan
+            analysis["fac"] = GpuFactorization(pencil, **plan)
+        except BaseException as e:  # re-raised by the solve
+            analysis["error"] = e
+
+    worker = None
+    if fac is None:
+        worker = threading.Thread(target=analyze, name="hsolve-analysis")
+        worker.start()
+    try:
+        ops = DeviceOps(pencil)
+        n2a = estimate_norm2(ops, ops.apply_a, 30, cfg["seed"])
+        n2b = 1.0 if B is None else estimate_norm2(ops, ops.apply_b, 30, cfg["seed"])
+    except BaseException:
+        if worker is not None:  # no factorization object outlives a failed solve
+            worker.join()
+            if "fac" in analysis:
+                analysis["fac"].close()
+        raise
     rep["norm2_a"], rep["norm2_b"] = n2a, n2b
     scale = n2a / n2b if n2b > 0 else n2a
     rep["scale"] = scale
@@ -190,9 +214,11 @@ def _solve(A, B, config, prepared, vectors, streams, fac: GpuFactorization | Non
     clock.pop()
     own_fac = fac is None
     with clock.phase("factor"):
-        if fac is None:
-            plan = autotune.options_for(pencil)
-            fac = GpuFactorization(pencil, **plan)
+        if worker is not None:
+            worker.join()
+            if "error" in analysis:
+                raise analysis["error"]
+            fac, plan = analysis["fac"], analysis["plan"]
             if plan:
                 rep["parallel"]["notes"].append(f"launch plan from the autotuner: {plan}")
         elif fac.pencil is not pencil:
