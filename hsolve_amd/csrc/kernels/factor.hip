@@ -1478,11 +1478,13 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
     L.n_small = (int)L.h_small.size();
     L.n_large = (int)L.h_large.size();
     L.fwd_tiles = (L.max_m + kGvTile - 1) / kGvTile;
-    // columns per chunk of the split products: halved from kGvSplit (down to 64) while the level's
-    // grid stays below ~6 workgroups per CU, so that the few large fronts of the top levels spread
-    // their G reads over the whole device instead of one workgroup per CU
+    // columns per chunk of the split products: on levels with fronts of more than 2 kGvSplit
+    // pivots, halved from kGvSplit (down to 64) while the level's grid stays below ~6 workgroups
+    // per CU, so that the few large fronts of the top levels spread their G reads over the whole
+    // device; smaller fronts keep kGvSplit (a split costs a second launch, which small problems,
+    // bound by launches, cannot afford)
     L.fwd_split = kGvSplit;
-    while (L.fwd_split > 64 &&
+    while (L.max_ncol > 2 * kGvSplit && L.fwd_split > 64 &&
            (int64_t)L.fwd_tiles * L.n_all * ((L.max_ncol + L.fwd_split - 1) / L.fwd_split) < kFwdTargetGroups)
       L.fwd_split /= 2;
     L.fwd_chunks = (L.max_ncol + L.fwd_split - 1) / L.fwd_split;
