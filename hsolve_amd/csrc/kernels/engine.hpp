@@ -51,7 +51,7 @@ struct Numeric {
   int64_t ws_cols_ = 0;
   void *d_G_ = nullptr, *d_Gt_ = nullptr, *d_T_ = nullptr;  // inverse panels (solve phase)
   int32_t *d_pos_ = nullptr, *d_rpos_ = nullptr;  // pivot position of each original row / front row
-  int64_t* d_zpull_ = nullptr;  // per pivot position: the range of its row's children update rows in cidx
+  PullRow* d_zpull_ = nullptr;  // per pivot position: its row's children update rows (the pull map entry)
   void* d_part_ = nullptr;       // partial sums of split product tiles
   bool g_ready_ = false;
 };
@@ -100,8 +100,13 @@ class Engine {
     int max_m = 0, max_ncol = 0;  // over all fronts of the level (solve launches)
     int fwd_tiles = 0, fwd_chunks = 0;   // forward split-GEMV grid
     int fwd_split = 0;                   // columns per chunk of the forward products
+    int64_t ft_begin = 0, ft_count = 0;  // forward tasks of the level
+    int64_t fh_begin = 0, fh_count = 0;  // its split row tiles (their chunk-0 tasks)
     int64_t bt_begin = 0, bt_count = 0;  // backward tasks of the level
     int64_t bh_begin = 0, bh_count = 0;  // its split column tiles (their chunk-0 tasks)
+    int ft_cols = kFwdCols;              // G entries per thread and pass of the forward tasks (2 ... 32)
+    int bt_rows = 32;                    // chunk rows per row group of the backward tasks (16 or 32)
+    int bt_regs = 32;                    // rows per thread of the backward tasks at most (4 ... 32)
     std::vector<double> upd_flops;       // algorithmic flops of the trailing update after panel p
     double cb_flops = 0.0;               // and of the contribution-block product
   };
@@ -128,16 +133,20 @@ class Engine {
   // and the pull map of the forward products (per front row, the children's U rows mapped onto it)
   std::vector<int32_t> h_pairs_, h_ttiles_;
   std::vector<GemmTask> h_tasks_;
+  std::vector<FwdTask> h_ftasks_, h_fheads_;
+  FwdTask *d_ftasks_ = nullptr, *d_fheads_ = nullptr;
   std::vector<BwdTask> h_btasks_, h_bheads_;
   BwdTask *d_btasks_ = nullptr, *d_bheads_ = nullptr;
   std::vector<TaskRange> gstages_;
   std::vector<int64_t> h_cptr_;
   std::vector<int32_t> h_cidx_;
+  std::vector<PullRow> h_cpull_;  // per front row: first two children's rows inline, the rest in cidx
   int64_t t_total_ = 0;
   double g_flops_ = 0.0;  // algorithmic flops of the inverse panels: sum of cb ncol^2 + ncol^3 / 3
   int n_pairs_ = 0, n_ttiles_ = 0;
   int32_t *d_pairs_ = nullptr, *d_ttiles_ = nullptr, *d_cidx_ = nullptr;
   int64_t* d_cptr_ = nullptr;
+  PullRow* d_cpull_ = nullptr;
   GemmTask* d_tasks_ = nullptr;
   bool on_device_ = false;
   int64_t nnz_ = 0;

@@ -1488,8 +1488,6 @@ Engine::Engine(Symbolic sym, std::vector<int64_t> asm_dst, std::vector<int64_t> 
            (int64_t)L.fwd_tiles * L.n_all * ((L.max_ncol + L.fwd_split - 1) / L.fwd_split) < kFwdTargetGroups)
       L.fwd_split /= 2;
     L.fwd_chunks = (L.max_ncol + L.fwd_split - 1) / L.fwd_split;
-    // scratch of the split products (slots and counters per group of right-hand sides)
-    if (L.fwd_chunks > 1) part_slots_ = std::max<int64_t>(part_slots_, (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks);
     n_small_ += L.n_small;
     n_large_ += L.n_large;
     max_large_per_level_ = std::max(max_large_per_level_, L.n_large);
@@ -1587,11 +1585,52 @@ void Engine::plan_solve() {
       }
   }
   n_ttiles_ = (int)(h_ttiles_.size() / 3);
+  // forward tasks per level: 64-row tiles of every front, each tile's columns (up to the tile's
+  // end inside G11) in chunks of the level's split width; split tiles store partial sums in slots
+  // numbered per level
+  for (Level& L : levels_) {
+    L.ft_begin = (int64_t)h_ftasks_.size();
+    L.fh_begin = (int64_t)h_fheads_.size();
+    int64_t slots = 0;
+    for (int s : L.h_all) {
+      const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
+      for (int r0 = 0; r0 < m; r0 += kGvTile) {
+        const int kend = std::min(ncol, r0 + kGvTile);
+        const int nchunk = (kend + L.fwd_split - 1) / L.fwd_split;
+        FwdTask t{};
+        t.m = m;
+        t.ncol = ncol;
+        t.first = sym_.sn_first[s];
+        t.r0 = r0;
+        t.nchunk = nchunk;
+        t.goff = sym_.panel_off[s];
+        t.rowoff = sym_.sn_rowoff[s];
+        for (int c = 0; c < nchunk; ++c) {
+          t.kbeg = c * L.fwd_split;
+          t.kstop = std::min(kend, (c + 1) * L.fwd_split);
+          t.slot = nchunk > 1 ? slots + c : -1;
+          h_ftasks_.push_back(t);
+          if (c == 0 && nchunk > 1) h_fheads_.push_back(t);
+        }
+        if (nchunk > 1) slots += nchunk;
+      }
+    }
+    L.ft_count = (int64_t)h_ftasks_.size() - L.ft_begin;
+    L.fh_count = (int64_t)h_fheads_.size() - L.fh_begin;
+    // the launch's register tile: columns per thread and pass, rounded up to a power of two
+    int cols = 1;
+    for (int64_t k = L.ft_begin; k < L.ft_begin + L.ft_count; ++k)
+      cols = std::max(cols, (std::min(kGvChunk, h_ftasks_[k].kstop - h_ftasks_[k].kbeg) + 3) / 4);
+    L.ft_cols = 2;
+    while (L.ft_cols < cols) L.ft_cols *= 2;
+    part_slots_ = std::max(part_slots_, slots);
+  }
   // backward tasks per level: column tiles of at most 64 (width rounded up to a power of two >= 8:
   // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group,
   // or of 16 where the level has split tiles already and fewer than 1024 tasks (a task's time is
   // mostly its chain of dependent loads: shorter chunks shorten it, at the cost of more partial
-  // sums; a level without splits would also gain a reduction launch)
+  // sums; a level without splits would also gain a reduction launch); a chunk has at most
+  // kBwdRows rows (the staging area)
   auto tile_width = [](int jn) {
     int jw = 8;
     while (jw < jn) jw *= 2;
@@ -1603,18 +1642,18 @@ void Engine::plan_solve() {
     int64_t slots = 0, tasks32 = 0, splits32 = 0;
     for (int s : L.h_all)
       for (int j0 = 0; j0 < sym_.sn_ncol[s]; j0 += kGvTile) {
-        const int rc = 32 * (256 / tile_width(std::min(kGvTile, sym_.sn_ncol[s] - j0)));
+        const int rc = std::min(kBwdRows, 32 * (256 / tile_width(std::min(kGvTile, sym_.sn_ncol[s] - j0))));
         const int64_t nch = (sym_.sn_m[s] - j0 + rc - 1) / rc;
         tasks32 += nch;
         splits32 += nch > 1;
       }
-    const int rows_per_group = splits32 > 0 && tasks32 < 1024 ? 16 : 32;
+    L.bt_rows = splits32 > 0 && tasks32 < 1024 ? 16 : 32;
     for (int s : L.h_all) {
       const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
       for (int j0 = 0; j0 < ncol; j0 += kGvTile) {
         const int jn = std::min(kGvTile, ncol - j0);
         const int jw = tile_width(jn);
-        const int rc = rows_per_group * (256 / jw);
+        const int rc = std::min(kBwdRows, L.bt_rows * (256 / jw));
         const int rows = m - j0;
         const int nchunk = (rows + rc - 1) / rc;
         BwdTask t{};
@@ -1624,10 +1663,14 @@ void Engine::plan_solve() {
         t.jw = jw;
         t.nchunk = nchunk;
         t.slot0 = nchunk > 1 ? slots : -1;
+        t.ncol = ncol;
+        t.first = sym_.sn_first[s];
         for (int c = 0; c < nchunk; ++c) {
           t.chunk = c;
           t.i0 = j0 + c * rc;
           t.in = std::min(rc, m - t.i0);
+          t.goff = sym_.panel_off[s] + (int64_t)t.i0 * ncol + j0;
+          t.rpoff = sym_.sn_rowoff[s] + t.i0;
           h_btasks_.push_back(t);
           if (c == 0 && nchunk > 1) h_bheads_.push_back(t);
         }
@@ -1636,6 +1679,11 @@ void Engine::plan_solve() {
     }
     L.bt_count = (int64_t)h_btasks_.size() - L.bt_begin;
     L.bh_count = (int64_t)h_bheads_.size() - L.bh_begin;
+    int rows = 1;
+    for (int64_t k = L.bt_begin; k < L.bt_begin + L.bt_count; ++k)
+      rows = std::max(rows, (h_btasks_[k].in + 256 / h_btasks_[k].jw - 1) / (256 / h_btasks_[k].jw));
+    L.bt_regs = 4;
+    while (L.bt_regs < rows) L.bt_regs *= 2;
     part_slots_ = std::max(part_slots_, slots);
   }
   // pull map of the forward products: for front row r (index rowoff + r), the children's U rows
@@ -1661,6 +1709,17 @@ void Engine::plan_solve() {
       }
   }
   if (rows_total_ >= INT32_MAX) throw std::runtime_error("front row lists too long for 32-bit U indices");
+  // the same map with the first two children's rows inline (most rows have at most two), so that a
+  // gather reads them without the indirection through cidx
+  h_cpull_.assign(rows_total_, PullRow{-1, -1, 0, 0});
+  for (int64_t r = 0; r < rows_total_; ++r) {
+    const int64_t b = h_cptr_[r], e = h_cptr_[r + 1];
+    PullRow& q = h_cpull_[r];
+    if (e > b) q.u0 = h_cidx_[b];
+    if (e > b + 1) q.u1 = h_cidx_[b + 1];
+    q.b = (int32_t)std::min(e, b + 2);
+    q.e = (int32_t)e;
+  }
   validate_solve_plan();
 }
 
@@ -1684,13 +1743,43 @@ void Engine::validate_solve_plan() const {
     if (r.begin < 0 || r.count < 0 || r.begin + r.count > (int64_t)h_tasks_.size()) bad("stage range");
   for (const Level& L : levels_) {
     if (L.n_all != (int)L.h_all.size()) bad("level size");
-    if (L.fwd_chunks > 1 && (int64_t)L.n_all * L.fwd_tiles * L.fwd_chunks > part_slots_) bad("forward split scratch");
+    if (L.ft_begin < 0 || L.ft_begin + L.ft_count > (int64_t)h_ftasks_.size()) bad("forward task range");
+    if (L.fh_begin < 0 || L.fh_begin + L.fh_count > (int64_t)h_fheads_.size()) bad("forward split range");
+    if (L.bt_regs != 4 && L.bt_regs != 8 && L.bt_regs != 16 && L.bt_regs != 32) bad("backward rows per thread");
+    for (int64_t k = L.bt_begin; k < L.bt_begin + L.bt_count; ++k) {
+      const BwdTask& t = h_btasks_[k];
+      if (t.in > L.bt_regs * (256 / t.jw)) bad("backward task rows per thread");
+    }
+    if (L.ft_cols != 2 && L.ft_cols != 4 && L.ft_cols != 8 && L.ft_cols != 16 && L.ft_cols != 32)
+      bad("forward columns per thread");
+    for (int64_t k = L.ft_begin; k < L.ft_begin + L.ft_count; ++k) {
+      const FwdTask& t = h_ftasks_[k];
+      if ((std::min(kGvChunk, t.kstop - t.kbeg) + 3) / 4 > L.ft_cols) bad("forward task columns per thread");
+    }
     if (L.bt_begin < 0 || L.bt_begin + L.bt_count > (int64_t)h_btasks_.size()) bad("backward task range");
     if (L.bh_begin < 0 || L.bh_begin + L.bh_count > (int64_t)h_bheads_.size()) bad("backward split range");
   }
+  if ((int64_t)h_cpull_.size() != rows_total_) bad("pull map size");
+  for (const PullRow& q : h_cpull_)
+    if (q.u0 >= rows_total_ || q.u1 >= rows_total_ || (q.u0 < 0 && q.u1 >= 0) || q.b < 0 || q.b > q.e ||
+        q.e > (int64_t)h_cidx_.size())
+      bad("pull map entry");
+  for (const FwdTask& t : h_ftasks_) {
+    if (t.m < 1 || t.ncol < 1 || t.ncol > t.m || t.first < 0 || t.first + t.ncol > sym_.n) bad("forward task front");
+    if (t.r0 < 0 || t.r0 >= t.m || t.kbeg < 0 || t.kstop <= t.kbeg || t.kstop > std::min(t.ncol, t.r0 + kGvTile))
+      bad("forward task range");
+    if (t.goff < 0 || t.goff + (int64_t)t.m * t.ncol > panel_total_) bad("forward task panel");
+    if (t.rowoff < 0 || t.rowoff + t.m > rows_total_) bad("forward task rows");
+    if (t.nchunk > 1 ? (t.slot < 0 || t.slot >= part_slots_) : t.slot != -1) bad("forward task scratch");
+  }
+  for (const FwdTask& t : h_fheads_)
+    if (t.nchunk < 2 || t.kbeg != 0 || t.slot < 0 || t.slot + t.nchunk > part_slots_) bad("forward split head");
   for (const BwdTask& t : h_btasks_) {
     if (t.s < 0 || t.s >= sym_.nsn) bad("backward task front");
     const int ncol = sym_.sn_ncol[t.s], m = sym_.sn_m[t.s];
+    if (t.ncol != ncol || t.first != sym_.sn_first[t.s] || t.in > kBwdRows) bad("backward task front data");
+    if (t.goff != sym_.panel_off[t.s] + (int64_t)t.i0 * ncol + t.j0 || t.rpoff != sym_.sn_rowoff[t.s] + t.i0)
+      bad("backward task offsets");
     if (t.jw < 8 || t.jw > kGvTile || (t.jw & (t.jw - 1)) || t.jn < 1 || t.jn > t.jw) bad("backward task width");
     if (t.j0 < 0 || t.j0 + t.jn > ncol || t.i0 < t.j0 || t.in < 1 || t.i0 + t.in > m) bad("backward task range");
     if (t.nchunk > 1 && (t.slot0 < 0 || t.slot0 + t.nchunk > part_slots_)) bad("backward task scratch");
@@ -1721,9 +1810,12 @@ void Engine::ensure_device() {
   d_pairs_ = upload(h_pairs_);
   d_ttiles_ = upload(h_ttiles_);
   d_tasks_ = upload(h_tasks_);
+  d_ftasks_ = upload(h_ftasks_);
+  d_fheads_ = upload(h_fheads_);
   d_btasks_ = upload(h_btasks_);
   d_bheads_ = upload(h_bheads_);
   d_cptr_ = upload(h_cptr_);
+  d_cpull_ = upload(h_cpull_);
   d_cidx_ = upload(h_cidx_);
   for (Level& L : levels_) {
     L.d_all = upload(L.h_all);
@@ -1737,7 +1829,7 @@ Engine::~Engine() {
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
                   (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
-                  (void*)d_tasks_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_cptr_,
+                  (void*)d_tasks_, (void*)d_ftasks_, (void*)d_fheads_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_cptr_, (void*)d_cpull_,
                   (void*)d_cidx_})
     dfree(p);
   for (auto& L : levels_) {
@@ -1921,7 +2013,7 @@ void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
   if (!nm.d_T_) dalloc_into(nm.d_T_, std::max<int64_t>(1, t_total_) * ts, "hipMalloc (inverse scratch)");
   if (!nm.d_pos_) dalloc_into(nm.d_pos_, (size_t)sym_.n * sizeof(int32_t), "hipMalloc (pivot positions)");
   if (!nm.d_zpull_)
-    dalloc_into(nm.d_zpull_, (size_t)sym_.n * 2 * sizeof(int64_t), "hipMalloc (pivot positions)");
+    dalloc_into(nm.d_zpull_, (size_t)sym_.n * sizeof(PullRow), "hipMalloc (pivot positions)");
   if (!nm.d_rpos_)
     dalloc_into(nm.d_rpos_, (size_t)std::max<int64_t>(1, rows_total_) * sizeof(int32_t),
                 "hipMalloc (pivot positions)");
@@ -1947,13 +2039,34 @@ void Engine::build_inverse_panels(Numeric& nm, hipStream_t st) {
   }
   // pivot positions of the rows after this factorization's local pivoting
   hipLaunchKernelGGL(k_pivot_positions<T>, dim3(sym_.nsn), dim3(256), 0, st, S, N, nm.d_pos_,
-                     (const int64_t*)d_cptr_, nm.d_zpull_);
+                     d_cpull_, nm.d_zpull_);
   hipLaunchKernelGGL(k_row_positions, dim3(blocks_for(rows_total_)), dim3(256), 0, st, rows_total_,
                      (const int32_t*)d_rows_orig_, (const int32_t*)nm.d_pos_, nm.d_rpos_);
   nl += 2;
   hip_check(hipGetLastError(), "inverse panel kernels");
   timing_end(tok, st, nl, g_flops_);
   nm.g_ready_ = true;
+}
+
+// The sweep kernels instantiated for the register tiles of a level (a power of two).
+template <class T, class... A>
+static void launch_fwd_task(int cols, dim3 grid, hipStream_t st, A... a) {
+  switch (cols) {
+    case 2: hipLaunchKernelGGL((k_fwd_task<T, 2>), grid, dim3(256), 0, st, a...); break;
+    case 4: hipLaunchKernelGGL((k_fwd_task<T, 4>), grid, dim3(256), 0, st, a...); break;
+    case 8: hipLaunchKernelGGL((k_fwd_task<T, 8>), grid, dim3(256), 0, st, a...); break;
+    case 16: hipLaunchKernelGGL((k_fwd_task<T, 16>), grid, dim3(256), 0, st, a...); break;
+    default: hipLaunchKernelGGL((k_fwd_task<T, kFwdCols>), grid, dim3(256), 0, st, a...); break;
+  }
+}
+template <class T, class... A>
+static void launch_bwd_task(int rows, dim3 grid, hipStream_t st, A... a) {
+  switch (rows) {
+    case 4: hipLaunchKernelGGL((k_bwd_task<T, 4>), grid, dim3(256), 0, st, a...); break;
+    case 8: hipLaunchKernelGGL((k_bwd_task<T, 8>), grid, dim3(256), 0, st, a...); break;
+    case 16: hipLaunchKernelGGL((k_bwd_task<T, 16>), grid, dim3(256), 0, st, a...); break;
+    default: hipLaunchKernelGGL((k_bwd_task<T, 32>), grid, dim3(256), 0, st, a...); break;
+  }
 }
 
 template <class T>
@@ -1994,12 +2107,12 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   const int64_t* cptr = (const int64_t*)d_cptr_;
   const int32_t* cidx = (const int32_t*)d_cidx_;
   for (const Level& L : levels_) {
-    hipLaunchKernelGGL(k_fwd_gemv<T>, dim3(L.fwd_tiles, L.n_all, L.fwd_chunks), dim3(256), 0, st, S, G, L.d_all, p,
-                       L.fwd_split, (const T*)Xp, Y, U, part, (const int64_t*)nm.d_zpull_, cptr, cidx);
+    launch_fwd_task<T>(L.ft_cols, dim3((unsigned)L.ft_count), st, d_ftasks_ + L.ft_begin, p, G, (const T*)Xp, Y, U,
+                       part, (const PullRow*)nm.d_zpull_, (const PullRow*)d_cpull_, cidx);
     ++nl;
-    if (L.fwd_chunks > 1) {
-      hipLaunchKernelGGL(k_fwd_reduce<T>, dim3(L.fwd_tiles, L.n_all), dim3(256), 0, st, S, L.d_all, p, L.fwd_split,
-                         L.fwd_chunks, Y, U, (const T*)part, cptr, cidx);
+    if (L.fh_count > 0) {
+      hipLaunchKernelGGL(k_fwd_reduce<T>, dim3((unsigned)L.fh_count), dim3(256), 0, st, d_fheads_ + L.fh_begin, p, Y,
+                         U, (const T*)part, cptr, cidx);
       ++nl;
     }
   }
@@ -2012,12 +2125,12 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
     if (L.bt_count > 0) {
-      hipLaunchKernelGGL(k_bwd_task<T>, dim3((unsigned)L.bt_count), dim3(256), 0, st, S, Gt, d_btasks_ + L.bt_begin, p,
-                         (const T*)Y, Xp, rpos, part);
+      launch_bwd_task<T>(L.bt_regs, dim3((unsigned)L.bt_count), st, Gt, d_btasks_ + L.bt_begin, p, (const T*)Y, Xp,
+                         rpos, part);
       ++nl;
     }
     if (L.bh_count > 0) {
-      hipLaunchKernelGGL(k_bwd_reduce<T>, dim3((unsigned)L.bh_count), dim3(256), 0, st, S, d_bheads_ + L.bh_begin, p, Xp,
+      hipLaunchKernelGGL(k_bwd_reduce<T>, dim3((unsigned)L.bh_count), dim3(256), 0, st, d_bheads_ + L.bh_begin, p, Xp,
                          (const T*)part);
       ++nl;
     }

@@ -39,45 +39,21 @@ constexpr int kGvTile = 64;       // rows (forward) / columns (backward) per GEM
 constexpr int kGvChunk = 128;     // vector entries staged in LDS per pass
 constexpr int kMaxRhsChunk = 8;   // right-hand sides per pass
 
-// Stages kn rows of right-hand-side values [c0, c0 + pc) into ws (zeros elsewhere): two threads
-// per row, four values each; row_ptr(t) gives the address of row t's value c0. All loads of a
-// thread are issued before any is used (no branch around them: invalid rows read row 0).
-static_assert(kGvChunk * 2 == 256 && kMaxRhsChunk == 8, "staging layout: 2 threads x 4 values per row");
-template <class T, class RowPtr>
-__device__ __forceinline__ void stage_rows(T (*ws)[kMaxRhsChunk], int kn, int pc, RowPtr row_ptr) {
-  const int t = threadIdx.x >> 1, h = (threadIdx.x & 1) * 4;
-  const bool valid = t < kn;
-  const T* src = row_ptr(valid ? t : 0);
-  T v[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = src[min(h + c, pc - 1)];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) ws[t][h + c] = (valid && h + c < pc) ? v[c] : (T)0;
-}
+// The children's update rows mapped onto one front row (the forward products' pull map): the
+// first two inline (U row indices, -1 when absent), the others at cidx[b, e) in child order.
+struct alignas(16) PullRow {
+  int32_t u0, u1, b, e;
+};
 
-// Stages kn rows of z for the pivot positions q0 + t: the right-hand side in Xp plus, in child
-// order, the children's update rows mapped onto that row (the forward gather, done where z is
-// read). Two threads per row, four values each; invalid rows read row q0 and add no children.
+// A pointer into global memory as the compiler's address space 1: a value selected from two such
+// pointers is still known to be global, so loads through it are global loads (a generic pointer
+// gives flat loads, which the LDS wait counter also tracks).
 template <class T>
-__device__ __forceinline__ void stage_gathered(T (*ws)[kMaxRhsChunk], int kn, int pc, int p, int c0, int64_t q0,
-                                               const T* __restrict__ Xp, const T* __restrict__ U,
-                                               const int64_t* __restrict__ zpull, const int32_t* __restrict__ cidx) {
-  const int t = threadIdx.x >> 1, h = (threadIdx.x & 1) * 4;
-  const bool valid = t < kn;
-  const int64_t q = q0 + (valid ? t : 0);
-  const int64_t b = zpull[2 * q], e = valid ? zpull[2 * q + 1] : b;
-  const T* src = Xp + q * p + c0;
-  T v[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = src[min(h + c, pc - 1)];
-  for (int64_t i = b; i < e; ++i) {
-    const T* u = U + (int64_t)cidx[i] * p + c0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] += u[min(h + c, pc - 1)];
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) ws[t][h + c] = (valid && h + c < pc) ? v[c] : (T)0;
-}
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
+// The sweeps stage right-hand-side rows with two threads per row, four values each: 128 rows per
+// 256-thread pass.
+static_assert(kGvChunk * 2 == 256 && kMaxRhsChunk == 8, "staging layout: 2 threads x 4 values per row");
 
 // Sum, in child order, of value c of the children's update rows mapped onto front row r (index
 // rowoff + local row): the part of a contribution row's update that comes from below.
@@ -233,19 +209,18 @@ __global__ void __launch_bounds__(256) k_transpose_g(DevSym S, const int32_t* __
 
 // Pivot position of every original row: pos[rows_orig[rowoff + r]] = first + ipiv[first + r] for
 // the pivot rows r < ncol of each front (one workgroup per front, all fronts).
-// With them, per pivot position q, the range [zpull[2q], zpull[2q + 1]) of cidx holding the
-// children's update rows of the front row pivoted to q (the forward product's pull map).
+// With them, per pivot position q, the pull map entry of the front row pivoted to q (the
+// children's update rows summed into it by the forward products).
 template <class T>
 __global__ void __launch_bounds__(256) k_pivot_positions(DevSym S, DevNum<T> N, int32_t* __restrict__ pos,
-                                                         const int64_t* __restrict__ cptr, int64_t* __restrict__ zpull) {
+                                                         const PullRow* __restrict__ cpull, PullRow* __restrict__ zpull) {
   const int s = blockIdx.x;
   const int ncol = S.sn_ncol[s], first = S.sn_first[s];
   const int64_t roff = S.sn_rowoff[s];
   for (int r = threadIdx.x; r < ncol; r += blockDim.x) {
     const int q = first + N.ipiv[first + r];
     pos[S.rows_orig[roff + r]] = q;
-    zpull[2 * (int64_t)q] = cptr[roff + r];
-    zpull[2 * (int64_t)q + 1] = cptr[roff + r + 1];
+    zpull[q] = cpull[roff + r];
   }
 }
 
@@ -301,100 +276,176 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ src, int64_t str
   return v;
 }
 
-// [y1; u] = G z1 + [0; u_children] for a tile of 64 front rows and one chunk of G's columns: rows
-// < ncol (G11, lower triangular, so only columns up to the tile's end) give y1 in Y, contribution
-// rows get the children's update rows plus G21 z1 in U. z1 (right-hand side plus the children's
-// update rows, gathered while staging) goes through LDS, G is read along columns. 256 threads =
-// 64 rows x 4 column groups. grid (row tiles, fronts, column chunks of `split` columns).
-template <class T>
-__global__ void __launch_bounds__(256) k_fwd_gemv(DevSym S, const T* __restrict__ G, const int32_t* __restrict__ list,
-                                                   int p, int split, const T* __restrict__ Xp, T* __restrict__ Y,
-                                                   T* __restrict__ U, T* __restrict__ part,
-                                                   const int64_t* __restrict__ zpull, const int64_t* __restrict__ cptr,
-                                                   const int32_t* __restrict__ cidx) {
+// One task of the forward sweep: rows [r0, r0 + 64) of front s's G and its columns [kbeg, kstop)
+// (a chunk of the columns up to min(ncol, r0 + 64): G11 is lower triangular). Everything a task
+// needs about its front is stored in the task, so the first memory accesses of a workgroup are
+// the G entries and the right-hand-side gathers themselves.
+struct FwdTask {
+  int32_t m, ncol, first, r0;   // front rows, pivots, first pivot position; first row of the tile
+  int32_t kbeg, kstop, nchunk;  // columns of this chunk; chunks of the tile (> 1: split tile)
+  int32_t pad;
+  int64_t goff;                 // offset of the front's panel in G (column-major, ld = m)
+  int64_t rowoff;               // offset of the front's rows in the row lists (U, the pull map)
+  int64_t slot;                 // partial slot of this chunk (split tiles; chunk 0's in a head), else -1
+};
+
+constexpr int kFwdCols = kGvChunk / 4;  // G entries per thread and pass at most (4 column groups)
+
+// [y1; u] = G z1 + [0; u_children] over one task: rows < ncol (G11) give y1 in Y, contribution
+// rows get the children's update rows plus G21 z1 in U; a split tile's chunks store partial sums
+// instead (k_fwd_reduce adds them). 256 threads = 64 rows x 4 column groups; per pass of up to
+// kGvChunk columns a thread's C G entries (columns g, g + 4, ...; C >= the columns per thread of
+// every pass of the launch, columns past the pass's end read its last column) are loaded into
+// registers before z is staged, so the G loads, the gathers and (for contribution rows) the
+// children's sums of the epilogue are in flight together. Products and sums run in the same order
+// as one pass over the columns: results do not depend on the pass structure.
+template <class T, int C>
+__global__ void __launch_bounds__(256) k_fwd_task(const FwdTask* __restrict__ tasks, int p, const T* __restrict__ G,
+                                                   const T* __restrict__ Xp, T* __restrict__ Y, T* __restrict__ U,
+                                                   T* __restrict__ part, const PullRow* __restrict__ zpull,
+                                                   const PullRow* __restrict__ cpull, const int32_t* __restrict__ cidx) {
   __shared__ T zs[kGvChunk][kMaxRhsChunk];
   __shared__ T red[4][kGvTile][kMaxRhsChunk + 1];
-  const int f = blockIdx.y;
-  const int s = list[f];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int tile = blockIdx.x, chunk = blockIdx.z;
-  const int r0 = tile * kGvTile;
-  if (r0 >= m) return;
-  const int kend = min(ncol, r0 + kGvTile);
-  const int nchunk = (kend + split - 1) / split;
-  if (chunk >= nchunk) return;
-  const int kbeg = chunk * split, kstop = min(kend, kbeg + split);
-  const T* Gs = G + S.panel_off[s];
-  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
-  const int i = r0 + lane;
-  const bool rin = i < m;
+  const FwdTask tk = tasks[blockIdx.x];
+  const int lane = threadIdx.x % kGvTile;
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / kGvTile);
+  const int i = tk.r0 + lane;
+  const bool rin = i < tk.m;
+  const T* grow = G + tk.goff + (rin ? i : tk.r0);
+  const int64_t r = tk.rowoff + (rin ? i : tk.r0);
+  const bool cb_out = tk.nchunk == 1 && rin && i >= tk.ncol;  // writes a contribution row
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
+  // staging layout: two threads per row of z, four values each
+  const int st = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 4;
+  // the children's rows mapped onto row r (loaded whatever the row, used where it writes a
+  // contribution row: a load under a condition would be waited for at once)
+  const PullRow er = cpull[r];
+  // one pass over columns [k0, k0 + kGvChunk) of the chunk for one group of right-hand sides
+  auto pass = [&](int c0, int pc, int k0, T(&acc)[kMaxRhsChunk], T(&csum)[2], bool first_pass)
+                  __attribute__((always_inline)) {
+    const int kn = min(kGvChunk, tk.kstop - k0);
+    // z row st of the pass: pull map entry and right-hand side first (they need the row index only)
+    const bool sval = st < kn;
+    const int64_t zq = (int64_t)tk.first + k0 + (sval ? st : 0);
+    const PullRow zr = zpull[zq];
+    T zv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zv[c] = Xp[zq * p + c0 + min(sh + c, pc - 1)];
+    // this thread's G entries of the pass (columns g, g + 4, ...)
+    T gv[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) gv[q] = grow[(int64_t)(k0 + min(g + 4 * q, kn - 1)) * tk.m];
+    // the children's update rows, in child order: of z row st, and (first pass) of the
+    // contribution row r. The two inline ones of each are loaded whatever the row (row 0 stands in
+    // for an absent one) before any of them is used; the others (rare) after.
+    T za[4], zb[4], ea[2], eb[2];
+    {
+      const T* ua = U + (int64_t)max(zr.u0, 0) * p + c0;
+      const T* ub = U + (int64_t)max(zr.u1, 0) * p + c0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        za[c] = ua[min(sh + c, pc - 1)];
+        zb[c] = ub[min(sh + c, pc - 1)];
+      }
+    }
+    const int c1 = min(g, pc - 1), c2 = min(g + 4, pc - 1);
+    if (first_pass) {
+      const T* ua = U + (int64_t)max(er.u0, 0) * p + c0;
+      const T* ub = U + (int64_t)max(er.u1, 0) * p + c0;
+      ea[0] = ua[c1];
+      ea[1] = ua[c2];
+      eb[0] = ub[c1];
+      eb[1] = ub[c2];
+    }
+    {
+      const bool ha = sval && zr.u0 >= 0, hb = sval && zr.u1 >= 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        zv[c] = ha ? zv[c] + za[c] : zv[c];
+        zv[c] = hb ? zv[c] + zb[c] : zv[c];
+      }
+      for (int q = zr.b; sval && q < zr.e; ++q) {
+        const T* u = U + (int64_t)cidx[q] * p + c0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) zv[c] += u[min(sh + c, pc - 1)];
+      }
+    }
+    if (first_pass) {
+      // csum starts at +0 and never becomes -0, so adding +0 for an absent row changes nothing (and
+      // keeps the loads unconditional)
+      const bool ha = cb_out && er.u0 >= 0, hb = cb_out && er.u1 >= 0;
+      csum[0] += ha ? ea[0] : (T)0;
+      csum[1] += ha ? ea[1] : (T)0;
+      csum[0] += hb ? eb[0] : (T)0;
+      csum[1] += hb ? eb[1] : (T)0;
+      for (int q = er.b; cb_out && q < er.e; ++q) {
+        const T* u = U + (int64_t)cidx[q] * p + c0;
+        csum[0] += u[c1];
+        csum[1] += u[c2];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zs[st][sh + c] = (sval && sh + c < pc) ? zv[c] : (T)0;
+    __syncthreads();
+    if (rin) {
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+        if (g + 4 * q < kn) {
+#pragma unroll
+          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv[q] * zs[g + 4 * q][c];
+        }
+    }
+  };
+  // one group of at most kMaxRhsChunk right-hand sides; its first pass (and the first group) is not
+  // inside a loop, so that the gathers need not wait for the G loads still in flight
+  auto rhs_group = [&](int c0) __attribute__((always_inline)) {
     const int pc = min(kMaxRhsChunk, p - c0);
-    // slot of this (front, tile, chunk)
-    const int64_t slot = (((int64_t)f * gridDim.x + tile) * gridDim.z + chunk) * nrc + c0 / kMaxRhsChunk;
     T acc[kMaxRhsChunk];
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
-    for (int k0 = kbeg; k0 < kstop; k0 += kGvChunk) {
-      const int kn = min(kGvChunk, kstop - k0);
-      __syncthreads();
-      stage_gathered<T>(zs, kn, pc, p, c0, (int64_t)first + k0, Xp, U, zpull, cidx);
-      __syncthreads();
-      if (rin) {
-        const T* gcol = Gs + (int64_t)k0 * m + i;
-#pragma unroll 4
-        for (int t = g; t < kn; t += 4) {
-          const T gv = gcol[(int64_t)t * m];
-#pragma unroll
-          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv * zs[t][c];
-        }
-      }
-    }
+    T csum[2] = {(T)0, (T)0};  // children's update rows at values c0 + g, c0 + g + 4 of row r
+    pass(c0, pc, tk.kbeg, acc, csum, true);
+    for (int k0 = tk.kbeg + kGvChunk; k0 < tk.kstop; k0 += kGvChunk) pass(c0, pc, k0, acc, csum, false);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) red[g][lane][c] = acc[c];
     __syncthreads();
-    for (int c = g; c < pc; c += 4) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = g + 4 * h;
+      if (c >= pc) continue;
       const T v = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
-      if (nchunk > 1) {
-        part[slot * kPartSlot + c * kGvTile + lane] = v;
+      if (tk.nchunk > 1) {
+        part[((tk.slot * nrc + c0 / kMaxRhsChunk) * kPartSlot) + c * kGvTile + lane] = v;
       } else if (rin) {
-        const int64_t r = S.sn_rowoff[s] + i;
-        if (i < ncol) Y[(int64_t)(first + i) * p + c0 + c] = v;
-        else U[r * p + c0 + c] = children_sum<T>(r, p, c0 + c, U, cptr, cidx) + v;
+        if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c0 + c] = v;
+        else U[r * p + c0 + c] = csum[h] + v;
       }
     }
     __syncthreads();
-  }
+  };
+  rhs_group(0);
+  for (int c0 = kMaxRhsChunk; c0 < p; c0 += kMaxRhsChunk) rhs_group(c0);
 }
 
-// Second launch of a split forward level: for every (front, row tile) with more than one column
-// chunk, the chunks' partials added in chunk order, written like k_fwd_gemv's result.
-// grid (row tiles, fronts), chunks = the forward launch's grid.z.
+// Second launch of a split forward level: one workgroup per split row tile (its chunk-0 task in
+// `heads`), the chunks' partials added in chunk order, written like k_fwd_task's result.
 template <class T>
-__global__ void __launch_bounds__(256) k_fwd_reduce(DevSym S, const int32_t* __restrict__ list, int p, int split, int chunks,
-                                                     T* __restrict__ Y, T* __restrict__ U, const T* __restrict__ part,
+__global__ void __launch_bounds__(256) k_fwd_reduce(const FwdTask* __restrict__ heads, int p, T* __restrict__ Y,
+                                                     T* __restrict__ U, const T* __restrict__ part,
                                                      const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
-  const int f = blockIdx.y;
-  const int s = list[f];
-  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
-  const int tile = blockIdx.x;
-  const int r0 = tile * kGvTile;
-  if (r0 >= m) return;
-  const int kend = min(ncol, r0 + kGvTile);
-  const int nchunk = (kend + split - 1) / split;
-  if (nchunk <= 1) return;
+  const FwdTask tk = heads[blockIdx.x];
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int idx = threadIdx.x; idx < kGvTile * p; idx += blockDim.x) {
     const int lane = idx % kGvTile, c = idx / kGvTile;
-    const int i = r0 + lane;
-    if (i >= m) continue;
+    const int i = tk.r0 + lane;
+    if (i >= tk.m) continue;
     const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
-    const int64_t slot0 = ((int64_t)f * gridDim.x + tile) * chunks * nrc + q;
-    const T v = sum_partials<T>(part + slot0 * kPartSlot + cc * kGvTile + lane, (int64_t)nrc * kPartSlot, nchunk);
-    const int64_t r = S.sn_rowoff[s] + i;
-    if (i < ncol) Y[(int64_t)(first + i) * p + c] = v;
+    const T v = sum_partials<T>(part + (tk.slot * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + lane,
+                                (int64_t)nrc * kPartSlot, tk.nchunk);
+    const int64_t r = tk.rowoff + i;
+    if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c] = v;
     else U[r * p + c] = children_sum<T>(r, p, c, U, cptr, cidx) + v;
   }
 }
@@ -429,58 +480,90 @@ __global__ void k_solve_diag(int n, const T* __restrict__ dval, const T* __restr
 
 // One task of the backward sweep: columns [j0, j0 + jn) of front s (jw = the power of two >= jn
 // that the 256 threads are laid out by: jw columns x 256 / jw row groups) and rows [i0, i0 + in)
-// of its Gt, chunk `chunk` of the nchunk row chunks of that column tile (slot0: the slot of its
-// chunk 0 when nchunk > 1).
+// of its Gt (in <= kBwdRows), chunk `chunk` of the nchunk row chunks of that column tile (slot0:
+// the slot of its chunk 0 when nchunk > 1). The front's data the task needs is stored in it.
 struct BwdTask {
   int32_t s, j0, jn, jw, i0, in, chunk, nchunk;
   int64_t slot0;
+  int32_t ncol, first;  // the front's pivots and first pivot position
+  int64_t goff;         // Gt offset of the task's element (i0, j0)
+  int64_t rpoff;        // offset of row i0 of the front in the row lists (pivot positions rpos)
 };
+
+constexpr int kBwdRows = 512;  // rows of a backward task at most (staged in LDS)
 
 // x1 = G^T [y1; x_cb] over one task: y1 from Y (pivot rows), x_cb from Xp at the pivot positions
 // of the front's contribution rows (rows of ancestors, final already); the result goes to Xp at
 // the front's own pivot positions, or, for a split tile, to the task's partial slot. Gt is read
-// along rows (jw consecutive columns per row group).
-template <class T>
-__global__ void __launch_bounds__(256) k_bwd_task(DevSym S, const T* __restrict__ Gt, const BwdTask* __restrict__ tasks,
-                                                   int p, const T* __restrict__ Y, T* Xp, const int32_t* __restrict__ rpos,
+// along rows (jw consecutive columns per row group). A thread's R rows (grp, grp + ngrp, ...) of
+// Gt are loaded into registers first, so they are in flight while the rows of [y1; x_cb] are
+// gathered (pivot positions, then values) and staged; the products run in row order as in one
+// pass over the rows. R >= the rows per thread of every task of the launch.
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, const BwdTask* __restrict__ tasks, int p,
+                                                   const T* __restrict__ Y, T* Xp, const int32_t* __restrict__ rpos,
                                                    T* __restrict__ part) {
-  __shared__ T ws[kGvChunk][kMaxRhsChunk];
-  __shared__ T red[256][kMaxRhsChunk + 1];
+  static_assert(R >= 1 && R <= kBwdRows / 8, "rows per thread");
+  __shared__ T ws[kBwdRows][kMaxRhsChunk];
+  T(*red)[kMaxRhsChunk + 1] = reinterpret_cast<T(*)[kMaxRhsChunk + 1]>(&ws[0][0]);  // after the products
+  static_assert(256 * (kMaxRhsChunk + 1) <= kBwdRows * kMaxRhsChunk, "reduction area inside the staging area");
   const BwdTask tk = tasks[blockIdx.x];
-  const int s = tk.s;
-  const int ncol = S.sn_ncol[s], first = S.sn_first[s];
   const int JW = tk.jw, ngrp = 256 / JW;
   const int tid = threadIdx.x, jj = tid % JW, grp = tid / JW;
-  const int j = tk.j0 + jj;
   const bool jin = jj < tk.jn;
-  const int32_t* rp = rpos + S.sn_rowoff[s];
-  const T* Gs = Gt + S.panel_off[s];
+  const int st = tid >> 1, sh = (tid & 1) * 4;  // staging: two threads per row, four values each
+  constexpr int kRounds = kBwdRows / 128;
+  // pivot positions of the staged rows first (the value loads wait for them, and the memory
+  // counter is in order), then the R rows of Gt: all R loads are issued (rows past the task's end
+  // read its last row), since loads under a condition would be waited for one by one
+  const int32_t* rp = rpos + tk.rpoff;
+  int32_t pos[kRounds];
+#pragma unroll
+  for (int k = 0; k < kRounds; ++k) pos[k] = rp[min(st + 128 * k, tk.in - 1)];
+  const T* g0 = Gt + tk.goff + (jin ? jj : 0);
+  T gv[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) gv[q] = g0[(int64_t)min(grp + q * ngrp, tk.in - 1) * tk.ncol];
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
-  const int iend = tk.i0 + tk.in;
-  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
+  // one group of at most kMaxRhsChunk right-hand sides; the first group is not inside a loop, so
+  // that its value loads need not wait for the Gt loads still in flight
+  auto rhs_group = [&](int c0) __attribute__((always_inline)) {
     const int pc = min(kMaxRhsChunk, p - c0);
+    // rows st + 128 k of [y1; x_cb] (rows past the task's end read its last row; global pointers, so
+    // that the loads are not counted with the LDS accesses)
+    T v[kRounds][4];
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+      if (k * 128 >= tk.in) break;
+      const int i = tk.i0 + min(st + 128 * k, tk.in - 1);
+      const GlobalPtr<const T> src = i < tk.ncol ? GlobalPtr<const T>(Y) + (int64_t)(tk.first + i) * p + c0
+                                                 : GlobalPtr<const T>(Xp) + (int64_t)pos[k] * p + c0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[k][c] = src[min(sh + c, pc - 1)];
+    }
+    __syncthreads();  // the previous group of right-hand sides is done with the staging area
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+      if (k * 128 >= tk.in) break;
+      const bool valid = st + 128 * k < tk.in;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ws[st + 128 * k][sh + c] = (valid && sh + c < pc) ? v[k][c] : (T)0;
+    }
+    __syncthreads();
     T acc[kMaxRhsChunk];
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
-    for (int i0 = tk.i0; i0 < iend; i0 += kGvChunk) {
-      const int kn = min(kGvChunk, iend - i0);
-      __syncthreads();
-      stage_rows<T>(ws, kn, pc, [&](int t) {
-        const int i = i0 + t;
-        const int64_t r = rp[i];  // every front row has a pivot position; used for contribution rows
-        return i < ncol ? Y + (int64_t)(first + i) * p + c0 : Xp + r * p + c0;
-      });
-      __syncthreads();
-      if (jin) {
-        const T* grow = Gs + (int64_t)i0 * ncol + j;
-#pragma unroll 8
-        for (int t = grp; t < kn; t += ngrp) {
-          const T gv = grow[(int64_t)t * ncol];
+    if (jin) {
 #pragma unroll
-          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv * ws[t][c];
+      for (int q = 0; q < R; ++q) {
+        const int t = grp + q * ngrp;
+        if (t < tk.in) {
+#pragma unroll
+          for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv[q] * ws[t][c];
         }
       }
     }
+    __syncthreads();  // every thread is done reading the staged rows
 #pragma unroll
     for (int c = 0; c < kMaxRhsChunk; ++c) red[tid][c] = acc[c];
     __syncthreads();
@@ -488,29 +571,29 @@ __global__ void __launch_bounds__(256) k_bwd_task(DevSym S, const T* __restrict_
     for (int idx = tid; idx < JW * pc; idx += 256) {
       const int j2 = idx % JW, c = idx / JW;
       if (j2 >= tk.jn) continue;
-      T v = (T)0;
-      for (int g2 = 0; g2 < ngrp; ++g2) v += red[g2 * JW + j2][c];
-      if (slot) slot[c * kGvTile + j2] = v;
-      else Xp[(int64_t)(first + tk.j0 + j2) * p + c0 + c] = v;
+      T s = (T)0;
+      for (int g2 = 0; g2 < ngrp; ++g2) s += red[g2 * JW + j2][c];
+      if (slot) slot[c * kGvTile + j2] = s;
+      else Xp[(int64_t)(tk.first + tk.j0 + j2) * p + c0 + c] = s;
     }
-    __syncthreads();
-  }
+  };
+  rhs_group(0);
+  for (int c0 = kMaxRhsChunk; c0 < p; c0 += kMaxRhsChunk) rhs_group(c0);
 }
 
 // Second launch of a split backward level: one workgroup per split column tile (the task of its
 // chunk 0, in `heads`), the chunks' partials added in chunk order and written to Xp.
 template <class T>
-__global__ void __launch_bounds__(256) k_bwd_reduce(DevSym S, const BwdTask* __restrict__ heads, int p, T* Xp,
+__global__ void __launch_bounds__(256) k_bwd_reduce(const BwdTask* __restrict__ heads, int p, T* Xp,
                                                      const T* __restrict__ part) {
   const BwdTask tk = heads[blockIdx.x];
-  const int first = S.sn_first[tk.s];
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   for (int idx = threadIdx.x; idx < tk.jn * p; idx += blockDim.x) {
     const int j2 = idx % tk.jn, c = idx / tk.jn;
     const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
     const T v = sum_partials<T>(part + (tk.slot0 * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + j2,
                                 (int64_t)nrc * kPartSlot, tk.nchunk);
-    Xp[(int64_t)(first + tk.j0 + j2) * p + c] = v;
+    Xp[(int64_t)(tk.first + tk.j0 + j2) * p + c] = v;
   }
 }
 
