@@ -261,16 +261,19 @@ constexpr int kGvSplit = 256;          // columns per chunk of a split forward p
 constexpr int kFwdTargetGroups = 1536;  // workgroups a forward level's grid should reach (6 per CU)
 constexpr int kPartSlot = kGvTile * kMaxRhsChunk;  // values per partial slot
 
-// src[0] + src[stride] + ... (n terms), added in that order; the loads are issued eight at a time.
+// src[0] + src[stride] + ... (n terms), added in that order. The loads are issued 32 at a time and
+// without conditions (terms past the end read the last one and are not added), so that a block's
+// loads are in flight together.
 template <class T>
 __device__ __forceinline__ T sum_partials(const T* __restrict__ src, int64_t stride, int n) {
+  constexpr int kB = 32;
   T v = (T)0;
-  for (int c0 = 0; c0 < n; c0 += 8) {
-    T x[8];
+  for (int c0 = 0; c0 < n; c0 += kB) {
+    T x[kB];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = c0 + u < n ? src[(int64_t)(c0 + u) * stride] : (T)0;
+    for (int u = 0; u < kB; ++u) x[u] = src[(int64_t)min(c0 + u, n - 1) * stride];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < kB; ++u)
       if (c0 + u < n) v += x[u];
   }
   return v;
