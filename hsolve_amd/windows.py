@@ -48,6 +48,15 @@ EXTRA_RERUN_ROUNDS = 4
 MAX_RERUN_ROUNDS = 32
 STALL_ROUNDS = 2
 MULTIPLET_OFFSET = 0.01
+# shifts counted inside an interval together with its ends (with enough worker streams); see
+# WindowSolver.blind_points. Five: seven concurrent counts in the first round (box48: 58 ms against
+# 29 + 44 ms for the ends and then the interpolated cuts, six slices of 73-93 eigenvalues); six
+# made seven smaller slices and a slower round of eight counts (77 ms)
+BLIND_POINTS = 5
+
+
+class _NeedsCount(Exception):
+    """A dry run of plan() reached a shift that has not been counted."""
 
 
 def end_resolution(scale: float, s: float) -> float:
@@ -136,6 +145,8 @@ class WindowSolver:
             for p in self.known:
                 if p.s == s:
                     return p
+        if getattr(self, "_dry", False):
+            raise _NeedsCount(s)
         step = 1e-10 * max(self.scale, abs(s))
         nu, s_used, off, near = count_below(self.c, s, outward, step, s_is_gap=False)
         p = CountPoint(s, nu, off, near)
@@ -581,6 +592,32 @@ class WindowSolver:
             self._concurrently(len(todo), lambda i: self.count(todo[i], side[todo[i]]), "certificate")
         return [self.count(s, o) for s, o in points]
 
+    def blind_points(self, lower: float, upper: float) -> list:
+        """Shifts counted in the same round as an interval's ends, before anything about the
+        window's eigenvalues is known: as many as the worker streams beyond the two ends hold (at
+        most BLIND_POINTS), equally spaced inside the window. plan() takes its cuts among them when
+        they fall within its size limits; otherwise a second round counts the interpolated cuts."""
+        k = min(BLIND_POINTS, self.streams - 2)
+        if k <= 0 or not (math.isfinite(lower) and math.isfinite(upper)) or not upper > lower:
+            return []
+        return [lower + (upper - lower) * j / (k + 1) for j in range(1, k + 1)]
+
+    def plan_from_known(self, w: Slice, c_target: int, c_max: int) -> bool:
+        """Whether plan() finds every cut of w among the points counted so far (a dry run that
+        counts nothing and leaves the known points as they were)."""
+        with self._lock:
+            saved = list(self.known)
+        self._dry = True
+        try:
+            self.plan(w, c_target, c_max)
+            return True
+        except _NeedsCount:
+            return False
+        finally:
+            self._dry = False
+            with self._lock:
+                self.known = saved
+
     def speculative_cuts(self, w: Slice, c_target: int, c_max: int):
         """The shifts plan() is likely to count at first: its sequence of eigenvalue-count targets
         over w, each placed by linear interpolation between the counts at w's ends (the guess plan()
@@ -715,7 +752,8 @@ class WindowSolver:
         cert = rep["certificate"]
         k = t["k"]
         if mode == "interval":
-            lo, hi = self.count_all([(t["lower"], -1.0), (t["upper"], +1.0)])
+            blind = [(x, +1.0) for x in self.blind_points(t["lower"], t["upper"])]
+            lo, hi = self.count_all([(t["lower"], -1.0), (t["upper"], +1.0)] + blind)[:2]
             windows.append(Slice(lo, hi))
             cert["lower"], cert["upper"] = lo.s, hi.s
         elif mode == "smallest":
@@ -754,7 +792,9 @@ class WindowSolver:
             empty = Block(c.ops, None, [])
             return empty, [], mode == "interval"
         if self.streams > 1:
-            self.count_all([(x, +1.0) for w in windows for x in self.speculative_cuts(w, c_target, c_max)])
+            # a second round of counts only for windows whose cuts the known points do not supply
+            todo = [w for w in windows if not self.plan_from_known(w, c_target, c_max)]
+            self.count_all([(x, +1.0) for w in todo for x in self.speculative_cuts(w, c_target, c_max)])
         for w in windows:
             self.slices.extend(self.plan(w, c_target, c_max))
         c.event("slices", slices=len(self.slices))

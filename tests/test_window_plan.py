@@ -101,3 +101,67 @@ def test_speculative_tail_point_is_the_midpoint_after_the_last_cut():
     assert len(spec) == 5 and spec[3] == pytest.approx(40.0 + 14.0 * 356 / 424)
     # a window within c_max needs no cut at all
     assert w.speculative_cuts(Slice(a, CountPoint(54.0, 1100)), 96, 120) == []
+
+
+def _dry_aware(w):
+    """The exact-count solver of _solver with the dry-run rule of WindowSolver.count: an uncounted
+    shift raises during plan_from_known."""
+    from hsolve_amd.windows import _NeedsCount
+    inner = w.count
+
+    def count(self, s, outward):
+        if getattr(self, "_dry", False) and all(p.s != s for p in self.known):
+            raise _NeedsCount(s)
+        return inner(s, outward)
+
+    w.count = types.MethodType(count, w)
+    return w
+
+
+@pytest.mark.parametrize("streams,k", [(1, 0), (2, 0), (5, 3), (8, 5), (16, 5)])
+def test_blind_points_are_equally_spaced_inside_the_window(streams, k):
+    w = _solver(np.linspace(1.0, 2.0, 10))
+    w.streams = streams
+    pts = w.blind_points(40.0, 54.0)
+    assert len(pts) == k
+    assert np.allclose(pts, 40.0 + 14.0 * np.arange(1, k + 1) / (k + 1))
+    assert w.blind_points(40.0, math.inf) == [] and w.blind_points(54.0, 40.0) == []
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_blind_points_supply_every_cut_of_a_box48_shaped_window(seed):
+    # 500 eigenvalues over a window where the density grows 1.6-fold: five equally spaced points
+    # leave intervals of 77-89 eigenvalues, all within plan()'s limits [48, 120]
+    lam = _cavity_spectrum(37835, 822, 600, seed)
+    w = _dry_aware(_solver(lam))
+    w.streams = 8
+    a = w.count(40.0, -1.0)
+    hi = float(np.sort(lam)[a.nu + 500 - 1]) + 1e-7
+    b = w.count(hi, +1.0)
+    for s in w.blind_points(40.0, hi):
+        w.count(s, +1.0)
+    known = [(p.s, p.nu) for p in w.known]
+    win = Slice(a, b)
+    assert w.plan_from_known(win, 96, 120)
+    assert [(p.s, p.nu) for p in w.known] == known  # the dry run leaves the known points alone
+    n_before = len(w.calls)
+    sl = w.plan(win, 96, 120)
+    assert len(w.calls) == n_before
+    assert sum(s.count() for s in sl) == 500 and all(48 <= s.count() <= 120 for s in sl)
+
+
+def test_dry_run_reports_a_window_the_blind_points_do_not_cover():
+    # about 2,400 eigenvalues in [40, 100]: intervals of ~350 between the blind points, so plan()
+    # needs more counts; the dry run says so, counts nothing and leaves the known points alone
+    lam = _cavity_spectrum(37835, 822, 3000)
+    w = _dry_aware(_solver(lam))
+    w.streams = 8
+    a, b = w.count(40.0, -1.0), w.count(100.0, +1.0)
+    assert b.nu - a.nu > 2000
+    for s in w.blind_points(40.0, 100.0):
+        w.count(s, +1.0)
+    known = [(p.s, p.nu) for p in w.known]
+    n_before = len(w.calls)
+    assert not w.plan_from_known(Slice(a, b), 96, 120)
+    assert len(w.calls) == n_before and [(p.s, p.nu) for p in w.known] == known
+    assert not w._dry
