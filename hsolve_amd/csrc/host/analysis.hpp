@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace hsg {
@@ -40,8 +41,13 @@ struct Graph {
 /// symmetrised and the diagonal dropped.
 Graph graph_from_csr(int32_t n, const int64_t* rowptr, const int32_t* colind);
 
-/// Nested-dissection ordering: perm[new] = old.
-std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt);
+/// Nested-dissection ordering: perm[new] = old. With `ranges`, also the ranges [lo, hi) of new
+/// numbers held by the subgraphs of the dissection that first have at most `range_max` vertices
+/// (disjoint; their separators above lie in no range). Each range is closed below: a vertex
+/// numbered in [lo, hi) has all of its lower-numbered neighbours inside [lo, hi).
+std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt,
+                                       std::vector<std::pair<int32_t, int32_t>>* ranges = nullptr,
+                                       int32_t range_max = 0);
 
 struct Symbolic {
   int32_t n = 0;

@@ -463,7 +463,8 @@ void min_degree_leaf(const Graph& g, const std::vector<int32_t>& verts, const st
 
 }  // namespace
 
-std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt) {
+std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& opt,
+                                       std::vector<std::pair<int32_t, int32_t>>* ranges, int32_t range_max) {
   // Subgraphs are independent once their separator is numbered, so they are dissected by a pool
   // of threads. Every task owns the number range [lo, hi) of its vertices and a generator seeded
   // from its position in the dissection tree, so the result does not depend on which thread
@@ -479,13 +480,15 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
     int32_t lo = 0, hi = 0;  // numbers [lo, hi) are assigned to these vertices
     int32_t id = 0;
     uint64_t seed = 0;
+    bool ranged = false;     // the task or an ancestor has its range recorded
   };
   auto child_seed = [](uint64_t parent, uint64_t k) {
     Rng r(parent ^ (0x9E3779B97F4A7C15ull * (k + 1)));
     return r.next();
   };
   std::atomic<int32_t> next_id{1};
-  std::mutex mu;
+  std::mutex mu, range_mu;
+  if (ranges) ranges->clear();
   std::condition_variable cv;
   std::vector<Task> stack;
   int busy = 0;
@@ -505,6 +508,13 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
   auto process = [&](Task& t, std::vector<int32_t>& local, std::vector<Task>& out) {
     const int32_t nv = static_cast<int32_t>(t.verts.size());
     if (nv == 0) return;
+    if (ranges && !t.ranged && nv <= range_max) {
+      // the first subgraph on this branch small enough: its numbers [lo, hi) go to no separator
+      // above it, and its vertices' neighbours outside it are such separators (numbered later)
+      std::lock_guard<std::mutex> lk(range_mu);
+      ranges->emplace_back(t.lo, t.hi);
+      t.ranged = true;
+    }
     for (int32_t v : t.verts) label[v].store(t.id, std::memory_order_relaxed);
     if (nv <= opt.leaf_size) {
       std::vector<int32_t> ord;
@@ -583,6 +593,7 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
         c.verts = std::move(parts[k]);
         c.id = next_id.fetch_add(1);
         c.seed = child_seed(t.seed, static_cast<uint64_t>(k));
+        c.ranged = t.ranged;
         out.push_back(std::move(c));
       }
       return;
@@ -641,6 +652,7 @@ std::vector<int32_t> nested_dissection(const Graph& g, const OrderingOptions& op
     c1.id = next_id.fetch_add(1);
     c0.seed = child_seed(t.seed, 0);
     c1.seed = child_seed(t.seed, 1);
+    c0.ranged = c1.ranged = t.ranged;
     out.push_back(std::move(c0));
     out.push_back(std::move(c1));
   };

@@ -9,7 +9,10 @@
 //   5. front row structures bottom-up (own columns, children's contribution rows, original
 //      entries), extend-add maps, storage offsets and the level schedule (height in the tree).
 #include <algorithm>
+#include <atomic>
 #include <map>
+#include <mutex>
+#include <condition_variable>
 #include <cstdint>
 #include <numeric>
 #include <stdexcept>
@@ -67,13 +70,23 @@ void permuted_lower(int32_t n, const Graph& g, const std::vector<int32_t>& order
 // i's entries left of the diagonal are the neighbours of order[i] numbered before i. The algorithm
 // gives the elimination tree whatever order a row's columns are visited in, so no sorted lower
 // pattern is needed.
+// Liu's algorithm with path compression: column i (new numbering) walks up from each lower
+// neighbour j to the root of j's current tree and makes i its parent. With `ranges` (closed below:
+// see nested_dissection), the columns of each range are processed first, the ranges on several
+// threads (a range's walks stay inside it), then the other columns in increasing order; the
+// parents are those of the plain column-by-column pass, since a range's walks touch no column a
+// later walk from outside it would see differently. A walk leaving its range (ranges not closed
+// below) makes the whole pass run again column by column.
 std::vector<int32_t> etree_from_graph(int32_t n, const Graph& g, const std::vector<int32_t>& order,
-                                      const std::vector<int32_t>& iorder) {
+                                      const std::vector<int32_t>& iorder,
+                                      const std::vector<std::pair<int32_t, int32_t>>& ranges) {
   std::vector<int32_t> parent(n, -1), anc(n, -1);
-  for (int32_t i = 0; i < n; ++i) {
+  auto column = [&](int32_t i, int32_t lo) {
     const int32_t v = order[i];
+    bool ok = true;
     for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p) {
       int32_t j = iorder[g.adj[p]];
+      if (j < lo) ok = false;
       while (j != -1 && j < i) {
         const int32_t next = anc[j];
         anc[j] = i;
@@ -84,7 +97,39 @@ std::vector<int32_t> etree_from_graph(int32_t n, const Graph& g, const std::vect
         j = next;
       }
     }
+    return ok;
+  };
+  std::vector<char> in_range;
+  if (!ranges.empty()) {
+    in_range.assign(n, 0);
+    for (const auto& r : ranges) {
+      if (r.first < 0 || r.second > n || r.first > r.second) throw std::runtime_error("etree: bad range");
+      for (int32_t i = r.first; i < r.second; ++i) {
+        if (in_range[i]) throw std::runtime_error("etree: overlapping ranges");
+        in_range[i] = 1;
+      }
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<bool> closed{true};
+    auto worker = [&]() {
+      for (size_t k; (k = next.fetch_add(1)) < ranges.size();)
+        for (int32_t i = ranges[k].first; i < ranges[k].second; ++i)
+          if (!column(i, ranges[k].first)) closed = false;
+    };
+    const int nt = std::min<int>((int)ranges.size(),
+                                 std::min(16, (int)std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    if (!closed) {
+      in_range.clear();
+      std::fill(parent.begin(), parent.end(), -1);
+      std::fill(anc.begin(), anc.end(), -1);
+    }
   }
+  for (int32_t i = 0; i < n; ++i)
+    if (in_range.empty() || !in_range[i]) column(i, 0);
   return parent;
 }
 
@@ -162,15 +207,17 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
                          const AmalgamationOptions& aopt, const int32_t* given_order) {
   if (n <= 0) throw std::runtime_error("analyze_pattern: empty matrix");
   Graph g = graph_from_csr(n, rowptr, colind);
+  // ranges of the dissection for the elimination tree on several threads (large problems only)
+  std::vector<std::pair<int32_t, int32_t>> ranges;
   std::vector<int32_t> order = given_order ? std::vector<int32_t>(given_order, given_order + n)
-                                           : nested_dissection(g, oopt);
+                               : nested_dissection(g, oopt, n >= 65536 ? &ranges : nullptr, n / 32);
   std::vector<int32_t> iorder(n, -1);
   for (int32_t k = 0; k < n; ++k) {
     if (order[k] < 0 || order[k] >= n || iorder[order[k]] != -1)
       throw std::runtime_error("analyze_pattern: invalid ordering");
     iorder[order[k]] = k;
   }
-  std::vector<int32_t> parent = etree_from_graph(n, g, order, iorder);
+  std::vector<int32_t> parent = etree_from_graph(n, g, order, iorder, ranges);
   // postorder and compose; a postorder is a topological order of the tree, so the tree of the
   // postordered pattern is the same tree relabelled (no second elimination-tree pass)
   {
@@ -351,20 +398,36 @@ Symbolic analyze_pattern(int32_t n, const int64_t* rowptr, const int32_t* colind
           if (tmp[q] < l) bad[t] = 1;
       }
     };
-    for (int32_t lv = 0; lv < S.nlevels; ++lv) {
-      const int32_t k0 = static_cast<int32_t>(S.level_ptr[lv]), k1 = static_cast<int32_t>(S.level_ptr[lv + 1]);
-      const int nt = (k1 - k0) >= 512 ? nthr : 1;
-      if (nt <= 1) {
-        fronts(0, k0, k1);
-        continue;
+    // one pool of threads for all levels: the fronts of a level are handed out one by one by an
+    // atomic counter, and the threads meet at a barrier between levels (a front's children lie on
+    // lower levels)
+    std::vector<std::atomic<int32_t>> next(S.nlevels);
+    for (int32_t lv = 0; lv < S.nlevels; ++lv) next[lv].store(S.level_ptr[lv], std::memory_order_relaxed);
+    std::mutex bmu;
+    std::condition_variable bcv;
+    int arrived = 0, generation = 0;
+    auto barrier = [&]() {
+      std::unique_lock<std::mutex> lk(bmu);
+      const int gen = generation;
+      if (++arrived == nthr) {
+        arrived = 0;
+        ++generation;
+        bcv.notify_all();
+      } else {
+        bcv.wait(lk, [&] { return generation != gen; });
       }
-      std::vector<std::thread> pool;
-      for (int t = 1; t < nt; ++t)
-        pool.emplace_back(fronts, t, k0 + static_cast<int32_t>(int64_t{k1 - k0} * t / nt),
-                          k0 + static_cast<int32_t>(int64_t{k1 - k0} * (t + 1) / nt));
-      fronts(0, k0, k0 + static_cast<int32_t>(int64_t{k1 - k0} / nt));
-      for (auto& th : pool) th.join();
-    }
+    };
+    auto run = [&](int t) {
+      for (int32_t lv = 0; lv < S.nlevels; ++lv) {
+        const int32_t k1 = S.level_ptr[lv + 1];
+        for (int32_t k; (k = next[lv].fetch_add(1, std::memory_order_relaxed)) < k1;) fronts(t, k, k + 1);
+        if (nthr > 1) barrier();
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; ++t) pool.emplace_back(run, t);
+    run(0);
+    for (auto& th : pool) th.join();
     for (char b : bad)
       if (b) throw std::runtime_error("analyze_pattern: row above the supernode's columns");
     for (int32_t s = 0; s < S.nsn; ++s) {
