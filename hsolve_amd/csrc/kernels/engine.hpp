@@ -105,7 +105,8 @@ class Engine {
     int64_t bt_begin = 0, bt_count = 0;  // backward tasks of the level
     int64_t bh_begin = 0, bh_count = 0;  // its split column tiles (their chunk-0 tasks)
     int ft_cols = kFwdCols;              // G entries per thread and pass of the forward tasks (2 ... 32)
-    int bt_rows = 32;                    // chunk rows per row group of the backward tasks (16 or 32)
+    bool ft_wave = false;                // forward tasks one per wave (every tile small, one chunk)
+    int bt_rows = 32;                    // chunk rows per row group of the backward tasks
     int bt_regs = 32;                    // rows per thread of the backward tasks at most (4 ... 32)
     std::vector<double> upd_flops;       // algorithmic flops of the trailing update after panel p
     double cb_flops = 0.0;               // and of the contribution-block product
@@ -137,6 +138,8 @@ class Engine {
   FwdTask *d_ftasks_ = nullptr, *d_fheads_ = nullptr;
   std::vector<BwdTask> h_btasks_, h_bheads_;
   BwdTask *d_btasks_ = nullptr, *d_bheads_ = nullptr;
+  std::vector<PivSplit> h_psplit_;  // per pivot position: its split tile (backward partials)
+  PivSplit* d_psplit_ = nullptr;
   std::vector<TaskRange> gstages_;
   std::vector<int64_t> h_cptr_;
   std::vector<int32_t> h_cidx_;

@@ -1623,14 +1623,26 @@ void Engine::plan_solve() {
       cols = std::max(cols, (std::min(kGvChunk, h_ftasks_[k].kstop - h_ftasks_[k].kbeg) + 3) / 4);
     L.ft_cols = 2;
     while (L.ft_cols < cols) L.ft_cols *= 2;
+    // levels whose tiles all have at most kWaveCols columns in one chunk: one task per wave, a
+    // lane per row, C = the tiles' columns (rounded up to a power of two >= 8)
+    L.ft_wave = L.ft_count > 0;
+    int wcols = 1;
+    for (int64_t k = L.ft_begin; k < L.ft_begin + L.ft_count; ++k) {
+      const FwdTask& t = h_ftasks_[k];
+      if (t.nchunk != 1 || t.kstop - t.kbeg > kWaveCols) L.ft_wave = false;
+      wcols = std::max(wcols, t.kstop - t.kbeg);
+    }
+    if (L.ft_wave) {
+      L.ft_cols = 8;
+      while (L.ft_cols < wcols) L.ft_cols *= 2;
+    }
     part_slots_ = std::max(part_slots_, slots);
   }
   // backward tasks per level: column tiles of at most 64 (width rounded up to a power of two >= 8:
-  // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group,
-  // or of 16 where the level has split tiles already and fewer than 1024 tasks (a task's time is
-  // mostly its chain of dependent loads: shorter chunks shorten it, at the cost of more partial
-  // sums; a level without splits would also gain a reduction launch); a chunk has at most
-  // kBwdRows rows (the staging area)
+  // the thread layout), rows from the tile's first column to m in chunks of 32 rows per row group
+  // (at most kBwdRows rows: the staging area). Chunks of 16 rows (shorter load chains per task,
+  // more partial sums) were measured slower once the partials were added by the next level's
+  // tasks (889 against 880 us per box48 solve).
   auto tile_width = [](int jn) {
     int jw = 8;
     while (jw < jn) jw *= 2;
@@ -1639,15 +1651,8 @@ void Engine::plan_solve() {
   for (Level& L : levels_) {
     L.bt_begin = (int64_t)h_btasks_.size();
     L.bh_begin = (int64_t)h_bheads_.size();
-    int64_t slots = 0, tasks32 = 0, splits32 = 0;
-    for (int s : L.h_all)
-      for (int j0 = 0; j0 < sym_.sn_ncol[s]; j0 += kGvTile) {
-        const int rc = std::min(kBwdRows, 32 * (256 / tile_width(std::min(kGvTile, sym_.sn_ncol[s] - j0))));
-        const int64_t nch = (sym_.sn_m[s] - j0 + rc - 1) / rc;
-        tasks32 += nch;
-        splits32 += nch > 1;
-      }
-    L.bt_rows = splits32 > 0 && tasks32 < 1024 ? 16 : 32;
+    int64_t slots = 0;
+    L.bt_rows = 32;
     for (int s : L.h_all) {
       const int ncol = sym_.sn_ncol[s], m = sym_.sn_m[s];
       for (int j0 = 0; j0 < ncol; j0 += kGvTile) {
@@ -1685,6 +1690,22 @@ void Engine::plan_solve() {
     L.bt_regs = 4;
     while (L.bt_regs < rows) L.bt_regs *= 2;
     part_slots_ = std::max(part_slots_, slots);
+  }
+  // per pivot position: the split tile holding it between its level's launch and the next one
+  h_psplit_.assign(sym_.n, PivSplit{-1, 0, 0, -1});
+  for (int l = 0; l < (int)levels_.size(); ++l) {
+    const Level& L = levels_[l];
+    for (int s2 : L.h_all)
+      for (int j = 0; j < sym_.sn_ncol[s2]; ++j) h_psplit_[sym_.sn_first[s2] + j].level = l;
+    for (int64_t k = L.bh_begin; k < L.bh_begin + L.bh_count; ++k) {
+      const BwdTask& h = h_bheads_[k];
+      for (int j2 = 0; j2 < h.jn; ++j2) {
+        PivSplit& q = h_psplit_[h.first + h.j0 + j2];
+        q.slot0 = (int32_t)h.slot0;
+        q.nchunk = h.nchunk;
+        q.jj = j2;
+      }
+    }
   }
   // pull map of the forward products: for front row r (index rowoff + r), the children's U rows
   // mapped onto it, children in child-list order
@@ -1754,7 +1775,9 @@ void Engine::validate_solve_plan() const {
       bad("forward columns per thread");
     for (int64_t k = L.ft_begin; k < L.ft_begin + L.ft_count; ++k) {
       const FwdTask& t = h_ftasks_[k];
-      if ((std::min(kGvChunk, t.kstop - t.kbeg) + 3) / 4 > L.ft_cols) bad("forward task columns per thread");
+      if (L.ft_wave ? (t.nchunk != 1 || t.kstop - t.kbeg > L.ft_cols || L.ft_cols < 8)
+                    : (std::min(kGvChunk, t.kstop - t.kbeg) + 3) / 4 > L.ft_cols)
+        bad("forward task columns per thread");
     }
     if (L.bt_begin < 0 || L.bt_begin + L.bt_count > (int64_t)h_btasks_.size()) bad("backward task range");
     if (L.bh_begin < 0 || L.bh_begin + L.bh_count > (int64_t)h_bheads_.size()) bad("backward split range");
@@ -1786,6 +1809,11 @@ void Engine::validate_solve_plan() const {
   }
   for (const BwdTask& t : h_bheads_)
     if (t.chunk != 0 || t.nchunk < 2 || t.slot0 < 0 || t.slot0 + t.nchunk > part_slots_) bad("backward split head");
+  if ((int64_t)h_psplit_.size() != sym_.n) bad("split state size");
+  for (const PivSplit& q : h_psplit_)
+    if (q.level < 0 || q.level >= (int)levels_.size() || q.jj < 0 || q.jj >= kGvTile ||
+        (q.slot0 >= 0 && (q.nchunk < 2 || q.slot0 + q.nchunk > part_slots_)))
+      bad("split state entry");
 }
 
 void Engine::ensure_device() {
@@ -1814,6 +1842,7 @@ void Engine::ensure_device() {
   d_fheads_ = upload(h_fheads_);
   d_btasks_ = upload(h_btasks_);
   d_bheads_ = upload(h_bheads_);
+  d_psplit_ = upload(h_psplit_);
   d_cptr_ = upload(h_cptr_);
   d_cpull_ = upload(h_cpull_);
   d_cidx_ = upload(h_cidx_);
@@ -1829,7 +1858,7 @@ Engine::~Engine() {
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
                   (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
-                  (void*)d_tasks_, (void*)d_ftasks_, (void*)d_fheads_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_cptr_, (void*)d_cpull_,
+                  (void*)d_tasks_, (void*)d_ftasks_, (void*)d_fheads_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_psplit_, (void*)d_cptr_, (void*)d_cpull_,
                   (void*)d_cidx_})
     dfree(p);
   for (auto& L : levels_) {
@@ -2060,6 +2089,14 @@ static void launch_fwd_task(int cols, dim3 grid, hipStream_t st, A... a) {
   }
 }
 template <class T, class... A>
+static void launch_fwd_wave(int cols, dim3 grid, hipStream_t st, A... a) {
+  switch (cols) {
+    case 8: hipLaunchKernelGGL((k_fwd_wave<T, 8>), grid, dim3(256), 0, st, a...); break;
+    case 16: hipLaunchKernelGGL((k_fwd_wave<T, 16>), grid, dim3(256), 0, st, a...); break;
+    default: hipLaunchKernelGGL((k_fwd_wave<T, kWaveCols>), grid, dim3(256), 0, st, a...); break;
+  }
+}
+template <class T, class... A>
 static void launch_bwd_task(int rows, dim3 grid, hipStream_t st, A... a) {
   switch (rows) {
     case 4: hipLaunchKernelGGL((k_bwd_task<T, 4>), grid, dim3(256), 0, st, a...); break;
@@ -2083,7 +2120,9 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
     dalloc_into(nm.d_Y_, (size_t)sym_.n * nrhs * sizeof(T), "hipMalloc (solve workspace)");
     dalloc_into(nm.d_U_, (size_t)std::max<int64_t>(1, rows_total_) * nrhs * sizeof(T),
                 "hipMalloc (solve workspace)");
-    dalloc_into(nm.d_part_, (size_t)std::max<int64_t>(1, part_slots_ * nrc) * kPartSlot * sizeof(T),
+    // two halves: a backward level writes its partials into one while the next level's launch
+    // reads the previous level's from the other
+    dalloc_into(nm.d_part_, (size_t)std::max<int64_t>(1, 2 * part_slots_ * nrc) * kPartSlot * sizeof(T),
                 "hipMalloc (solve workspace)");
     nm.ws_cols_ = nrhs;
   }
@@ -2107,8 +2146,12 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   const int64_t* cptr = (const int64_t*)d_cptr_;
   const int32_t* cidx = (const int32_t*)d_cidx_;
   for (const Level& L : levels_) {
-    launch_fwd_task<T>(L.ft_cols, dim3((unsigned)L.ft_count), st, d_ftasks_ + L.ft_begin, p, G, (const T*)Xp, Y, U,
-                       part, (const PullRow*)nm.d_zpull_, (const PullRow*)d_cpull_, cidx);
+    if (L.ft_wave)
+      launch_fwd_wave<T>(L.ft_cols, dim3((unsigned)((L.ft_count + 3) / 4)), st, d_ftasks_ + L.ft_begin, (int)L.ft_count,
+                         p, G, (const T*)Xp, Y, U, (const PullRow*)nm.d_zpull_, (const PullRow*)d_cpull_, cidx);
+    else
+      launch_fwd_task<T>(L.ft_cols, dim3((unsigned)L.ft_count), st, d_ftasks_ + L.ft_begin, p, G, (const T*)Xp, Y, U,
+                         part, (const PullRow*)nm.d_zpull_, (const PullRow*)d_cpull_, cidx);
     ++nl;
     if (L.fh_count > 0) {
       hipLaunchKernelGGL(k_fwd_reduce<T>, dim3((unsigned)L.fh_count), dim3(256), 0, st, d_fheads_ + L.fh_begin, p, Y,
@@ -2122,18 +2165,25 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   tok = timing_begin(kPhaseBackward, st);
   nl = 0;
   // backward: x1 = G^T [y1; x_cb] per front, top-down, into Xp
+  // one launch per level: its tasks, plus one workgroup per split tile of the level above, which
+  // adds that level's partials for the levels below (the level's own tasks add the ones they read)
+  const int64_t half = part_slots_ * ((p + kMaxRhsChunk - 1) / kMaxRhsChunk) * (int64_t)kPartSlot;
+  auto part_of = [&](int l) { return part + (l & 1) * half; };
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
-    if (L.bt_count > 0) {
-      launch_bwd_task<T>(L.bt_regs, dim3((unsigned)L.bt_count), st, Gt, d_btasks_ + L.bt_begin, p, (const T*)Y, Xp,
-                         rpos, part);
+    const Level* up = l + 1 < (int)levels_.size() ? &levels_[l + 1] : nullptr;
+    const int64_t nh = up ? up->bh_count : 0;
+    if (L.bt_count + nh > 0) {
+      launch_bwd_task<T>(L.bt_regs, dim3((unsigned)(L.bt_count + nh)), st, Gt, d_btasks_ + L.bt_begin, (int)L.bt_count,
+                         d_bheads_ + (up ? up->bh_begin : 0), p, (const T*)Y, Xp, rpos, (const PivSplit*)d_psplit_, l,
+                         (const T*)part_of(l + 1), part_of(l));
       ++nl;
     }
-    if (L.bh_count > 0) {
-      hipLaunchKernelGGL(k_bwd_reduce<T>, dim3((unsigned)L.bh_count), dim3(256), 0, st, d_bheads_ + L.bh_begin, p, Xp,
-                         (const T*)part);
-      ++nl;
-    }
+  }
+  if (!levels_.empty() && levels_[0].bh_count > 0) {  // split tiles of the lowest level
+    hipLaunchKernelGGL(k_bwd_reduce<T>, dim3((unsigned)levels_[0].bh_count), dim3(256), 0, st,
+                       d_bheads_ + levels_[0].bh_begin, p, Xp, (const T*)part_of(0));
+    ++nl;
   }
   timing_end(tok, st, nl, sweep_bytes);
   hipLaunchKernelGGL((k_permute_out<T, double>), dim3(blocks_for(sym_.n)), dim3(256), 0, st, sym_.n, p, (const T*)Xp,

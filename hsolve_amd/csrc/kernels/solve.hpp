@@ -253,10 +253,12 @@ __global__ void k_permute_out(int n, int p, const T* __restrict__ Xp, const int3
 
 // Split products: a tile's long dimension (columns of G forward, rows of Gt backward) is cut into
 // chunks, one workgroup each. With more than one chunk, each workgroup stores its partial sums
-// (64 x 8 per group of right-hand sides) in its slot of `part`, and a second launch adds the
-// chunks' partials in chunk order and writes the result: the sum does not depend on the order
-// the workgroups ran in, and stream order makes the partials visible (no cross-workgroup
-// synchronization inside a launch). Slot index: (slot of chunk) * rhs_groups + rhs_group.
+// (64 x 8 per group of right-hand sides; forward [value][row], backward [column][value]) in its
+// slot of `part`, and a later launch adds the chunks' partials in chunk order: forward a second
+// launch of the level, backward the next level's launch (its own tasks for the rows they read,
+// extra workgroups for Xp). The sum does not depend on the order the workgroups ran in, and
+// stream order makes the partials visible (no cross-workgroup synchronization inside a launch).
+// Slot index: (slot of chunk) * rhs_groups + rhs_group.
 constexpr int kGvSplit = 256;          // columns per chunk of a split forward product, at most
 constexpr int kFwdTargetGroups = 1536;  // workgroups a forward level's grid should reach (6 per CU)
 constexpr int kPartSlot = kGvTile * kMaxRhsChunk;  // values per partial slot
@@ -432,6 +434,116 @@ __global__ void __launch_bounds__(256) k_fwd_task(const FwdTask* __restrict__ ta
   for (int c0 = kMaxRhsChunk; c0 < p; c0 += kMaxRhsChunk) rhs_group(c0);
 }
 
+// Forward tasks of the levels whose tiles have at most kWaveCols columns in one chunk: one task
+// per 64-lane wave, four per workgroup (a 256-thread workgroup per small tile leaves most of its
+// threads without work and holds the device's wave slots, which concurrent solves share). Lane =
+// row of the tile: a lane loads its row's C entries of G, the wave stages the tile's z rows
+// (right-hand side + children's update rows) into an LDS area of its own, and every lane forms its
+// row of [y1; u] alone, columns in order, so no reduction between waves is needed. The pull-map
+// and right-hand-side loads are issued before the G loads, all children's rows before their use.
+constexpr int kWaveCols = 32;
+
+template <class T, int C>
+__global__ void __launch_bounds__(256) k_fwd_wave(const FwdTask* __restrict__ tasks, int ntasks, int p,
+                                                   const T* __restrict__ G, const T* __restrict__ Xp,
+                                                   T* __restrict__ Y, T* __restrict__ U,
+                                                   const PullRow* __restrict__ zpull, const PullRow* __restrict__ cpull,
+                                                   const int32_t* __restrict__ cidx) {
+  static_assert(C <= kWaveCols && 2 * kWaveCols == 64, "staging: 32 rows x 2 halves per wave");
+  __shared__ T zs[4][kWaveCols][kMaxRhsChunk];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  const int ti = blockIdx.x * 4 + w;
+  const bool live = ti < ntasks;
+  const FwdTask tk = tasks[live ? ti : ntasks - 1];
+  const int kn = tk.kstop - tk.kbeg;
+  const int i = tk.r0 + lane;
+  const bool rin = live && i < tk.m;
+  const T* grow = G + tk.goff + (rin ? i : tk.r0);
+  const int64_t r = tk.rowoff + (rin ? i : tk.r0);
+  const bool cb_out = rin && i >= tk.ncol;
+  const int st = lane >> 1, sh = (lane & 1) * 4;  // staging: two lanes per z row, four values each
+  const bool sval = live && st < kn;
+  const int64_t zq = (int64_t)tk.first + tk.kbeg + (sval ? st : 0);
+  auto rhs_group = [&](int c0) __attribute__((always_inline)) {
+    const int pc = min(kMaxRhsChunk, p - c0);
+    // pull-map entries and right-hand side first (they need the row indices only)
+    const PullRow zr = zpull[zq];
+    const PullRow er = cpull[r];
+    T zv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zv[c] = Xp[zq * p + c0 + min(sh + c, pc - 1)];
+    T gv[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) gv[q] = grow[(int64_t)(tk.kbeg + min(q, kn - 1)) * tk.m];
+    // the inline children's rows of z row st and of row r (row 0 stands in for an absent one)
+    T za[4], zb[4], ea[kMaxRhsChunk], eb[kMaxRhsChunk];
+    {
+      const T* ua = U + (int64_t)max(zr.u0, 0) * p + c0;
+      const T* ub = U + (int64_t)max(zr.u1, 0) * p + c0;
+      const T* va = U + (int64_t)max(er.u0, 0) * p + c0;
+      const T* vb = U + (int64_t)max(er.u1, 0) * p + c0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        za[c] = ua[min(sh + c, pc - 1)];
+        zb[c] = ub[min(sh + c, pc - 1)];
+      }
+#pragma unroll
+      for (int c = 0; c < kMaxRhsChunk; ++c) {
+        ea[c] = va[min(c, pc - 1)];
+        eb[c] = vb[min(c, pc - 1)];
+      }
+    }
+    {
+      const bool ha = sval && zr.u0 >= 0, hb = sval && zr.u1 >= 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        zv[c] = ha ? zv[c] + za[c] : zv[c];
+        zv[c] = hb ? zv[c] + zb[c] : zv[c];
+      }
+      for (int q = zr.b; sval && q < zr.e; ++q) {
+        const T* u = U + (int64_t)cidx[q] * p + c0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) zv[c] += u[min(sh + c, pc - 1)];
+      }
+    }
+    // the children's sum of contribution row r: starts at +0 and never becomes -0, so adding +0 for
+    // an absent row changes nothing
+    T csum[kMaxRhsChunk];
+    {
+      const bool ha = cb_out && er.u0 >= 0, hb = cb_out && er.u1 >= 0;
+#pragma unroll
+      for (int c = 0; c < kMaxRhsChunk; ++c) csum[c] = ((T)0 + (ha ? ea[c] : (T)0)) + (hb ? eb[c] : (T)0);
+      for (int q = er.b; cb_out && q < er.e; ++q) {
+        const T* u = U + (int64_t)cidx[q] * p + c0;
+#pragma unroll
+        for (int c = 0; c < kMaxRhsChunk; ++c) csum[c] += u[min(c, pc - 1)];
+      }
+    }
+    __syncthreads();  // the previous group of right-hand sides is done with the staging area
+#pragma unroll
+    for (int c = 0; c < 4; ++c) zs[w][st][sh + c] = (sval && sh + c < pc) ? zv[c] : (T)0;
+    __syncthreads();
+    T acc[kMaxRhsChunk];
+#pragma unroll
+    for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] = (T)0;
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      if (q < kn) {
+#pragma unroll
+        for (int c = 0; c < kMaxRhsChunk; ++c) acc[c] += gv[q] * zs[w][q][c];
+      }
+    if (rin) {
+      for (int c = 0; c < pc; ++c) {
+        if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c0 + c] = acc[c];
+        else U[r * p + c0 + c] = csum[c] + acc[c];
+      }
+    }
+  };
+  rhs_group(0);
+  for (int c0 = kMaxRhsChunk; c0 < p; c0 += kMaxRhsChunk) rhs_group(c0);
+}
+
 // Second launch of a split forward level: one workgroup per split row tile (its chunk-0 task in
 // `heads`), the chunks' partials added in chunk order, written like k_fwd_task's result.
 template <class T>
@@ -495,6 +607,14 @@ struct BwdTask {
 
 constexpr int kBwdRows = 512;  // rows of a backward task at most (staged in LDS)
 
+// Per pivot position q: where its backward value stands while the level that computes it ends. A
+// pivot of a split column tile has partial sums instead (slot0: the tile's chunk-0 slot, nchunk
+// chunks, jj: its column in the tile) until the next level's launch, which adds them; level = the
+// height of its front. slot0 = -1 for the pivots of unsplit tiles.
+struct alignas(16) PivSplit {
+  int32_t slot0, nchunk, jj, level;
+};
+
 // x1 = G^T [y1; x_cb] over one task: y1 from Y (pivot rows), x_cb from Xp at the pivot positions
 // of the front's contribution rows (rows of ancestors, final already); the result goes to Xp at
 // the front's own pivot positions, or, for a split tile, to the task's partial slot. Gt is read
@@ -503,13 +623,28 @@ constexpr int kBwdRows = 512;  // rows of a backward task at most (staged in LDS
 // gathered (pivot positions, then values) and staged; the products run in row order as in one
 // pass over the rows. R >= the rows per thread of every task of the launch.
 template <class T, int R>
-__global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, const BwdTask* __restrict__ tasks, int p,
-                                                   const T* __restrict__ Y, T* Xp, const int32_t* __restrict__ rpos,
-                                                   T* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, const BwdTask* __restrict__ tasks, int ntasks,
+                                                   const BwdTask* __restrict__ heads, int p, const T* __restrict__ Y,
+                                                   T* Xp, const int32_t* __restrict__ rpos, const PivSplit* __restrict__ psplit,
+                                                   int level, const T* __restrict__ part_in, T* __restrict__ part) {
   static_assert(R >= 1 && R <= kBwdRows / 8, "rows per thread");
   __shared__ T ws[kBwdRows][kMaxRhsChunk];
   T(*red)[kMaxRhsChunk + 1] = reinterpret_cast<T(*)[kMaxRhsChunk + 1]>(&ws[0][0]);  // after the products
   static_assert(256 * (kMaxRhsChunk + 1) <= kBwdRows * kMaxRhsChunk, "reduction area inside the staging area");
+  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
+  if ((int)blockIdx.x >= ntasks) {
+    // the workgroups past the tasks: the split tiles of the level above, their partials (part_in)
+    // added in chunk order and written to Xp for the levels below this one
+    const BwdTask hd = heads[blockIdx.x - ntasks];
+    for (int idx = threadIdx.x; idx < hd.jn * p; idx += blockDim.x) {
+      const int j2 = idx % hd.jn, c = idx / hd.jn;
+      const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
+      const T v = sum_partials<T>(part_in + (hd.slot0 * nrc + q) * (int64_t)kPartSlot + j2 * kMaxRhsChunk + cc,
+                                  (int64_t)nrc * kPartSlot, hd.nchunk);
+      Xp[(int64_t)(hd.first + hd.j0 + j2) * p + c] = v;
+    }
+    return;
+  }
   const BwdTask tk = tasks[blockIdx.x];
   const int JW = tk.jw, ngrp = 256 / JW;
   const int tid = threadIdx.x, jj = tid % JW, grp = tid / JW;
@@ -527,20 +662,22 @@ __global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, cons
   T gv[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) gv[q] = g0[(int64_t)min(grp + q * ngrp, tk.in - 1) * tk.ncol];
-  const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
   // one group of at most kMaxRhsChunk right-hand sides; the first group is not inside a loop, so
   // that its value loads need not wait for the Gt loads still in flight
   auto rhs_group = [&](int c0) __attribute__((always_inline)) {
     const int pc = min(kMaxRhsChunk, p - c0);
     // rows st + 128 k of [y1; x_cb] (rows past the task's end read its last row; global pointers, so
-    // that the loads are not counted with the LDS accesses)
+    // that the loads are not counted with the LDS accesses), with the split state of the
+    // contribution rows' pivot positions
     T v[kRounds][4];
+    PivSplit ps[kRounds];
 #pragma unroll
     for (int k = 0; k < kRounds; ++k) {
       if (k * 128 >= tk.in) break;
       const int i = tk.i0 + min(st + 128 * k, tk.in - 1);
       const GlobalPtr<const T> src = i < tk.ncol ? GlobalPtr<const T>(Y) + (int64_t)(tk.first + i) * p + c0
                                                  : GlobalPtr<const T>(Xp) + (int64_t)pos[k] * p + c0;
+      ps[k] = psplit[i < tk.ncol ? tk.first + i : pos[k]];
 #pragma unroll
       for (int c = 0; c < 4; ++c) v[k][c] = src[min(sh + c, pc - 1)];
     }
@@ -551,6 +688,38 @@ __global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, cons
       const bool valid = st + 128 * k < tk.in;
 #pragma unroll
       for (int c = 0; c < 4; ++c) ws[st + 128 * k][sh + c] = (valid && sh + c < pc) ? v[k][c] : (T)0;
+    }
+    // a contribution row on a split tile of the level just above: its value is still in that
+    // level's partials (this launch's extra workgroups write it to Xp for the levels below), added
+    // here in chunk order as they do, over the staged value
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+      if (k * 128 >= tk.in) break;
+      const int t = st + 128 * k;
+      if (t < tk.in && tk.i0 + t >= tk.ncol && ps[k].level == level + 1 && ps[k].slot0 >= 0) {
+        // the row's four values are contiguous in every chunk's slot: eight chunks' loads at a time
+        const T* src = part_in + (ps[k].slot0 * (int64_t)nrc + c0 / kMaxRhsChunk) * kPartSlot +
+                       ps[k].jj * kMaxRhsChunk + sh;
+        const int64_t stride = (int64_t)nrc * kPartSlot;
+        const int n = ps[k].nchunk;
+        T a[4] = {(T)0, (T)0, (T)0, (T)0};
+#pragma unroll 1
+        for (int k0 = 0; k0 < n; k0 += 8) {
+          T x[8][4];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) x[u][c] = src[(int64_t)min(k0 + u, n - 1) * stride + c];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (k0 + u < n)
+#pragma unroll
+              for (int c = 0; c < 4; ++c) a[c] += x[u][c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (sh + c < pc) ws[t][sh + c] = a[c];
+      }
     }
     __syncthreads();
     T acc[kMaxRhsChunk];
@@ -576,7 +745,7 @@ __global__ void __launch_bounds__(256) k_bwd_task(const T* __restrict__ Gt, cons
       if (j2 >= tk.jn) continue;
       T s = (T)0;
       for (int g2 = 0; g2 < ngrp; ++g2) s += red[g2 * JW + j2][c];
-      if (slot) slot[c * kGvTile + j2] = s;
+      if (slot) slot[j2 * kMaxRhsChunk + c] = s;
       else Xp[(int64_t)(tk.first + tk.j0 + j2) * p + c0 + c] = s;
     }
   };
@@ -594,7 +763,7 @@ __global__ void __launch_bounds__(256) k_bwd_reduce(const BwdTask* __restrict__ 
   for (int idx = threadIdx.x; idx < tk.jn * p; idx += blockDim.x) {
     const int j2 = idx % tk.jn, c = idx / tk.jn;
     const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
-    const T v = sum_partials<T>(part + (tk.slot0 * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + j2,
+    const T v = sum_partials<T>(part + (tk.slot0 * nrc + q) * (int64_t)kPartSlot + j2 * kMaxRhsChunk + cc,
                                 (int64_t)nrc * kPartSlot, tk.nchunk);
     Xp[(int64_t)(tk.first + tk.j0 + j2) * p + c] = v;
   }
