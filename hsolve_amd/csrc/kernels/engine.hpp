@@ -148,7 +148,6 @@ class Engine {
   double g_flops_ = 0.0;  // algorithmic flops of the inverse panels: sum of cb ncol^2 + ncol^3 / 3
   int n_pairs_ = 0, n_ttiles_ = 0;
   int32_t *d_pairs_ = nullptr, *d_ttiles_ = nullptr, *d_cidx_ = nullptr;
-  int64_t* d_cptr_ = nullptr;
   PullRow* d_cpull_ = nullptr;
   GemmTask* d_tasks_ = nullptr;
   bool on_device_ = false;

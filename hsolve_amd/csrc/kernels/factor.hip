@@ -1843,7 +1843,6 @@ void Engine::ensure_device() {
   d_btasks_ = upload(h_btasks_);
   d_bheads_ = upload(h_bheads_);
   d_psplit_ = upload(h_psplit_);
-  d_cptr_ = upload(h_cptr_);
   d_cpull_ = upload(h_cpull_);
   d_cidx_ = upload(h_cidx_);
   for (Level& L : levels_) {
@@ -1858,7 +1857,7 @@ Engine::~Engine() {
   for (void* p : {(void*)d_first_, (void*)d_ncol_, (void*)d_m_, (void*)d_child_ptr_, (void*)d_child_list_,
                   (void*)d_relmap_, (void*)d_rows_orig_, (void*)d_rowoff_, (void*)d_panel_off_, (void*)d_cb_off_,
                   (void*)d_w_off_, (void*)d_asm_, (void*)d_ia_, (void*)d_ib_, (void*)d_pairs_, (void*)d_ttiles_,
-                  (void*)d_tasks_, (void*)d_ftasks_, (void*)d_fheads_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_psplit_, (void*)d_cptr_, (void*)d_cpull_,
+                  (void*)d_tasks_, (void*)d_ftasks_, (void*)d_fheads_, (void*)d_btasks_, (void*)d_bheads_, (void*)d_psplit_, (void*)d_cpull_,
                   (void*)d_cidx_})
     dfree(p);
   for (auto& L : levels_) {
@@ -2143,7 +2142,6 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
   int64_t tok = timing_begin(kPhaseForward, st);
   int64_t nl = 0;
   // forward: [y1; u] = G z1 + [0; u_children] per front, z gathered from the children where read
-  const int64_t* cptr = (const int64_t*)d_cptr_;
   const int32_t* cidx = (const int32_t*)d_cidx_;
   for (const Level& L : levels_) {
     if (L.ft_wave)
@@ -2155,7 +2153,7 @@ void Engine::solve_t(Numeric& nm, double* x, int64_t nrhs, int64_t ldx, hipStrea
     ++nl;
     if (L.fh_count > 0) {
       hipLaunchKernelGGL(k_fwd_reduce<T>, dim3((unsigned)L.fh_count), dim3(256), 0, st, d_fheads_ + L.fh_begin, p, Y,
-                         U, (const T*)part, cptr, cidx);
+                         U, (const T*)part, (const PullRow*)d_cpull_, cidx);
       ++nl;
     }
   }

@@ -55,16 +55,6 @@ using GlobalPtr = __attribute__((address_space(1))) T*;
 // 256-thread pass.
 static_assert(kGvChunk * 2 == 256 && kMaxRhsChunk == 8, "staging layout: 2 threads x 4 values per row");
 
-// Sum, in child order, of value c of the children's update rows mapped onto front row r (index
-// rowoff + local row): the part of a contribution row's update that comes from below.
-template <class T>
-__device__ __forceinline__ T children_sum(int64_t r, int p, int c, const T* __restrict__ U,
-                                          const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
-  T acc = (T)0;
-  for (int64_t i = cptr[r]; i < cptr[r + 1]; ++i) acc += U[(int64_t)cidx[i] * p + c];
-  return acc;
-}
-
 // L(i, j) of a pivot block, 0 at the (j+1, j) position of a 2x2 or null pivot pair (that slot of
 // the panel holds D's off-diagonal).
 template <class T>
@@ -549,19 +539,47 @@ __global__ void __launch_bounds__(256) k_fwd_wave(const FwdTask* __restrict__ ta
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_reduce(const FwdTask* __restrict__ heads, int p, T* __restrict__ Y,
                                                      T* __restrict__ U, const T* __restrict__ part,
-                                                     const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx) {
+                                                     const PullRow* __restrict__ cpull, const int32_t* __restrict__ cidx) {
+  // 64 rows x 4 value groups; a thread adds the partials of values g, g + 4 of its row (all their
+  // loads issued together) and, for a contribution row, the children's rows (inline ones loaded
+  // whatever the row, as in the products)
   const FwdTask tk = heads[blockIdx.x];
   const int nrc = (p + kMaxRhsChunk - 1) / kMaxRhsChunk;
-  for (int idx = threadIdx.x; idx < kGvTile * p; idx += blockDim.x) {
-    const int lane = idx % kGvTile, c = idx / kGvTile;
-    const int i = tk.r0 + lane;
-    if (i >= tk.m) continue;
-    const int q = c / kMaxRhsChunk, cc = c % kMaxRhsChunk;
-    const T v = sum_partials<T>(part + (tk.slot * nrc + q) * (int64_t)kPartSlot + cc * kGvTile + lane,
-                                (int64_t)nrc * kPartSlot, tk.nchunk);
-    const int64_t r = tk.rowoff + i;
-    if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c] = v;
-    else U[r * p + c] = children_sum<T>(r, p, c, U, cptr, cidx) + v;
+  const int lane = threadIdx.x % kGvTile, g = threadIdx.x / kGvTile;
+  const int i = tk.r0 + lane;
+  const bool rin = i < tk.m;
+  const int64_t r = tk.rowoff + (rin ? i : tk.r0);
+  const bool cb_out = rin && i >= tk.ncol;
+  const PullRow er = cpull[r];
+  for (int c0 = 0; c0 < p; c0 += kMaxRhsChunk) {
+    const int pc = min(kMaxRhsChunk, p - c0);
+    const int c1 = min(g, pc - 1), c2 = min(g + 4, pc - 1);
+    const T* src = part + (tk.slot * nrc + c0 / kMaxRhsChunk) * (int64_t)kPartSlot + lane;
+    const T v1 = sum_partials<T>(src + c1 * kGvTile, (int64_t)nrc * kPartSlot, tk.nchunk);
+    const T v2 = sum_partials<T>(src + c2 * kGvTile, (int64_t)nrc * kPartSlot, tk.nchunk);
+    const T* ua = U + (int64_t)max(er.u0, 0) * p + c0;
+    const T* ub = U + (int64_t)max(er.u1, 0) * p + c0;
+    const T a1 = ua[c1], a2 = ua[c2], b1 = ub[c1], b2 = ub[c2];
+    // the children's sum starts at +0 and never becomes -0: adding +0 for an absent row changes
+    // nothing
+    const bool ha = cb_out && er.u0 >= 0, hb = cb_out && er.u1 >= 0;
+    T s1 = ((T)0 + (ha ? a1 : (T)0)) + (hb ? b1 : (T)0);
+    T s2 = ((T)0 + (ha ? a2 : (T)0)) + (hb ? b2 : (T)0);
+    for (int q = er.b; cb_out && q < er.e; ++q) {
+      const T* u = U + (int64_t)cidx[q] * p + c0;
+      s1 += u[c1];
+      s2 += u[c2];
+    }
+    if (rin) {
+      if (g < pc) {
+        if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c0 + g] = v1;
+        else U[r * p + c0 + g] = s1 + v1;
+      }
+      if (g + 4 < pc) {
+        if (i < tk.ncol) Y[(int64_t)(tk.first + i) * p + c0 + g + 4] = v2;
+        else U[r * p + c0 + g + 4] = s2 + v2;
+      }
+    }
   }
 }
 
