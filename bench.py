@@ -14,10 +14,11 @@ recomputed here from the original matrices, and every slice's inertia certificat
 value = certified pairs / wall seconds over the timed steps (higher is better).
 
 Also printed, in the same JSON line: p50/p99 step latency; the plain form (FP64 factors, one
-request at a time) beside the chosen form; the roofline of the dominant kernel (the backward
-sweep's transposed products, timed with HIP events on its stream during the timed steps); and
-the CPU baseline (the oracle's SuperLU elimination of the same pencil in the same fill-reducing
-order plus one 8-RHS solve, scaled by the operation counts of a full solve).
+request at a time) beside the chosen form; the roofline of the backward sweep (its launches
+timed with HIP events on the engine's stream during the plain form's timed steps, HBM traffic
+from the committed counter profile) and of the factorization's update GEMM; and the CPU baseline
+(the oracle's C++/OpenMP multifrontal LDL^T and the same block Krylov-Schur driver on the host
+cores, solving the first spectrum slice of the window end to end).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-plain | --plain-only]
 """
