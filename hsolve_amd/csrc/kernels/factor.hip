@@ -29,6 +29,7 @@
 #include <stdexcept>
 #include <unordered_map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
@@ -879,6 +880,410 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
     if (pos) atomicAdd(&cnt[kCntPos], (unsigned long long)pos);
     if (two) atomicAdd(&cnt[kCnt2x2], (unsigned long long)two);
     if (exz) atomicAdd(&cnt[kCntExactZero], (unsigned long long)exz);
+  }
+}
+
+// Bunch-Kaufman on the nb x nb diagonal block of panel p of each front with one 64-lane wave per
+// front and no barrier between pivot steps (a workgroup of one wave: the LDS fences are waits on
+// its own LDS operations). Lane i keeps row i of the symmetric block in registers, both triangles,
+// and the two copies of every entry stay bitwise equal: each step subtracts from entry (i, j) the
+// product of two per-lane factors u_i u_j times a sign, rounded once by one FMA, which is the same
+// in row i and row j. For a 1x1 pivot d, u_i = c_i / sqrt|d| (c: the pivot column); for a 2x2 pivot
+// D = J diag(l1, l2) J^T (one Jacobi rotation, Golub and Van Loan's sym.schur2) and
+// [u1_i, u2_i] = diag(1 / sqrt|l|) J^T [c1_i, c2_i], two FMAs. Lane i = 16 r + t holds column j of
+// its row in register (j - 16 r) mod 64, so that the update needs no LDS: the factors are rotated by
+// 16, 32 and 48 lanes (ds_bpermute) and each of the 64 FMAs takes its factor u_j from lane n of its
+// own 16-lane row of one rotation (64-bit DPP row_newbcast). Interchanges only exchange the logical
+// positions of two lanes, so no data moves; the pivot column is read out of every lane's row at the
+// uniform offset j mod 16 of its four quarters and a per-lane choice of the rotation.
+// The diagonal is kept in a separate register, updated with the same FMA as its row copy. The
+// pivot search reduces over the lanes below the pivot position (ties to the lowest position, as
+// with physical interchanges). The multipliers go to LDS by physical row and logical column and
+// are written to the panel in logical order at the end, with D, the pivot types and the block
+// permutation, in the same layout as k_bk_diag. The pivot rules (alpha, the row search, the
+// null-pivot tests and their tallies) are those of k_bk_diag.
+template <class T>
+__device__ __forceinline__ double wave_max_abs_masked(T v, bool on) {
+  return wave_max_abs(on ? fabs((double)v) : 0.0);
+}
+
+// 1 / sqrt(x) for x > 0 to full double precision: the hardware estimate and two Newton steps
+// y <- y + y (1 - x y^2) / 2.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+// 1 / x to full double precision: the hardware estimate and two Newton steps y <- y + y (1 - x y).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+// acc[n] += b * (src of lane n of this lane's 16-lane row), n = 0..15: v_fmac_f64 with the
+// 64-bit DPP row broadcast. The s_nop covers the wait states between a VALU write of src and its
+// DPP read.
+__device__ __forceinline__ void fmac_row_bcast16(double (&acc)[16], double src, double b) {
+  __asm__("s_nop 1\n"
+          "v_fmac_f64_dpp %0, %16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %1, %16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %2, %16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %3, %16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %4, %16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %5, %16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %6, %16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %7, %16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %8, %16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %9, %16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %10, %16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %11, %16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %12, %16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %13, %16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %14, %16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+          "v_fmac_f64_dpp %15, %16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+          : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),
+            "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13]),
+            "+v"(acc[14]), "+v"(acc[15])
+          : "v"(src), "v"(b));
+}
+
+// v of lanes (lane + 16), (lane + 32) and (lane + 48) mod 64, by the gfx950 row swaps:
+// permlane32_swap(a, b) = ([a0 a1 b0 b1], [a2 a3 b2 b3]) and permlane16_swap(a, b) =
+// ([a0 b0 a2 b2], [a1 b1 a3 b3]) in 16-lane rows.
+__device__ __forceinline__ void rotate_rows(unsigned v, bool lower, unsigned& r1, unsigned& r2, unsigned& r3) {
+  const auto h = __builtin_amdgcn_permlane32_swap(v, v, false, false);  // [v0 v1 v0 v1], [v2 v3 v2 v3]
+  r2 = lower ? h[1] : h[0];                                             // [v2 v3 v0 v1]
+  const auto e = __builtin_amdgcn_permlane16_swap(v, v, false, false);  // [v0 v0 v2 v2], [v1 v1 v3 v3]
+  r1 = __builtin_amdgcn_permlane16_swap(e[1], r2, false, false)[0];     // [v1 v2 v3 v0]
+  r3 = __builtin_amdgcn_permlane16_swap(r2, e[0], false, false)[1];     // [v3 v0 v1 v2]
+}
+__device__ __forceinline__ void rotate_rows(double v, bool lower, double& r1, double& r2, double& r3) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  unsigned l1, l2, l3, h1, h2, h3;
+  rotate_rows((unsigned)bits, lower, l1, l2, l3);
+  rotate_rows((unsigned)(bits >> 32), lower, h1, h2, h3);
+  r1 = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
+  r2 = __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
+  r3 = __longlong_as_double((long long)(((unsigned long long)h3 << 32) | l3));
+}
+
+// Maximum of a non-negative v over the 64 lanes, returned to every lane: the maximum of the high
+// words by DPP (v_max_u32 with row_shr 1, 2, 4, 8 and row_bcast 15 / 31), the low word of the
+// lowest lane holding it, and a ballot that no lane with the same high word holds a larger value
+// (else the full 64-bit reduction).
+__device__ __forceinline__ double wave_max_nonneg(double v) {
+  int unused;
+  return wave_argmax_nonneg(v, unused);
+}
+
+constexpr int kBkWaveLoaders = 8;  // waves of k_bk_wave that load and store the block
+
+__global__ void __launch_bounds__(64 * kBkWaveLoaders) k_bk_wave(DevSym S, DevNum<double> N,
+                                                                 const int32_t* __restrict__ list, int p, int nb,
+                                                                 double null_tol, int detect) {
+  typedef double T;
+  N = N.at(blockIdx.z);
+  __shared__ T F[64 * kBkLd];  // the block's lower triangle, then the multipliers [physical row][logical column]
+  __shared__ T sdv[64], sdof[64];
+  __shared__ int sphys[64];
+  __shared__ unsigned long long spairs;
+  const int f = blockIdx.x;
+  const int s = list[f];
+  const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
+  const int c0 = p * nb;
+  if (c0 >= ncol) return;
+  const int w = min(nb, ncol - c0);
+  T* P = N.L + S.panel_off[s];
+  const int lane = threadIdx.x % 64;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  {
+    // the lower triangle by columns, kBkWaveLoaders columns at a time (loads issued before stores)
+    const T* src = P + (int64_t)c0 * m + c0 + min(lane, w - 1);
+    T v[64 / kBkWaveLoaders];
+#pragma unroll
+    for (int t = 0; t < 64 / kBkWaveLoaders; ++t) v[t] = src[(int64_t)min(wv + kBkWaveLoaders * t, w - 1) * m];
+#pragma unroll
+    for (int t = 0; t < 64 / kBkWaveLoaders; ++t) {
+      const int j = wv + kBkWaveLoaders * t;
+      F[lane + j * kBkLd] = (lane >= j && lane < w && j < w) ? v[t] : (T)0;
+    }
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const int rr = lane >> 4;  // this lane's 16-lane row: register R holds column (R + 16 rr) mod 64
+    const bool lower = lane < 32;
+    typedef T Quarter __attribute__((ext_vector_type(16)));
+    Quarter q0, q1, q2, q3;  // registers 16 s .. 16 s + 15 of the row in qs
+    auto load_quarter = [&](Quarter& q, int sq) {
+#pragma unroll
+      for (int n = 0; n < 16; ++n) {
+        const int j = (16 * (sq + rr) + n) & 63;
+        q[n] = lane >= j ? F[lane + j * kBkLd] : F[j + lane * kBkLd];
+      }
+    };
+    load_quarter(q0, 0);
+    load_quarter(q1, 1);
+    load_quarter(q2, 2);
+    load_quarter(q3, 3);
+    T col = F[lane];  // column 0 (lane 0's row): the first pivot column
+    // F holds the multipliers from here on (this wave's LDS operations complete in order)
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // column j of the row (j uniform): register (j - 16 rr) mod 64, read with a uniform branch on
+    // j mod 16 (the quarters live in separately allocated registers, which a register-indexed read
+    // cannot address)
+    auto pick = [&](int j) -> T {
+      j = __builtin_amdgcn_readfirstlane(j);
+      T a, b, c, d;
+      switch (j & 15) {
+#define HSG_BK_PICK_CASE(n) \
+  case n:                   \
+    a = q0[n];              \
+    b = q1[n];              \
+    c = q2[n];              \
+    d = q3[n];              \
+    break;
+        HSG_BK_PICK_CASE(0) HSG_BK_PICK_CASE(1) HSG_BK_PICK_CASE(2) HSG_BK_PICK_CASE(3)
+        HSG_BK_PICK_CASE(4) HSG_BK_PICK_CASE(5) HSG_BK_PICK_CASE(6) HSG_BK_PICK_CASE(7)
+        HSG_BK_PICK_CASE(8) HSG_BK_PICK_CASE(9) HSG_BK_PICK_CASE(10) HSG_BK_PICK_CASE(11)
+        HSG_BK_PICK_CASE(12) HSG_BK_PICK_CASE(13) HSG_BK_PICK_CASE(14) default: HSG_BK_PICK_CASE(15)
+#undef HSG_BK_PICK_CASE
+      }
+      const int sq = ((j >> 4) - rr) & 3;
+      return sq < 2 ? (sq == 0 ? a : b) : (sq == 2 ? c : d);
+    };
+    // row += b * u_j over all columns j (u_j: the factor of the lane holding column j; r1..r3: u
+    // rotated by 16, 32, 48 lanes)
+    auto apply = [&](T u, T r1, T r2, T r3, T b) {
+      auto one = [&](Quarter& q, T src) {
+        T acc[16];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) acc[n] = q[n];
+        fmac_row_bcast16(acc, src, b);
+#pragma unroll
+        for (int n = 0; n < 16; ++n) q[n] = acc[n];
+      };
+      one(q0, u);
+      one(q1, r1);
+      one(q2, r2);
+      one(q3, r3);
+    };
+    const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
+    const bool det = detect != 0;
+    unsigned long long pairs = 0;  // bit j: position j starts a 2x2 step
+    int at = lane;                 // this lane's logical position
+    // pivot data of position `lane`: D entries, pivot type, and the step's inertia counts (neg,
+    // zero, pos: 2 bits each; bit 6: an exactly zero pivot)
+    T my_dv = 0, my_dof = 0;
+    int my_pt = kPiv1, my_code = 0;
+    // the update of the previous step, applied to the row at the start of the next one: one term
+    // (u1, b1) after a 1x1 pivot, two (u2, b2 first) after a 2x2 pivot; `col` already includes it
+    int npend = 0;
+    T u1 = 0, b1 = 0, u2 = 0, b2 = 0;
+    T x1 = 0, y1 = 0, z1 = 0, x2 = 0, y2 = 0, z2 = 0;  // u1, u2 rotated by 16, 32, 48 lanes
+    int k = 0, pk = 0;  // pk: the lane at position k
+    while (k < w) {
+      if (npend == 2) apply(u2, x2, y2, z2, b2);
+      if (npend >= 1) apply(u1, x1, y1, z1, b1);
+      // ---- column k (exact: the pending update is included), its diagonal and its maximum below
+      const T fk = col;
+      const double dk = readlane_of(fk, pk);
+      const double absakk = fabs(dk);
+      const bool below = at > k && at < w;
+      const double colmax = wave_max_nonneg(below ? fabs(fk) : 0.0);
+      int kn;
+      if (__builtin_expect(absakk >= kBkAlpha * colmax && dk != 0.0 && !(det && absakk <= tau), 1)) {
+        // ---- the common step: a 1x1 pivot in place, not small enough for the null-pivot test
+        const double y = rsqrt_nr(absakk);  // 1 / sqrt|d|
+        const double inv = dk < 0 ? -(y * y) : y * y;
+        const bool mine = lane == k;
+        my_dv = mine ? dk : my_dv;
+        my_dof = mine ? 0.0 : my_dof;
+        my_pt = mine ? (int)kPiv1 : my_pt;
+        my_code = mine ? (dk < 0 ? 1 : 16) : my_code;
+        const T ci = below ? fk : 0.0;
+        F[lane + k * kBkLd] = ci * inv;  // multiplier of this row (zero outside the rows below)
+        u1 = ci * y;
+        b1 = dk < 0 ? u1 : -u1;
+        npend = 1;
+        kn = k + 1;
+      } else {
+        // ---- the general step: the row search, interchanges, 2x2 pivots, null pivots
+        int imax;
+        wave_argmax_nonneg(below ? fabs(fk) : 0.0, imax);
+        {
+          // ties: the lowest position among the lanes holding the maximum
+          const unsigned long long hold = __ballot(below && fabs(fk) == colmax);
+          if (colmax > 0.0 && __popcll(hold) > 1) {
+            int bp = 1 << 30;
+            for (unsigned long long h = hold; h; h &= h - 1) {
+              const int l = __ffsll((long long)h) - 1;
+              const int pl = __builtin_amdgcn_readlane(at, l);
+              if (pl < bp) {
+                bp = pl;
+                imax = l;
+              }
+            }
+          }
+        }
+        if (!(colmax > 0.0)) imax = -1;
+        int kind = 1;
+        bool swapped = false;
+        T ri = 0;  // column imax, when the row search reads it
+        if (!(absakk >= kBkAlpha * colmax) && imax >= 0) {
+          ri = pick(imax);
+          const double rowmax = wave_max_abs_masked(ri, at >= k && at < w && lane != imax);
+          if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
+            kind = 1;
+          } else if (fabs(readlane_of(ri, imax)) >= kBkAlpha * rowmax) {
+            kind = 1;
+            swapped = true;
+          } else {
+            kind = 2;
+            swapped = __builtin_amdgcn_readlane(at, imax) != k + 1;
+          }
+        }
+        if (swapped) {
+          // positions kk and at(imax) exchange their lanes
+          const int kk = kind == 1 ? k : k + 1;
+          const int q = __builtin_amdgcn_readlane(at, imax);
+          const int lkk = __ffsll((long long)__ballot(at == kk)) - 1;
+          if (lane == lkk) at = q;
+          if (lane == imax) at = kk;
+        }
+        kn = k + kind;
+        if (kind == 1) {
+          const int piv = swapped ? imax : pk;
+          const T c = swapped ? ri : fk;
+          const double d = swapped ? readlane_of(ri, imax) : dk;
+          const bool low = at > k && at < w;  // rows below the pivot, after the interchange
+          bool null_piv = d == 0.0;
+          if (det && fabs(d) <= tau) {
+            const double mbl = swapped ? wave_max_abs_masked(c, low) : colmax;
+            if (mbl <= tau) {
+              double mb = 0.0;
+              for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs(P[(int64_t)(c0 + piv) * m + r2]));
+              mb = wave_max_abs(mb);
+              if (mb <= tau) null_piv = true;
+            }
+          }
+          const double y = null_piv ? 0.0 : rsqrt_nr(fabs(d));  // 1 / sqrt|d|
+          const double inv = d < 0 ? -(y * y) : y * y;
+          const bool mine = lane == k;
+          my_dv = mine ? d : my_dv;
+          my_dof = mine ? 0.0 : my_dof;
+          my_pt = mine ? (null_piv ? (int)kPivNull : (int)kPiv1) : my_pt;
+          my_code = mine ? (null_piv ? (4 | (d == 0.0 ? 64 : 0)) : (d < 0 ? 1 : 16)) : my_code;
+          const T ci = low ? c : 0.0;
+          F[lane + k * kBkLd] = ci * inv;
+          u1 = ci * y;
+          b1 = d < 0 ? u1 : -u1;
+          npend = 1;
+        } else {
+          const int pv2 = imax;  // at position k + 1 now
+          const T c1 = fk, c2 = ri;
+          const double a = dk, b = readlane_of(c1, pv2), cc = readlane_of(c2, pv2);
+          const double dt = a * cc - b * b;
+          const double mean = 0.5 * (a + cc), rad = sqrt(0.25 * (a - cc) * (a - cc) + b * b);
+          const double e1 = mean - rad, e2 = mean + rad;
+          const bool low = at > k + 1 && at < w;
+          const bool maybe_null = det && (fabs(e1) <= tau || fabs(e2) <= tau);
+          bool coupled = false;
+          if (maybe_null) coupled = wave_max_abs(low ? fmax(fabs(c1), fabs(c2)) : 0.0) > tau;
+          int code = 0;
+          for (double e : {e1, e2}) {
+            const bool z = (det && fabs(e) <= tau && !coupled) || e == 0.0;
+            code += z ? 4 : (e < 0 ? 1 : 16);
+          }
+          const bool dead = (code & 12) != 0 || dt == 0.0;
+          const double rdt = dead ? 0.0 : rcp_nr(dt);
+          const double i11 = cc * rdt, i12 = -b * rdt, i22 = a * rdt;
+          pairs |= 1ull << k;
+          const bool mine = lane == k, next = lane == k + 1;
+          my_dv = mine ? a : (next ? cc : my_dv);
+          my_dof = mine ? b : (next ? 0.0 : my_dof);
+          my_pt = mine ? (dead ? (int)kPivNull : (int)kPiv2First) : (next ? (dead ? (int)kPivNull : (int)kPiv2Second) : my_pt);
+          my_code = mine ? (code | (dt == 0.0 ? 64 : 0)) : (next ? 0 : my_code);
+          const T w1 = low ? c1 : 0.0, w2 = low ? c2 : 0.0;
+          F[lane + k * kBkLd] = w1 * i11 + w2 * i12;
+          F[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
+          // D = J diag(l1, l2) J^T, J = [cs sn; -sn cs] (uniform values)
+          double cs = 1.0, sn = 0.0, l1 = a, l2 = cc;
+          if (b != 0.0) {
+            const double tq = (cc - a) * (0.5 * rcp_nr(b));
+            const double h = fma(tq, tq, 1.0);
+            const double t = (tq >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(tq) + h * rsqrt_nr(h));
+            cs = rsqrt_nr(fma(t, t, 1.0));
+            sn = t * cs;
+            l1 = a - t * b;
+            l2 = cc + t * b;
+          }
+          const bool upd = !dead && l1 != 0.0 && l2 != 0.0;
+          const double s1 = upd ? rsqrt_nr(fabs(l1)) : 0.0, s2 = upd ? rsqrt_nr(fabs(l2)) : 0.0;
+          u1 = (cs * w1 - sn * w2) * s1;
+          u2 = (sn * w1 + cs * w2) * s2;
+          b1 = l1 < 0.0 ? u1 : -u1;
+          b2 = l2 < 0.0 ? u2 : -u2;
+          npend = 2;
+        }
+      }
+      if (kn < w) {
+        // the next pivot column: the row's column of the lane at position kn, with this step's update
+        // formed by the same FMAs that the row gets at the start of the next step
+        pk = __ffsll((long long)__ballot(at == kn)) - 1;
+        T cn = pick(pk);
+        if (npend == 2) cn = fma(readlane_of(u2, pk), b2, cn);
+        col = fma(readlane_of(u1, pk), b1, cn);
+        rotate_rows(u1, lower, x1, y1, z1);
+        if (npend == 2) rotate_rows(u2, lower, x2, y2, z2);
+      }
+      k = kn;
+    }
+    if (lane < w) sphys[at] = lane;
+    sdv[lane] = my_dv;
+    sdof[lane] = my_dof;
+    if (lane < w) {
+      N.dval[first + c0 + lane] = my_dv;
+      N.doff[first + c0 + lane] = my_dof;
+      N.ptype[first + c0 + lane] = (int8_t)my_pt;
+    }
+    if (lane == 0) spairs = pairs;
+    // tallies: the steps' counts summed over the positions (bit b of each 2-bit count by ballot)
+    const int code = lane < w ? my_code : 0;
+    auto count2 = [&](int sh) -> int {
+      return __popcll(__ballot((code >> sh) & 1)) + 2 * __popcll(__ballot((code >> (sh + 1)) & 1));
+    };
+    const int neg = count2(0), zero = count2(2), pos = count2(4);
+    const int exz = __popcll(__ballot((code >> 6) & 1));
+    const int two = __popcll(pairs);
+    if (lane == 0) {
+      unsigned long long* cnt = N.cnt;
+      if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
+      if (zero) atomicAdd(&cnt[kCntZero], (unsigned long long)zero);
+      if (pos) atomicAdd(&cnt[kCntPos], (unsigned long long)pos);
+      if (two) atomicAdd(&cnt[kCnt2x2], (unsigned long long)two);
+      if (exz) atomicAdd(&cnt[kCntExactZero], (unsigned long long)exz);
+    }
+  }
+  __syncthreads();
+  // ---- write back L11 and D in logical order (column j by wave j mod kBkWaveLoaders), the permutation
+  const unsigned long long pairs = spairs;
+  const int ph = lane < w ? sphys[lane] : 0;
+  for (int j = wv; j < w; j += kBkWaveLoaders) {
+    if (lane >= j && lane < w) {
+      const T v = lane == j ? sdv[j] : ((lane == j + 1 && ((pairs >> j) & 1)) ? sdof[j] : F[ph + j * kBkLd]);
+      P[(int64_t)(c0 + j) * m + c0 + lane] = v;
+    }
+  }
+  if (wv == 0 && lane < w) {
+    N.piv[first + c0 + lane] = c0 + ph;
+    N.ipiv[first + c0 + ph] = c0 + lane;
+    N.bperm[(int64_t)f * nb + lane] = ph;
   }
 }
 
@@ -1961,8 +2366,12 @@ void Engine::factorize_t(Numeric& nm, int Z, const BatchValues& bv, double null_
       const int cpg = std::max(1, kThreads / nb);
       const int nsw = (L.large_max_ncol + cpg - 1) / cpg;
       for (int p = 0; p < npan; ++p) {
-        HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large, 1, Z), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb,
-                           null_tol, detect);
+        if constexpr (std::is_same<T, double>::value)
+          HSG_LAUNCH_COUNTED(k_bk_wave, dim3(L.n_large, 1, Z), dim3(64 * kBkWaveLoaders), 0, st, S, N, L.d_large, p,
+                             nb, null_tol, detect);
+        else
+          HSG_LAUNCH_COUNTED(k_bk_diag<T>, dim3(L.n_large, 1, Z), dim3(64 * kBkWaves), 0, st, S, N, L.d_large, p, nb,
+                             null_tol, detect);
         HSG_LAUNCH_COUNTED(k_panel_apply<T>, dim3(nrc + nsw, L.n_large, Z), dim3(kThreads), shm_pa, st, S, N, L.d_large,
                            p, nb);
         const int rem = L.large_maxm - (p + 1) * nb;
