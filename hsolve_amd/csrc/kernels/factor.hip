@@ -904,7 +904,8 @@ __global__ void __launch_bounds__(64 * kBkWaves) k_bk_diag(DevSym S, DevNum<T> N
 // null-pivot tests and their tallies) are those of k_bk_diag.
 template <class T>
 __device__ __forceinline__ double wave_max_abs_masked(T v, bool on) {
-  return wave_max_abs(on ? fabs((double)v) : 0.0);
+  int unused;
+  return wave_argmax_nonneg(on ? fabs((double)v) : 0.0, unused);
 }
 
 // 1 / sqrt(x) for x > 0 to full double precision: the hardware estimate and two Newton steps
@@ -953,24 +954,31 @@ __device__ __forceinline__ void fmac_row_bcast16(double (&acc)[16], double src, 
           : "v"(src), "v"(b));
 }
 
-// v of lanes (lane + 16), (lane + 32) and (lane + 48) mod 64, by the gfx950 row swaps:
-// permlane32_swap(a, b) = ([a0 a1 b0 b1], [a2 a3 b2 b3]) and permlane16_swap(a, b) =
-// ([a0 b0 a2 b2], [a1 b1 a3 b3]) in 16-lane rows.
-__device__ __forceinline__ void rotate_rows(unsigned v, bool lower, unsigned& r1, unsigned& r2, unsigned& r3) {
-  const auto h = __builtin_amdgcn_permlane32_swap(v, v, false, false);  // [v0 v1 v0 v1], [v2 v3 v2 v3]
-  r2 = lower ? h[1] : h[0];                                             // [v2 v3 v0 v1]
-  const auto e = __builtin_amdgcn_permlane16_swap(v, v, false, false);  // [v0 v0 v2 v2], [v1 v1 v3 v3]
-  r1 = __builtin_amdgcn_permlane16_swap(e[1], r2, false, false)[0];     // [v1 v2 v3 v0]
-  r3 = __builtin_amdgcn_permlane16_swap(r2, e[0], false, false)[1];     // [v3 v0 v1 v2]
+// An upper bound of the maximum of a non-negative v over the 64 lanes, returned to every lane: the
+// maximum of the high words by DPP with all low-word bits set (at least every lane's value; a NaN,
+// which fails every comparison, when the maximum is infinite).
+__device__ __forceinline__ double wave_max_nonneg_bound(double v) {
+  unsigned m = (unsigned)((unsigned long long)__double_as_longlong(v) >> 32);
+#define HSG_DPP_UMAX_STEP(ctrl, rmask) \
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, ctrl, rmask, 0xf, true));
+  HSG_DPP_UMAX_STEP(0x111, 0xf)  // row_shr:1
+  HSG_DPP_UMAX_STEP(0x112, 0xf)  // row_shr:2
+  HSG_DPP_UMAX_STEP(0x114, 0xf)  // row_shr:4
+  HSG_DPP_UMAX_STEP(0x118, 0xf)  // row_shr:8
+  HSG_DPP_UMAX_STEP(0x142, 0xa)  // row_bcast:15 into rows 1, 3
+  HSG_DPP_UMAX_STEP(0x143, 0xc)  // row_bcast:31 into rows 2, 3
+#undef HSG_DPP_UMAX_STEP
+  const unsigned mh = (unsigned)__builtin_amdgcn_readlane((int)m, 63);
+  return __longlong_as_double((long long)(((unsigned long long)mh << 32) | 0xffffffffull));
 }
-__device__ __forceinline__ void rotate_rows(double v, bool lower, double& r1, double& r2, double& r3) {
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-  unsigned l1, l2, l3, h1, h2, h3;
-  rotate_rows((unsigned)bits, lower, l1, l2, l3);
-  rotate_rows((unsigned)(bits >> 32), lower, h1, h2, h3);
-  r1 = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
-  r2 = __longlong_as_double((long long)(((unsigned long long)h2 << 32) | l2));
-  r3 = __longlong_as_double((long long)(((unsigned long long)h3 << 32) | l3));
+
+// v of lane (lane + 16 s) mod 64
+__device__ __forceinline__ double rotate_lanes(double v, int lane, int s) {
+  const int addr = ((lane + 16 * s) & 63) * 4;
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)bits);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(bits >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Maximum of a non-negative v over the 64 lanes, returned to every lane: the maximum of the high
@@ -990,9 +998,7 @@ __global__ void __launch_bounds__(64 * kBkWaveLoaders) k_bk_wave(DevSym S, DevNu
   typedef double T;
   N = N.at(blockIdx.z);
   __shared__ T F[64 * kBkLd];  // the block's lower triangle, then the multipliers [physical row][logical column]
-  __shared__ T sdv[64], sdof[64];
-  __shared__ int sphys[64];
-  __shared__ unsigned long long spairs;
+  __shared__ T sdof[64];
   const int f = blockIdx.x;
   const int s = list[f];
   const int m = S.sn_m[s], ncol = S.sn_ncol[s], first = S.sn_first[s];
@@ -1015,32 +1021,35 @@ __global__ void __launch_bounds__(64 * kBkWaveLoaders) k_bk_wave(DevSym S, DevNu
     }
   }
   __syncthreads();
-  if (wv == 0) {
-    const int rr = lane >> 4;  // this lane's 16-lane row: register R holds column (R + 16 rr) mod 64
-    const bool lower = lane < 32;
-    typedef T Quarter __attribute__((ext_vector_type(16)));
-    Quarter q0, q1, q2, q3;  // registers 16 s .. 16 s + 15 of the row in qs
-    auto load_quarter = [&](Quarter& q, int sq) {
+  // the other waves end here and give their registers back to the CU (a workgroup of eight waves
+  // at this register count would fill it); the pivot steps and the write-back are wave 0's
+  if (wv != 0) return;
+  const int rr = lane >> 4;  // this lane's 16-lane row: register R holds column (R + 16 rr) mod 64
+  typedef T Quarter __attribute__((ext_vector_type(16)));
+  Quarter q0, q1, q2, q3;  // registers 16 s .. 16 s + 15 of the row in qs
+  auto load_quarter = [&](Quarter& q, int sq) {
 #pragma unroll
-      for (int n = 0; n < 16; ++n) {
-        const int j = (16 * (sq + rr) + n) & 63;
-        q[n] = lane >= j ? F[lane + j * kBkLd] : F[j + lane * kBkLd];
-      }
-    };
-    load_quarter(q0, 0);
-    load_quarter(q1, 1);
-    load_quarter(q2, 2);
-    load_quarter(q3, 3);
-    T col = F[lane];  // column 0 (lane 0's row): the first pivot column
-    // F holds the multipliers from here on (this wave's LDS operations complete in order)
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // column j of the row (j uniform): register (j - 16 rr) mod 64, read with a uniform branch on
-    // j mod 16 (the quarters live in separately allocated registers, which a register-indexed read
-    // cannot address)
-    auto pick = [&](int j) -> T {
-      j = __builtin_amdgcn_readfirstlane(j);
-      T a, b, c, d;
-      switch (j & 15) {
+    for (int n = 0; n < 16; ++n) {
+      const int j = (16 * (sq + rr) + n) & 63;
+      q[n] = lane >= j ? F[lane + j * kBkLd] : F[j + lane * kBkLd];
+    }
+  };
+  load_quarter(q0, 0);
+  load_quarter(q1, 1);
+  load_quarter(q2, 2);
+  load_quarter(q3, 3);
+  T col = F[lane];  // column 0 (lane 0's row): the first pivot column
+  // F holds the multipliers from here on (this wave's LDS operations complete in order)
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // lanes of 16-lane row r (a uniform 64-bit mask, used as the lane condition of a select)
+  auto row_lanes = [](int r) -> unsigned long long { return 0xffffull << (16 * (r & 3)); };
+  // column j of the row (j uniform): register (j - 16 rr) mod 64, read with a uniform branch on
+  // j mod 16 (the quarters live in separately allocated registers, which a register-indexed read
+  // cannot address); the lanes of row r take it from quarter (j / 16 - r) mod 4
+  auto pick = [&](int j) -> T {
+    j = __builtin_amdgcn_readfirstlane(j);
+    T a, b, c, d;
+    switch (j & 15) {
 #define HSG_BK_PICK_CASE(n) \
   case n:                   \
     a = q0[n];              \
@@ -1048,239 +1057,261 @@ __global__ void __launch_bounds__(64 * kBkWaveLoaders) k_bk_wave(DevSym S, DevNu
     c = q2[n];              \
     d = q3[n];              \
     break;
-        HSG_BK_PICK_CASE(0) HSG_BK_PICK_CASE(1) HSG_BK_PICK_CASE(2) HSG_BK_PICK_CASE(3)
-        HSG_BK_PICK_CASE(4) HSG_BK_PICK_CASE(5) HSG_BK_PICK_CASE(6) HSG_BK_PICK_CASE(7)
-        HSG_BK_PICK_CASE(8) HSG_BK_PICK_CASE(9) HSG_BK_PICK_CASE(10) HSG_BK_PICK_CASE(11)
-        HSG_BK_PICK_CASE(12) HSG_BK_PICK_CASE(13) HSG_BK_PICK_CASE(14) default: HSG_BK_PICK_CASE(15)
+      HSG_BK_PICK_CASE(0) HSG_BK_PICK_CASE(1) HSG_BK_PICK_CASE(2) HSG_BK_PICK_CASE(3)
+      HSG_BK_PICK_CASE(4) HSG_BK_PICK_CASE(5) HSG_BK_PICK_CASE(6) HSG_BK_PICK_CASE(7)
+      HSG_BK_PICK_CASE(8) HSG_BK_PICK_CASE(9) HSG_BK_PICK_CASE(10) HSG_BK_PICK_CASE(11)
+      HSG_BK_PICK_CASE(12) HSG_BK_PICK_CASE(13) HSG_BK_PICK_CASE(14) default: HSG_BK_PICK_CASE(15)
 #undef HSG_BK_PICK_CASE
-      }
-      const int sq = ((j >> 4) - rr) & 3;
-      return sq < 2 ? (sq == 0 ? a : b) : (sq == 2 ? c : d);
-    };
-    // row += b * u_j over all columns j (u_j: the factor of the lane holding column j; r1..r3: u
-    // rotated by 16, 32, 48 lanes)
-    auto apply = [&](T u, T r1, T r2, T r3, T b) {
-      auto one = [&](Quarter& q, T src) {
-        T acc[16];
+    }
+    const int hi = j >> 4;
+    T v = __builtin_amdgcn_inverse_ballot_w64(row_lanes(hi + 2)) ? c : d;
+    v = __builtin_amdgcn_inverse_ballot_w64(row_lanes(hi + 3)) ? b : v;
+    return __builtin_amdgcn_inverse_ballot_w64(row_lanes(hi)) ? a : v;
+  };
+  // quarter q += b * u_j over its columns j (src: u rotated so that lane n of each 16-lane row
+  // holds the factor of the column of register n)
+  auto apply_quarter = [&](Quarter& q, T src, T b) {
+    T acc[16];
 #pragma unroll
-        for (int n = 0; n < 16; ++n) acc[n] = q[n];
-        fmac_row_bcast16(acc, src, b);
+    for (int n = 0; n < 16; ++n) acc[n] = q[n];
+    fmac_row_bcast16(acc, src, b);
 #pragma unroll
-        for (int n = 0; n < 16; ++n) q[n] = acc[n];
-      };
-      one(q0, u);
-      one(q1, r1);
-      one(q2, r2);
-      one(q3, r3);
-    };
-    const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
-    const bool det = detect != 0;
-    unsigned long long pairs = 0;  // bit j: position j starts a 2x2 step
-    int at = lane;                 // this lane's logical position
-    // pivot data of position `lane`: D entries, pivot type, and the step's inertia counts (neg,
-    // zero, pos: 2 bits each; bit 6: an exactly zero pivot)
-    T my_dv = 0, my_dof = 0;
-    int my_pt = kPiv1, my_code = 0;
-    // the update of the previous step, applied to the row at the start of the next one: one term
-    // (u1, b1) after a 1x1 pivot, two (u2, b2 first) after a 2x2 pivot; `col` already includes it
-    int npend = 0;
-    T u1 = 0, b1 = 0, u2 = 0, b2 = 0;
-    T x1 = 0, y1 = 0, z1 = 0, x2 = 0, y2 = 0, z2 = 0;  // u1, u2 rotated by 16, 32, 48 lanes
-    int k = 0, pk = 0;  // pk: the lane at position k
-    while (k < w) {
-      if (npend == 2) apply(u2, x2, y2, z2, b2);
-      if (npend >= 1) apply(u1, x1, y1, z1, b1);
-      // ---- column k (exact: the pending update is included), its diagonal and its maximum below
-      const T fk = col;
-      const double dk = readlane_of(fk, pk);
-      const double absakk = fabs(dk);
-      const bool below = at > k && at < w;
+    for (int n = 0; n < 16; ++n) q[n] = acc[n];
+  };
+  const double tau = null_tol * as_double_bits(N.cnt[kCntKnorm]);
+  const bool det = detect != 0;
+  const unsigned long long valid = w == 64 ? ~0ull : ((1ull << w) - 1);  // lanes of the block's rows
+  unsigned long long done = 0;   // lanes of the positions before k
+  unsigned long long pairs = 0;  // bit j: position j starts a 2x2 step
+  int at = lane;                 // this lane's logical position
+  int lop = lane;                // the lane at logical position `lane`
+  // pivot data of position `lane`: D entries, pivot type, and the step's inertia counts (neg,
+  // zero, pos: 2 bits each; bit 6: an exactly zero pivot)
+  T my_dv = 0, my_dof = 0;
+  int my_pt = kPiv1, my_code = 0;
+  // the previous step's update, applied to the row during the next step (quarter 0 first, the
+  // others after the column maximum): one term (u1, b1) after a 1x1 pivot, two (u2, b2
+  // first) after a 2x2 pivot; `col` already includes it. x, y, z: u rotated by 16, 32, 48 lanes.
+  int npend = 0;
+  T u1 = 0, b1 = 0, u2 = 0, b2 = 0;
+  T x1 = 0, y1 = 0, z1 = 0, x2 = 0, y2 = 0, z2 = 0;
+  auto apply_rest = [&]() {
+    if (npend == 2) {
+      apply_quarter(q1, x2, b2);
+      apply_quarter(q2, y2, b2);
+      apply_quarter(q3, z2, b2);
+    }
+    if (npend >= 1) {
+      apply_quarter(q1, x1, b1);
+      apply_quarter(q2, y1, b1);
+      apply_quarter(q3, z1, b1);
+    }
+  };
+  int k = 0, pk = 0;  // pk: the lane at position k
+  while (k < w) {
+    if (npend == 2) apply_quarter(q0, u2, b2);
+    if (npend >= 1) apply_quarter(q0, u1, b1);
+    // ---- column k (exact: the pending update is included), its diagonal and its maximum below
+    const unsigned long long bmask = valid & ~done & ~(1ull << pk);  // lanes of the rows below position k
+    const bool below = __builtin_amdgcn_inverse_ballot_w64(bmask);
+    const T fk = col;
+    const double dk = readlane_of(fk, pk);
+    const double absakk = fabs(dk);
+    const double cbound = wave_max_nonneg_bound(below ? fabs(fk) : 0.0);  // >= the column maximum
+    const bool mine = __builtin_amdgcn_inverse_ballot_w64(1ull << k);  // lane k keeps position k's data
+    apply_rest();  // (after the reduction above, which gives the rotated factors time to arrive)
+    int kn;
+    T nu1, nb1, nu2 = 0, nb2 = 0;  // this step's update
+    int nterms;
+    if (__builtin_expect(absakk >= kBkAlpha * cbound && dk != 0.0 && !(det && absakk <= tau), 1)) {
+      // ---- the common step: a 1x1 pivot in place (it passes the test against the exact column
+      //      maximum too), not small enough for the null-pivot test
+      const double y = rsqrt_nr(absakk);  // 1 / sqrt|d|
+      const double inv = dk < 0 ? -(y * y) : y * y;
+      my_dv = mine ? dk : my_dv;
+      my_dof = mine ? 0.0 : my_dof;
+      my_pt = mine ? (int)kPiv1 : my_pt;
+      my_code = mine ? (dk < 0 ? 1 : 16) : my_code;
+      F[lane + k * kBkLd] = below ? fk * inv : 0.0;  // multiplier of this row (zero outside the rows below)
+      nu1 = below ? fk * y : 0.0;
+      nb1 = dk < 0 ? nu1 : -nu1;
+      nterms = 1;
+      done |= 1ull << pk;
+      kn = k + 1;
+    } else {
+      // ---- the general step: the row search, interchanges, 2x2 pivots, null pivots
       const double colmax = wave_max_nonneg(below ? fabs(fk) : 0.0);
-      int kn;
-      if (__builtin_expect(absakk >= kBkAlpha * colmax && dk != 0.0 && !(det && absakk <= tau), 1)) {
-        // ---- the common step: a 1x1 pivot in place, not small enough for the null-pivot test
-        const double y = rsqrt_nr(absakk);  // 1 / sqrt|d|
-        const double inv = dk < 0 ? -(y * y) : y * y;
-        const bool mine = lane == k;
-        my_dv = mine ? dk : my_dv;
-        my_dof = mine ? 0.0 : my_dof;
-        my_pt = mine ? (int)kPiv1 : my_pt;
-        my_code = mine ? (dk < 0 ? 1 : 16) : my_code;
-        const T ci = below ? fk : 0.0;
-        F[lane + k * kBkLd] = ci * inv;  // multiplier of this row (zero outside the rows below)
-        u1 = ci * y;
-        b1 = dk < 0 ? u1 : -u1;
-        npend = 1;
-        kn = k + 1;
-      } else {
-        // ---- the general step: the row search, interchanges, 2x2 pivots, null pivots
-        int imax;
-        wave_argmax_nonneg(below ? fabs(fk) : 0.0, imax);
-        {
-          // ties: the lowest position among the lanes holding the maximum
-          const unsigned long long hold = __ballot(below && fabs(fk) == colmax);
-          if (colmax > 0.0 && __popcll(hold) > 1) {
-            int bp = 1 << 30;
-            for (unsigned long long h = hold; h; h &= h - 1) {
-              const int l = __ffsll((long long)h) - 1;
-              const int pl = __builtin_amdgcn_readlane(at, l);
-              if (pl < bp) {
-                bp = pl;
-                imax = l;
-              }
-            }
+      // the lane of the maximum below the pivot; ties: the lowest position among the lanes holding it
+      const unsigned long long hold = __ballot(below && fabs(fk) == colmax);
+      int imax = __ffsll((long long)hold) - 1;
+      if (colmax > 0.0 && __popcll(hold) > 1) {
+        int bp = 1 << 30;
+        for (unsigned long long h = hold; h; h &= h - 1) {
+          const int l = __ffsll((long long)h) - 1;
+          const int pl = __builtin_amdgcn_readlane(at, l);
+          if (pl < bp) {
+            bp = pl;
+            imax = l;
           }
         }
-        if (!(colmax > 0.0)) imax = -1;
-        int kind = 1;
-        bool swapped = false;
-        T ri = 0;  // column imax, when the row search reads it
-        if (!(absakk >= kBkAlpha * colmax) && imax >= 0) {
-          ri = pick(imax);
-          const double rowmax = wave_max_abs_masked(ri, at >= k && at < w && lane != imax);
-          if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
-            kind = 1;
-          } else if (fabs(readlane_of(ri, imax)) >= kBkAlpha * rowmax) {
-            kind = 1;
-            swapped = true;
-          } else {
-            kind = 2;
-            swapped = __builtin_amdgcn_readlane(at, imax) != k + 1;
-          }
-        }
-        if (swapped) {
-          // positions kk and at(imax) exchange their lanes
-          const int kk = kind == 1 ? k : k + 1;
-          const int q = __builtin_amdgcn_readlane(at, imax);
-          const int lkk = __ffsll((long long)__ballot(at == kk)) - 1;
-          if (lane == lkk) at = q;
-          if (lane == imax) at = kk;
-        }
-        kn = k + kind;
-        if (kind == 1) {
-          const int piv = swapped ? imax : pk;
-          const T c = swapped ? ri : fk;
-          const double d = swapped ? readlane_of(ri, imax) : dk;
-          const bool low = at > k && at < w;  // rows below the pivot, after the interchange
-          bool null_piv = d == 0.0;
-          if (det && fabs(d) <= tau) {
-            const double mbl = swapped ? wave_max_abs_masked(c, low) : colmax;
-            if (mbl <= tau) {
-              double mb = 0.0;
-              for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs(P[(int64_t)(c0 + piv) * m + r2]));
-              mb = wave_max_abs(mb);
-              if (mb <= tau) null_piv = true;
-            }
-          }
-          const double y = null_piv ? 0.0 : rsqrt_nr(fabs(d));  // 1 / sqrt|d|
-          const double inv = d < 0 ? -(y * y) : y * y;
-          const bool mine = lane == k;
-          my_dv = mine ? d : my_dv;
-          my_dof = mine ? 0.0 : my_dof;
-          my_pt = mine ? (null_piv ? (int)kPivNull : (int)kPiv1) : my_pt;
-          my_code = mine ? (null_piv ? (4 | (d == 0.0 ? 64 : 0)) : (d < 0 ? 1 : 16)) : my_code;
-          const T ci = low ? c : 0.0;
-          F[lane + k * kBkLd] = ci * inv;
-          u1 = ci * y;
-          b1 = d < 0 ? u1 : -u1;
-          npend = 1;
+      }
+      if (!(colmax > 0.0)) imax = -1;
+      int kind = 1;
+      bool swapped = false;
+      T ri = 0;  // column imax, when the row search reads it
+      if (!(absakk >= kBkAlpha * colmax) && imax >= 0) {
+        ri = pick(imax);
+        const bool rs = __builtin_amdgcn_inverse_ballot_w64(valid & ~done & ~(1ull << imax));
+        const double rowmax = wave_max_nonneg(rs ? fabs(ri) : 0.0);
+        if (absakk * rowmax >= kBkAlpha * colmax * colmax) {
+          kind = 1;
+        } else if (fabs(readlane_of(ri, imax)) >= kBkAlpha * rowmax) {
+          kind = 1;
+          swapped = true;
         } else {
-          const int pv2 = imax;  // at position k + 1 now
-          const T c1 = fk, c2 = ri;
-          const double a = dk, b = readlane_of(c1, pv2), cc = readlane_of(c2, pv2);
-          const double dt = a * cc - b * b;
-          const double mean = 0.5 * (a + cc), rad = sqrt(0.25 * (a - cc) * (a - cc) + b * b);
-          const double e1 = mean - rad, e2 = mean + rad;
-          const bool low = at > k + 1 && at < w;
-          const bool maybe_null = det && (fabs(e1) <= tau || fabs(e2) <= tau);
-          bool coupled = false;
-          if (maybe_null) coupled = wave_max_abs(low ? fmax(fabs(c1), fabs(c2)) : 0.0) > tau;
-          int code = 0;
-          for (double e : {e1, e2}) {
-            const bool z = (det && fabs(e) <= tau && !coupled) || e == 0.0;
-            code += z ? 4 : (e < 0 ? 1 : 16);
-          }
-          const bool dead = (code & 12) != 0 || dt == 0.0;
-          const double rdt = dead ? 0.0 : rcp_nr(dt);
-          const double i11 = cc * rdt, i12 = -b * rdt, i22 = a * rdt;
-          pairs |= 1ull << k;
-          const bool mine = lane == k, next = lane == k + 1;
-          my_dv = mine ? a : (next ? cc : my_dv);
-          my_dof = mine ? b : (next ? 0.0 : my_dof);
-          my_pt = mine ? (dead ? (int)kPivNull : (int)kPiv2First) : (next ? (dead ? (int)kPivNull : (int)kPiv2Second) : my_pt);
-          my_code = mine ? (code | (dt == 0.0 ? 64 : 0)) : (next ? 0 : my_code);
-          const T w1 = low ? c1 : 0.0, w2 = low ? c2 : 0.0;
-          F[lane + k * kBkLd] = w1 * i11 + w2 * i12;
-          F[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
-          // D = J diag(l1, l2) J^T, J = [cs sn; -sn cs] (uniform values)
-          double cs = 1.0, sn = 0.0, l1 = a, l2 = cc;
-          if (b != 0.0) {
-            const double tq = (cc - a) * (0.5 * rcp_nr(b));
-            const double h = fma(tq, tq, 1.0);
-            const double t = (tq >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(tq) + h * rsqrt_nr(h));
-            cs = rsqrt_nr(fma(t, t, 1.0));
-            sn = t * cs;
-            l1 = a - t * b;
-            l2 = cc + t * b;
-          }
-          const bool upd = !dead && l1 != 0.0 && l2 != 0.0;
-          const double s1 = upd ? rsqrt_nr(fabs(l1)) : 0.0, s2 = upd ? rsqrt_nr(fabs(l2)) : 0.0;
-          u1 = (cs * w1 - sn * w2) * s1;
-          u2 = (sn * w1 + cs * w2) * s2;
-          b1 = l1 < 0.0 ? u1 : -u1;
-          b2 = l2 < 0.0 ? u2 : -u2;
-          npend = 2;
+          kind = 2;
+          swapped = __builtin_amdgcn_readlane(at, imax) != k + 1;
         }
       }
-      if (kn < w) {
-        // the next pivot column: the row's column of the lane at position kn, with this step's update
-        // formed by the same FMAs that the row gets at the start of the next step
-        pk = __ffsll((long long)__ballot(at == kn)) - 1;
-        T cn = pick(pk);
-        if (npend == 2) cn = fma(readlane_of(u2, pk), b2, cn);
-        col = fma(readlane_of(u1, pk), b1, cn);
-        rotate_rows(u1, lower, x1, y1, z1);
-        if (npend == 2) rotate_rows(u2, lower, x2, y2, z2);
+      if (swapped) {
+        // positions kk and q = at(imax) exchange their lanes lkk and imax
+        const int kk = kind == 1 ? k : k + 1;
+        const int q = __builtin_amdgcn_readlane(at, imax);
+        const int lkk = __builtin_amdgcn_readlane(lop, kk);
+        at = lane == lkk ? q : (lane == imax ? kk : at);
+        lop = lane == kk ? imax : (lane == q ? lkk : lop);
       }
-      k = kn;
+      kn = k + kind;
+      if (kind == 1) {
+        const int piv = swapped ? imax : pk;
+        const T c = swapped ? ri : fk;
+        const double d = swapped ? readlane_of(ri, imax) : dk;
+        done |= 1ull << piv;
+        const bool low = __builtin_amdgcn_inverse_ballot_w64(valid & ~done);  // rows below, after the interchange
+        bool null_piv = d == 0.0;
+        if (det && fabs(d) <= tau) {
+          const double mbl = swapped ? wave_max_nonneg(low ? fabs(c) : 0.0) : colmax;
+          if (mbl <= tau) {
+            double mb = 0.0;
+            for (int r2 = c0 + w + lane; r2 < m; r2 += 64) mb = fmax(mb, fabs(P[(int64_t)(c0 + piv) * m + r2]));
+            mb = wave_max_abs(mb);
+            if (mb <= tau) null_piv = true;
+          }
+        }
+        const double y = null_piv ? 0.0 : rsqrt_nr(fabs(d));  // 1 / sqrt|d|
+        const double inv = d < 0 ? -(y * y) : y * y;
+        my_dv = mine ? d : my_dv;
+        my_dof = mine ? 0.0 : my_dof;
+        my_pt = mine ? (null_piv ? (int)kPivNull : (int)kPiv1) : my_pt;
+        my_code = mine ? (null_piv ? (4 | (d == 0.0 ? 64 : 0)) : (d < 0 ? 1 : 16)) : my_code;
+        const T ci = low ? c : 0.0;
+        F[lane + k * kBkLd] = ci * inv;
+        nu1 = ci * y;
+        nb1 = d < 0 ? nu1 : -nu1;
+        nterms = 1;
+      } else {
+        const int pv2 = imax;  // at position k + 1 now
+        const T c1 = fk, c2 = ri;
+        const double a = dk, b = readlane_of(c1, pv2), cc = readlane_of(c2, pv2);
+        const double dt = a * cc - b * b;
+        const double mean = 0.5 * (a + cc), rad = sqrt(0.25 * (a - cc) * (a - cc) + b * b);
+        const double e1 = mean - rad, e2 = mean + rad;
+        done |= (1ull << pk) | (1ull << pv2);
+        const bool low = __builtin_amdgcn_inverse_ballot_w64(valid & ~done);
+        const bool maybe_null = det && (fabs(e1) <= tau || fabs(e2) <= tau);
+        bool coupled = false;
+        if (maybe_null) coupled = wave_max_abs(low ? fmax(fabs(c1), fabs(c2)) : 0.0) > tau;
+        int code = 0;
+        for (double e : {e1, e2}) {
+          const bool z = (det && fabs(e) <= tau && !coupled) || e == 0.0;
+          code += z ? 4 : (e < 0 ? 1 : 16);
+        }
+        const bool dead = (code & 12) != 0 || dt == 0.0;
+        const double rdt = dead ? 0.0 : rcp_nr(dt);
+        const double i11 = cc * rdt, i12 = -b * rdt, i22 = a * rdt;
+        pairs |= 1ull << k;
+        const bool next = __builtin_amdgcn_inverse_ballot_w64(1ull << (k + 1));
+        my_dv = mine ? a : (next ? cc : my_dv);
+        my_dof = mine ? b : (next ? 0.0 : my_dof);
+        my_pt = mine ? (dead ? (int)kPivNull : (int)kPiv2First) : (next ? (dead ? (int)kPivNull : (int)kPiv2Second) : my_pt);
+        my_code = mine ? (code | (dt == 0.0 ? 64 : 0)) : (next ? 0 : my_code);
+        const T w1 = low ? c1 : 0.0, w2 = low ? c2 : 0.0;
+        F[lane + k * kBkLd] = w1 * i11 + w2 * i12;
+        F[lane + (k + 1) * kBkLd] = w1 * i12 + w2 * i22;
+        // D = J diag(l1, l2) J^T, J = [cs sn; -sn cs] (uniform values; b = 0: no rotation, the
+        // selects keep the step free of branches)
+        const bool rot = b != 0.0;
+        const double tq = (cc - a) * (0.5 * rcp_nr(rot ? b : 1.0));
+        const double h = fma(tq, tq, 1.0);
+        const double t = rot ? (tq >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(tq) + h * rsqrt_nr(h)) : 0.0;
+        const double cs = rsqrt_nr(fma(t, t, 1.0)), sn = t * cs;
+        const double l1 = a - t * b, l2 = cc + t * b;
+        const bool upd = !dead && l1 != 0.0 && l2 != 0.0;
+        const double s1 = upd ? rsqrt_nr(fabs(l1)) : 0.0, s2 = upd ? rsqrt_nr(fabs(l2)) : 0.0;
+        nu1 = (cs * w1 - sn * w2) * s1;
+        nu2 = (sn * w1 + cs * w2) * s2;
+        nb1 = l1 < 0.0 ? nu1 : -nu1;
+        nb2 = l2 < 0.0 ? nu2 : -nu2;
+        nterms = 2;
+      }
     }
-    if (lane < w) sphys[at] = lane;
-    sdv[lane] = my_dv;
-    sdof[lane] = my_dof;
-    if (lane < w) {
-      N.dval[first + c0 + lane] = my_dv;
-      N.doff[first + c0 + lane] = my_dof;
-      N.ptype[first + c0 + lane] = (int8_t)my_pt;
+    npend = nterms;
+    u1 = nu1;
+    b1 = nb1;
+    u2 = nu2;
+    b2 = nb2;
+    if (kn < w) {
+      // the next pivot column: the row's column of the lane at position kn, with this step's update
+      // formed by the same FMAs that the row gets during the next step
+      pk = __builtin_amdgcn_readlane(lop, kn);
+      T cn = pick(pk);
+      if (npend == 2) cn = fma(readlane_of(u2, pk), b2, cn);
+      col = fma(readlane_of(u1, pk), b1, cn);
+      x1 = rotate_lanes(u1, lane, 1);
+      y1 = rotate_lanes(u1, lane, 2);
+      z1 = rotate_lanes(u1, lane, 3);
+      if (npend == 2) {
+        x2 = rotate_lanes(u2, lane, 1);
+        y2 = rotate_lanes(u2, lane, 2);
+        z2 = rotate_lanes(u2, lane, 3);
+      }
     }
-    if (lane == 0) spairs = pairs;
-    // tallies: the steps' counts summed over the positions (bit b of each 2-bit count by ballot)
-    const int code = lane < w ? my_code : 0;
-    auto count2 = [&](int sh) -> int {
-      return __popcll(__ballot((code >> sh) & 1)) + 2 * __popcll(__ballot((code >> (sh + 1)) & 1));
-    };
-    const int neg = count2(0), zero = count2(2), pos = count2(4);
-    const int exz = __popcll(__ballot((code >> 6) & 1));
-    const int two = __popcll(pairs);
-    if (lane == 0) {
-      unsigned long long* cnt = N.cnt;
-      if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
-      if (zero) atomicAdd(&cnt[kCntZero], (unsigned long long)zero);
-      if (pos) atomicAdd(&cnt[kCntPos], (unsigned long long)pos);
-      if (two) atomicAdd(&cnt[kCnt2x2], (unsigned long long)two);
-      if (exz) atomicAdd(&cnt[kCntExactZero], (unsigned long long)exz);
-    }
+    k = kn;
   }
-  __syncthreads();
-  // ---- write back L11 and D in logical order (column j by wave j mod kBkWaveLoaders), the permutation
-  const unsigned long long pairs = spairs;
-  const int ph = lane < w ? sphys[lane] : 0;
-  for (int j = wv; j < w; j += kBkWaveLoaders) {
-    if (lane >= j && lane < w) {
-      const T v = lane == j ? sdv[j] : ((lane == j + 1 && ((pairs >> j) & 1)) ? sdof[j] : F[ph + j * kBkLd]);
-      P[(int64_t)(c0 + j) * m + c0 + lane] = v;
-    }
+  if (lane < w) {
+    N.dval[first + c0 + lane] = my_dv;
+    N.doff[first + c0 + lane] = my_dof;
+    N.ptype[first + c0 + lane] = (int8_t)my_pt;
   }
-  if (wv == 0 && lane < w) {
+  sdof[lane] = my_dof;
+  // tallies: the steps' counts summed over the positions (bit b of each 2-bit count by ballot)
+  const int code = lane < w ? my_code : 0;
+  auto count2 = [&](int sh) -> int {
+    return __popcll(__ballot((code >> sh) & 1)) + 2 * __popcll(__ballot((code >> (sh + 1)) & 1));
+  };
+  const int neg = count2(0), zero = count2(2), pos = count2(4);
+  const int exz = __popcll(__ballot((code >> 6) & 1));
+  const int two = __popcll(pairs);
+  if (lane == 0) {
+    unsigned long long* cnt = N.cnt;
+    if (neg) atomicAdd(&cnt[kCntNeg], (unsigned long long)neg);
+    if (zero) atomicAdd(&cnt[kCntZero], (unsigned long long)zero);
+    if (pos) atomicAdd(&cnt[kCntPos], (unsigned long long)pos);
+    if (two) atomicAdd(&cnt[kCnt2x2], (unsigned long long)two);
+    if (exz) atomicAdd(&cnt[kCntExactZero], (unsigned long long)exz);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // ---- write back L11 and D in logical order, the permutation: the multipliers of row `lane`
+  //      (columns j < lane), then its diagonal entry and the coupling entry of a 2x2 pivot
+  const int ph = lop;  // the physical row at logical position `lane`
+  T* prow = P + (int64_t)c0 * m + c0 + lane;
+  const T* frow = F + ph;
+#pragma unroll 8
+  for (int j = 0; j < w; ++j)
+    if (lane > j && lane < w) prow[(int64_t)j * m] = frow[j * kBkLd];
+  if (lane < w) {
+    prow[(int64_t)lane * m] = my_dv;
+    if (lane > 0 && ((pairs >> (lane - 1)) & 1)) prow[(int64_t)(lane - 1) * m] = sdof[lane - 1];
     N.piv[first + c0 + lane] = c0 + ph;
     N.ipiv[first + c0 + ph] = c0 + lane;
     N.bperm[(int64_t)f * nb + lane] = ph;
