@@ -1,5 +1,8 @@
 """Builds libhsg.so in-tree: host C++ (ordering, symbolic analysis) with g++, the HIP kernels and
-the C ABI with hipcc for gfx950. Always recompiles (a fresh build is the build check).
+the C ABI with hipcc for gfx950; and libhsg_solve.so, the C entry of the whole solve
+(include/hsg_solve.h), with g++ against the Python headers (it does not link libpython: the
+interpreter's symbols come from the process that loads it). Always recompiles (a fresh build is
+the build check).
 
     python -m hsolve_amd.build
 """
@@ -16,6 +19,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libhsg.so"
+SOLVE_LIB = HERE / "libhsg_solve.so"
 ARCH = os.environ.get("HSG_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -54,9 +58,21 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])
     os.replace(tmp, LIB)
+    build_solve_lib()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {SOLVE_LIB}")
     return LIB
+
+
+def build_solve_lib() -> Path:
+    """libhsg_solve.so from csrc/upper/solve_capi.cpp."""
+    import sysconfig
+    cxx = shutil.which("g++") or "g++"
+    tmp = SOLVE_LIB.with_suffix(".so.tmp")
+    _run([cxx, *CXX_FLAGS, "-shared", f"-I{sysconfig.get_paths()['include']}", str(CSRC / "upper" / "solve_capi.cpp"),
+          "-o", str(tmp), "-ldl"])
+    os.replace(tmp, SOLVE_LIB)
+    return SOLVE_LIB
 
 
 if __name__ == "__main__":
