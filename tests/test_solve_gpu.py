@@ -56,6 +56,36 @@ def test_ghep_nearest_vs_dense(gpu):
     _check(res, A, B, {"mode": "nearest", "sigma": 12.0, "k": 10}, w)
 
 
+def _nondiagonal_b(A, B0, seed):
+    """An HPD B with off-diagonal entries on A's pattern: B0 (diagonal, positive) plus a symmetric
+    random part scaled so that every row stays strictly diagonally dominant (margin 1/2)."""
+    rng = np.random.default_rng(seed)
+    U = sp.triu(A, 1).tocsr()
+    U.data = rng.uniform(-1.0, 1.0, U.nnz)
+    E = (U + U.T).tocsr()
+    d = B0.diagonal()
+    scale = 0.5 * d.min() / np.abs(E).sum(axis=1).max()
+    return (sp.diags(d) + scale * E).tocsr()
+
+
+@pytest.mark.parametrize("mode", ["nearest", "interval"])
+def test_ghep_nondiagonal_b_vs_dense(gpu, mode):
+    # a general (non-diagonal) B: the B-SpMV, the B-inner products and the definiteness check of a
+    # non-diagonal B on the device, against the dense generalized eigenvalues
+    A, B0, g = yee.jittered_box(cells=(7, 5, 6), lengths=(2 * np.pi, np.pi, 1.5 * np.pi), seed=8)
+    B = _nondiagonal_b(A, B0, 21)
+    assert (B - sp.diags(B.diagonal())).nnz > 0
+    w = la.eigvalsh(A.toarray(), B.toarray())
+    if mode == "nearest":
+        res = _solve(A, B, sigma=12.0, k=10)
+        _check(res, A, B, {"mode": "nearest", "sigma": 12.0, "k": 10}, w)
+    else:
+        lo, hi = 8.0, 16.0
+        res = _solve(A, B, lower=lo, upper=hi)
+        _check(res, A, B, {"mode": "interval", "lower": lo, "upper": hi}, w)
+        assert len(res.eigenvalues) == np.count_nonzero((w > lo) & (w <= hi))
+
+
 def test_interval_vs_dense(gpu):
     A, B, g = yee.jittered_box(cells=(8, 5, 6), lengths=(2 * np.pi, np.pi, 1.5 * np.pi), seed=5)
     w = la.eigvalsh(A.toarray(), B.toarray())
