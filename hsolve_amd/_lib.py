@@ -61,6 +61,8 @@ SIGNATURES = {
     "hsg_column_dots": (C.c_int, [_I64, _P, _I64, _P, _I64, _I64, _P, _P, _P]),
     "hsg_axpby": (C.c_int, [_I64, _D, _P, _I64, _D, _P, _I64, _I64, _P]),
     "hsg_pair_sums": (C.c_int, [_I64, _P, _P, _P, _I64, _I64, _P, _P, _P]),
+    "hsg_block_orthonormalize": (C.c_int, [C.c_int32, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _P, _P, _P, _D, _P,
+                                           _P, _P, _P, C.POINTER(C.c_int32), _P]),
     "hsg_timing_enable": (C.c_int, [C.c_int]),
     "hsg_timing_read": (C.c_int, [C.c_int, _P, _P, _P, _P]),
 }
@@ -69,12 +71,13 @@ for _name in ("csr_spmm", "diag_spmm", "shifted_residual", "gram", "block_update
     SIGNATURES["hsg_z" + _name] = SIGNATURES["hsg_" + _name]
 
 # Entry points that run long on the host (analysis, the numeric factorization and its inertia, a
-# solve's level-by-level launches, factor copies) release the GIL while they run. Every other entry
+# solve's level-by-level launches, factor copies, a block orthonormalization with its two waits on
+# the device) release the GIL while they run. Every other entry
 # point only enqueues one or two kernels on a stream and returns within microseconds: those keep the
 # GIL. Concurrent spectrum slices drive the device from several Python threads, and a thread that
 # gave up the GIL for a short call could wait a full interpreter switch interval to get it back.
 RELEASES_GIL = frozenset({"hsg_analyze", "hsg_factorize", "hsg_factorize_batch", "hsg_factor_solve", "hsg_factor_clone", "hsg_release_factors",
-                          "hsg_factor_destroy", "hsg_release_cached_memory"})
+                          "hsg_factor_destroy", "hsg_release_cached_memory", "hsg_block_orthonormalize"})
 
 
 class Bound:

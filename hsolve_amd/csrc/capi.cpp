@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
+#include <complex>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -75,6 +78,216 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 void require(bool ok, const char* msg) {
   if (!ok) throw InvalidInput(msg);
+}
+
+}  // namespace
+
+namespace {
+
+// Host side of hsg_block_orthonormalize: Cholesky factors and triangular inverses of p x p
+// Hermitian matrices (p is a Krylov block width, at most a few dozen), real or complex.
+template <class S>
+double sq_abs(const S& x) {
+  return std::norm(std::complex<double>(x));
+}
+inline double conj_of(double x) { return x; }
+inline std::complex<double> conj_of(const std::complex<double>& x) { return std::conj(x); }
+inline bool finite_of(double x) { return std::isfinite(x); }
+inline bool finite_of(const std::complex<double>& x) { return std::isfinite(x.real()) && std::isfinite(x.imag()); }
+
+// Upper Cholesky factor of Hermitian G (row-major p x p): G = R^H R; false when G is not
+// numerically positive definite.
+template <class S>
+bool cholesky_upper(const std::vector<S>& G, int64_t p, std::vector<S>& R) {
+  R.assign((size_t)(p * p), S(0));
+  for (int64_t j = 0; j < p; ++j) {
+    double s = std::real(std::complex<double>(G[j * p + j]));
+    for (int64_t i = 0; i < j; ++i) s -= sq_abs(R[i * p + j]);
+    if (!(s > 0.0) || !std::isfinite(s)) return false;
+    const double d = std::sqrt(s);
+    R[j * p + j] = S(d);
+    for (int64_t c = j + 1; c < p; ++c) {
+      S t = G[j * p + c];
+      for (int64_t i = 0; i < j; ++i) t -= conj_of(R[i * p + j]) * R[i * p + c];
+      R[j * p + c] = t / d;
+    }
+  }
+  return true;
+}
+
+// Inverse of an upper-triangular R (row-major), by columns of back substitution.
+template <class S>
+std::vector<S> upper_inverse(const std::vector<S>& R, int64_t p) {
+  std::vector<S> X((size_t)(p * p), S(0));
+  for (int64_t j = 0; j < p; ++j) {
+    X[j * p + j] = S(1) / R[j * p + j];
+    for (int64_t i = j - 1; i >= 0; --i) {
+      S t = S(0);
+      for (int64_t l = i + 1; l <= j; ++l) t += R[i * p + l] * X[l * p + j];
+      X[i * p + j] = -t / R[i * p + i];
+    }
+  }
+  return X;
+}
+
+// A pinned host buffer per thread for the one device-to-host copy of a block step.
+struct PinnedBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need) {
+    if (need > bytes) {
+      if (ptr) (void)hipHostFree(ptr);
+      ptr = nullptr;
+      bytes = 0;
+      if (hipHostMalloc(&ptr, need, hipHostMallocDefault) != hipSuccess)
+        throw std::runtime_error("hsg_block_orthonormalize: hipHostMalloc failed");
+      bytes = need;
+    }
+    return ptr;
+  }
+  ~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+};
+thread_local PinnedBuffer g_pinned;
+// R^-1 staged for its host-to-device copy, and the event after that copy: the buffer is rewritten
+// only once the previous copy from it has run.
+thread_local PinnedBuffer g_pinned_rinv;
+struct CopyEvent {
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+  ~CopyEvent() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
+};
+thread_local CopyEvent g_rinv_copy;
+
+void hip_ok(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class S>
+int32_t block_orthonormalize(int64_t n, const double* v, int64_t ldv, int64_t k, double* w, int64_t ldw, int64_t p,
+                             const int64_t* b_rowptr, const int32_t* b_colind, const double* b_values,
+                             const double* b_diag, double breakdown_tol, double* work, double* norms_in,
+                             double* coef_out, double* r_out, hipStream_t st) {
+  constexpr bool cplx = std::is_same<S, std::complex<double>>::value;
+  constexpr int64_t sw = cplx ? 2 : 1;  // doubles per scalar
+  const bool general_b = b_rowptr != nullptr;
+  // device work: [norms p][C1 k p][C2 k p][G p p][Rinv p p][tmp n p][B w n p]
+  double* buf = work;
+  double* d_rinv = buf + sw * (p + 2 * k * p + p * p);
+  double* tmp = d_rinv + sw * p * p;
+  double* bw = tmp + sw * n * p;
+  StreamScratch& sc = scratch_for(st);
+  std::lock_guard<std::mutex> lock(sc.mu);
+  // B-weighted products of this engine's kernels: diagonal weight, or B W formed first
+  auto b_rhs = [&](const double* x) -> const double* {
+    if (!general_b) return x;
+    if (cplx) hsg::zcsr_spmm(n, b_rowptr, b_colind, b_values, x, ldw, bw, n, p, 1.0, 0.0, st);
+    else hsg::csr_spmm(n, b_rowptr, b_colind, b_values, x, ldw, bw, n, p, 1.0, 0.0, st);
+    return bw;
+  };
+  const double* wd = general_b ? nullptr : b_diag;
+  const int64_t ldr = general_b ? n : ldw;
+  auto gram = [&](const double* vv, int64_t ldvv, int64_t kk, const double* rhs, double* c) {
+    if (cplx) hsg::zgram(sc.arena, n, vv, ldvv, kk, wd, rhs, ldr, p, c, kk, st);
+    else hsg::gram(sc.arena, n, vv, ldvv, kk, wd, rhs, ldr, p, c, kk, st);
+  };
+  auto update = [&](const double* vv, int64_t ldvv, int64_t kk, const double* c, double* dst, int64_t ldd,
+                    double alpha, double beta) {
+    if (cplx) hsg::zblock_update(n, vv, ldvv, kk, c, kk, dst, ldd, p, alpha, beta, st);
+    else hsg::block_update(n, vv, ldvv, kk, c, kk, dst, ldd, p, alpha, beta, st);
+  };
+  // the entry norms, two CGS passes against V, the block's Gram matrix
+  {
+    const double* rhs = b_rhs(w);
+    if (cplx) hsg::zcolumn_dots(sc.arena, n, w, ldw, rhs, ldr, p, wd, buf, st);
+    else hsg::column_dots(sc.arena, n, w, ldw, rhs, ldr, p, wd, buf, st);
+  }
+  for (int i = 0; i < (k ? 2 : 0); ++i) {
+    double* c = buf + sw * (p + i * k * p);
+    gram(v, ldv, k, b_rhs(w), c);
+    update(v, ldv, k, c, w, ldw, -1.0, 1.0);
+  }
+  double* g_dev = buf + sw * (p + 2 * k * p);
+  gram(w, ldw, p, b_rhs(w), g_dev);
+  const size_t head = (size_t)sw * (size_t)(p + 2 * k * p + p * p) * sizeof(double);
+  double* h = static_cast<double*>(g_pinned.get(head));
+  hip_ok(hipMemcpyAsync(h, buf, head, hipMemcpyDeviceToHost, st), "hsg_block_orthonormalize: copy");
+  hip_ok(hipStreamSynchronize(st), "hsg_block_orthonormalize: synchronize");
+  const S* hs = reinterpret_cast<const S*>(h);
+  for (int64_t j = 0; j < p; ++j) norms_in[j] = std::sqrt(std::max(std::real(std::complex<double>(hs[j])), 0.0));
+  // coef (k x p, row-major) = C1 + C2, with C stored column-major k x p
+  S* coef = reinterpret_cast<S*>(coef_out);
+  for (int64_t i = 0; i < k; ++i)
+    for (int64_t j = 0; j < p; ++j) {
+      S t = S(0);
+      for (int pass = 0; pass < 2; ++pass) t += hs[p + pass * k * p + j * k + i];
+      coef[i * p + j] = t;
+    }
+  // G (row-major) = the Gram matrix made exactly Hermitian
+  auto gram_host = [&](const S* g) {
+    std::vector<S> G((size_t)(p * p));
+    for (int64_t i = 0; i < p; ++i)
+      for (int64_t j = 0; j < p; ++j) G[i * p + j] = 0.5 * (g[j * p + i] + conj_of(g[i * p + j]));
+    return G;
+  };
+  std::vector<S> G = gram_host(hs + p + 2 * k * p);
+  for (int64_t j = 0; j < p; ++j) {
+    const double gjj = std::real(std::complex<double>(G[j * p + j]));
+    const double floor = breakdown_tol * norms_in[j];
+    if (!(gjj > floor * floor)) return 1;
+  }
+  for (const S& x : G)
+    if (!finite_of(x)) return 1;
+  std::vector<S> R((size_t)(p * p), S(0));
+  for (int64_t j = 0; j < p; ++j) R[j * p + j] = S(1);
+  S* hrinv = static_cast<S*>(g_pinned_rinv.get(sizeof(S) * p * p));
+  for (int it = 0; it < 2; ++it) {
+    std::vector<S> Rk;
+    if (!cholesky_upper(G, p, Rk)) return it + 1;
+    double dmin = INFINITY, dmax = 0.0;
+    for (int64_t j = 0; j < p; ++j) {
+      const double d = std::abs(std::complex<double>(Rk[j * p + j]));
+      dmin = std::min(dmin, d);
+      dmax = std::max(dmax, d);
+    }
+    if (dmin <= 1e-7 * dmax) return it + 1;  // too ill-conditioned for Cholesky-QR
+    const std::vector<S> X = upper_inverse(Rk, p);
+    // W <- W X: the update takes C column-major with C[i + j p] = X[i][j]
+    if (g_rinv_copy.pending) {
+      hip_ok(hipEventSynchronize(g_rinv_copy.ev), "hsg_block_orthonormalize: event");
+      g_rinv_copy.pending = false;
+    }
+    for (int64_t i = 0; i < p; ++i)
+      for (int64_t j = 0; j < p; ++j) hrinv[j * p + i] = X[i * p + j];
+    hip_ok(hipMemcpyAsync(d_rinv, hrinv, sizeof(S) * p * p, hipMemcpyHostToDevice, st),
+           "hsg_block_orthonormalize: copy");
+    if (!g_rinv_copy.ev) hip_ok(hipEventCreateWithFlags(&g_rinv_copy.ev, hipEventDisableTiming), "hsg_block_orthonormalize: event");
+    hip_ok(hipEventRecord(g_rinv_copy.ev, st), "hsg_block_orthonormalize: event");
+    g_rinv_copy.pending = true;
+    update(w, ldw, p, d_rinv, tmp, n, 1.0, 0.0);
+    hip_ok(hipMemcpy2DAsync(w, sizeof(S) * ldw, tmp, sizeof(S) * n, sizeof(S) * n, p, hipMemcpyDeviceToDevice, st),
+           "hsg_block_orthonormalize: copy");
+    std::vector<S> RR((size_t)(p * p), S(0));
+    for (int64_t i = 0; i < p; ++i)
+      for (int64_t j = i; j < p; ++j) {
+        S t = S(0);
+        for (int64_t l = i; l <= j; ++l) t += Rk[i * p + l] * R[l * p + j];
+        RR[i * p + j] = t;
+      }
+    R.swap(RR);
+    if (it == 0) {
+      gram(w, ldw, p, b_rhs(w), g_dev);
+      hip_ok(hipMemcpyAsync(h, g_dev, sizeof(S) * p * p, hipMemcpyDeviceToHost, st),
+             "hsg_block_orthonormalize: copy");
+      hip_ok(hipStreamSynchronize(st), "hsg_block_orthonormalize: synchronize");
+      G = gram_host(reinterpret_cast<const S*>(h));
+    }
+  }
+  std::memcpy(r_out, R.data(), sizeof(S) * p * p);
+  return 0;
 }
 
 }  // namespace
@@ -412,6 +625,23 @@ int hsg_zcolumn_dots(int64_t n, const double* x, int64_t ldx, const double* y, i
     StreamScratch& sc = scratch_for(as_stream(stream));
     std::lock_guard<std::mutex> lock(sc.mu);
     hsg::zcolumn_dots(sc.arena, n, x, ldx, y, ldy, p, w_diag, out, as_stream(stream));
+  });
+}
+
+int hsg_block_orthonormalize(int32_t is_complex, int64_t n, const double* v, int64_t ldv, int64_t k, double* w,
+                             int64_t ldw, int64_t p, const int64_t* b_rowptr, const int32_t* b_colind,
+                             const double* b_values, const double* b_diag, double breakdown_tol, double* work,
+                             double* norms_in, double* coef, double* r, int32_t* status, void* stream) {
+  return guarded([&] {
+    require(n >= 0 && k >= 0 && p > 0 && ldw >= n && (k == 0 || ldv >= n) && work && norms_in && r && status &&
+                (k == 0 || coef),
+            "hsg_block_orthonormalize: invalid arguments");
+    require(!b_rowptr || (b_colind && b_values), "hsg_block_orthonormalize: incomplete CSR B");
+    *status = is_complex ? block_orthonormalize<std::complex<double>>(n, v, ldv, k, w, ldw, p, b_rowptr, b_colind,
+                                                                      b_values, b_diag, breakdown_tol, work, norms_in,
+                                                                      coef, r, as_stream(stream))
+                         : block_orthonormalize<double>(n, v, ldv, k, w, ldw, p, b_rowptr, b_colind, b_values, b_diag,
+                                                        breakdown_tol, work, norms_in, coef, r, as_stream(stream));
   });
 }
 

@@ -15,6 +15,7 @@ block corrections, only the active columns are solved.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass
 
@@ -138,6 +139,32 @@ class DeviceOps:
         _lib.call(self._k + "block_update", self.n, ptr(v), self.n, k, ptr(c), k, ptr(w), self.n, p, float(alpha),
                   float(beta), stream_ptr())
         return w
+
+    def block_orthonormalize(self, v: torch.Tensor | None, w: torch.Tensor, breakdown_tol: float):
+        """Block CGS2 of w (p, n) against v[:k] in the B inner product, then two Cholesky-QR passes,
+        in one library call that leaves the interpreter free while it runs (hsg_block_orthonormalize).
+        Returns (status, norms_in, coef k x p, R p x p): status 0 when w is orthonormalized; 1 or 2
+        when the Cholesky-QR declined before or after its first pass (w then holds the block as
+        it was at that point, coef and norms_in are final, R is not)."""
+        k = 0 if v is None else v.shape[0]
+        p = w.shape[0]
+        sw = 2 if self.is_complex else 1
+        # work comes from torch's stream-ordered caching allocator on the current stream, so the
+        # last update still queued there when the call returns reads it before anyone reuses it
+        general = not self.pencil.b_is_diagonal
+        need = sw * (p + 2 * k * p + 2 * p * p) + sw * self.n * p * (2 if general else 1)
+        work = torch.empty(need, dtype=torch.float64, device=self.dev)
+        norms = np.zeros(p, dtype=np.float64)
+        coef = np.zeros((k, p), dtype=self.np_dtype)
+        R = np.zeros((p, p), dtype=self.np_dtype)
+        status = C.c_int32(0)
+        d = self.d
+        brow, bcol, bval = (d["b_rowptr"], d["b_col"], d["b_val"]) if general else (None, None, None)
+        bdiag = None if general else self.b_weight()
+        _lib.call("hsg_block_orthonormalize", int(self.is_complex), self.n, ptr(v) if k else None, self.n, k, ptr(w),
+                  self.n, p, ptr(brow), ptr(bcol), ptr(bval), ptr(bdiag), float(breakdown_tol), ptr(work),
+                  norms.ctypes.data, coef.ctypes.data if k else None, R.ctypes.data, C.byref(status), stream_ptr())
+        return int(status.value), norms, coef, R
 
     def dots(self, x: torch.Tensor, y: torch.Tensor, weight=None, out: torch.Tensor | None = None) -> torch.Tensor:
         """x_c^H diag(weight) y_c per row pair; real parts (the values the driver uses: norms and the

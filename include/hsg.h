@@ -228,6 +228,24 @@ int hsg_block_update(int64_t n, const double* v, int64_t ldv, int64_t k, const d
 int hsg_column_dots(int64_t n, const double* x, int64_t ldx, const double* y, int64_t ldy, int64_t p,
                     const double* w_diag, double* out, void* stream);
 
+/* One block step of the Krylov basis in one call: block CGS2 of W (n x p) against V (n x k) in the
+ * B inner product, W -= V (V^H B W) twice, then two passes of Cholesky-QR within the block,
+ * W <- W R_1^-1, W <- W R_2^-1 (the counterpart of BlockOrthonormalizer<T>::run,
+ * src/eigen/block_ortho.cpp:164-232, with this engine's order of passes). B is CSR (b_rowptr !=
+ * NULL), diagonal (b_diag) or the identity (both NULL); is_complex: interleaved complex blocks.
+ * work: device, at least s (p + 2 k p + 2 p^2) + s n p (s n p more for a CSR B) doubles, s = 2 for
+ * complex blocks. Host outputs: norms_in (p doubles, the B-norms W had on entry), coef (k x p
+ * row-major scalars, the sum of both passes' V^H B W), r (p x p row-major, R = R_2 R_1). status: 0
+ * when W is orthonormal; 1 when the block is rank deficient to breakdown_tol or the first
+ * Cholesky-QR pass declined (a Gram matrix not numerically positive definite, or diag(R)
+ * spreading more than 1e7), W then holding the CGS2 result; 2 when the second pass declined, W
+ * holding W R_1^-1. Waits on the stream twice (the host factors the p x p Gram matrices); the last
+ * update of W is left queued on the stream. work must stay allocated until the stream has run it. */
+int hsg_block_orthonormalize(int32_t is_complex, int64_t n, const double* v, int64_t ldv, int64_t k, double* w,
+                             int64_t ldw, int64_t p, const int64_t* b_rowptr, const int32_t* b_colind,
+                             const double* b_values, const double* b_diag, double breakdown_tol, double* work,
+                             double* norms_in, double* coef, double* r, int32_t* status, void* stream);
+
 /* Residual sums of eigenpair candidates (pair_metrics, src/driver/projection.cpp:53-70): for each of
  * the p columns, out[3c] = ||ax_c - lambda_c bx_c||^2, out[3c+1] = ||bx_c||^2, out[3c+2] = ||x_c||^2,
  * all blocks n x p with leading dimension ld; lambda and out are device arrays. Deterministic. */
