@@ -216,8 +216,61 @@ class BlockLanczos:
         coefficient blocks and the Gram matrix, the host Cholesky factors, a second Gram matrix),
         during which the other slice threads keep the interpreter; when the Cholesky-QR declines,
         the column-by-column fallback below finishes the block from where the call left it."""
-        status, norms_in, coef, R = self.ops.block_orthonormalize(self.V[:k] if k else None, w, breakdown_tol)
-        if status == 0:
+        native = getattr(self.ops, "block_orthonormalize", None)
+        if native is not None:
+            status, norms_in, coef, R = native(self.V[:k] if k else None, w, breakdown_tol)
+            if status == 0:
+                return coef, R
+            return self._column_fallback(k, w, coef, norms_in, breakdown_tol)
+        return self._orthonormalize_steps(k, w, breakdown_tol)
+
+    def _orthonormalize_steps(self, k: int, w: torch.Tensor, breakdown_tol: float):
+        """The same block step through the ops' own products (host ops without the one-call form):
+        the entry norms, both CGS2 coefficient blocks and the first Gram matrix land in one buffer
+        fetched with one copy, so a block step waits twice (before each Cholesky factor)."""
+        ops = self.ops
+        p = w.shape[0]
+        kp = k * p
+        buf = torch.empty(p + 2 * kp + p * p, dtype=ops.dtype, device=ops.dev)
+        ops.b_dots(w, w, out=buf[:p])
+        if k:
+            vk = self.V[:k]
+            for i in range(2):
+                c = buf[p + i * kp:p + (i + 1) * kp].view(p, k)
+                ops.b_gram(vk, w, out=c)
+                ops.update(vk, c, w, alpha=-1.0, beta=1.0)
+        ops.b_gram(w, w, out=buf[p + 2 * kp:].view(p, p))
+        h = buf.cpu().numpy()
+        norms_in = np.sqrt(np.maximum(h[:p].real, 0.0))
+        coef = np.zeros((k, p), dtype=self.dt)
+        for i in range(2 if k else 0):
+            coef += h[p + i * kp:p + (i + 1) * kp].reshape(p, k).T
+        G = h[p + 2 * kp:].reshape(p, p).T
+        G = 0.5 * (G + G.conj().T)
+        ok = True
+        R = np.eye(p, dtype=self.dt)
+        if np.all(np.diag(G).real > (breakdown_tol * norms_in) ** 2) and np.all(np.isfinite(G)):
+            for it in range(2):
+                try:
+                    Rk = sla.cholesky(G, lower=False)
+                except np.linalg.LinAlgError:
+                    ok = False
+                    break
+                d = np.abs(np.diag(Rk))
+                if d.min() <= 1e-7 * d.max():  # too ill-conditioned for Cholesky-QR
+                    ok = False
+                    break
+                Rinv = np.linalg.inv(Rk)
+                tmp = ops.empty(p)
+                ops.update(w, ops.coef(Rinv), tmp, 1.0, 0.0)
+                w.copy_(tmp)
+                R = Rk @ R
+                if it == 0:
+                    G = ops.b_gram(w, w).cpu().numpy().T
+                    G = 0.5 * (G + G.conj().T)
+        else:
+            ok = False
+        if ok:
             return coef, R
         return self._column_fallback(k, w, coef, norms_in, breakdown_tol)
 
