@@ -49,6 +49,35 @@ def test_cavity_nearest_multiplicities(gpu):
     assert cert["count_inertia"] == yee_exact.count_below(ref, cert["upper"]) - yee_exact.count_below(ref, cert["lower"])
 
 
+def test_rescue_repairs_loosely_converged_pairs(gpu, monkeypatch):
+    """Forces the rescue step (definite.cpp:152-188, 370-405), which converged runs never reach:
+    the Krylov runs stop at a tolerance 10^3 times looser than the solve's, so some of the pairs
+    they return fail the backward-error check and verify_and_rescue has to repair them with
+    shift-invert corrections before the evaluator sees them (10^2 looser: every pair passes
+    without rescue; 10^4 looser: the cluster at the window's edge is not resolved and the solve
+    ends with its certificate failed, 16 of 18, which it reports)."""
+    looser = 1e3
+    from hsolve_amd import driver_core
+    real_options = driver_core.KrylovOptions
+    monkeypatch.setattr(driver_core, "KrylovOptions", lambda **kw: real_options(**dict(kw, tol=kw["tol"] * looser)))
+    calls = []
+    real_rescue = driver_core.rescue_pairs
+
+    def counting_rescue(ctx, sel, cols, *rest):
+        r = real_rescue(ctx, sel, cols, *rest)
+        calls.append((len(cols), r))
+        return r
+
+    monkeypatch.setattr(driver_core, "rescue_pairs", counting_rescue)
+    A, _, g = yee.cube_cavity(8)
+    ref = yee_exact.pec_box_spectrum(g.cells, g.lengths)
+    sigma, k = 9.0, 16
+    res = _solve(A, None, sigma=sigma, k=k)
+    print("rescue calls (failing columns, rescued)", calls, flush=True)
+    assert calls and sum(c[0] for c in calls) > 0
+    _check(res, A, None, {"mode": "nearest", "sigma": sigma, "k": k}, ref)
+
+
 def test_ghep_nearest_vs_dense(gpu):
     A, B, g = yee.jittered_box(cells=(8, 5, 6), lengths=(2 * np.pi, np.pi, 1.5 * np.pi), seed=4)
     w = la.eigvalsh(A.toarray(), B.toarray())
